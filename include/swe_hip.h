@@ -1,0 +1,150 @@
+/*
+ * swe_hip.h -- C ABI of the MI355X SWE block library (libswe_hip.so).
+ *
+ * This is the drop-in boundary for SWE-PPM's dimensional-splitting step.  The
+ * reference has no FFI: its blocks are C++ subclasses of SWE_Block
+ * (src/blocks/SWE_Block.hh:110-251) whose step is driven through
+ *     setGhostLayer()            e.g. SWE_DimensionalSplittingMpi.cpp:188-376
+ *     computeNumericalFluxes()   SWE_Block.hh:195 (pure virtual)
+ *     getMaxTimestep()           SWE_Block.hh:594-597
+ *     updateUnknowns(float dt)   SWE_Block.hh:197 (pure virtual)
+ * The C++ subclass SWE_DimensionalSplittingHip (include/swe/) implements
+ * those methods on top of the entry points below; anything else (Python
+ * ctypes, other languages) binds these symbols directly.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative SWE_HIP_E* code;
+ *    swe_hip_last_error() returns a description of the last failure on the
+ *    calling thread.  The reference methods return void and assert
+ *    (SWE_Block.hh:785), so the C++ subclass turns codes into exceptions.
+ *  - Host arrays use the reference's Float2D layout: (nx+2) x (ny+2) floats,
+ *    a[x][y] = raw[x*(ny+2) + y], ghost cells at 0 and n+1
+ *    (src/tools/Float2D.hh:74-80).
+ *  - A handle is used by one host thread at a time; calls on one handle are
+ *    ordered on the handle's HIP stream.  Different handles may be driven
+ *    from different threads (the overdecomposed drivers do this,
+ *    swe_mpi_overdecomp.cpp:269-323).
+ *  - Enumerations match the reference: BoundaryType and Boundary
+ *    (src/types/Boundary.hh:5-28), solver ids of WAVE_PROPAGATION_SOLVER
+ *    (SWE_DimensionalSplittingMpi.hh:42-50) plus 3 = Hybrid.
+ */
+#ifndef SWE_HIP_H
+#define SWE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_HIP_OK 0
+#define SWE_HIP_EINVAL (-1)  /* bad argument (size, enum, null pointer)        */
+#define SWE_HIP_EDEVICE (-2) /* HIP runtime error (allocation, launch, copy)   */
+#define SWE_HIP_ESTATE (-3)  /* call not valid in the handle's current state   */
+#define SWE_HIP_ETIMEOUT (-4) /* a bounded device-side wait gave up             */
+
+/* solvers */
+#define SWE_SOLVER_HLLE 0
+#define SWE_SOLVER_FWAVE 1
+#define SWE_SOLVER_AUGRIE 2
+#define SWE_SOLVER_HYBRID 3
+
+/* step flavours: U = the step of every parallel block of the reference
+ * (SWE_DimensionalSplittingMpi.cpp:383-518), S = swe_simple's split sweep
+ * (SWE_DimensionalSplitting.cpp:105-210) */
+#define SWE_FLAVOUR_U 0
+#define SWE_FLAVOUR_S 1
+
+typedef struct swe_hip_block swe_hip_block;
+
+/* Thread-local description of the last error ("" if none). */
+const char* swe_hip_last_error(void);
+
+/* Library version string. */
+const char* swe_hip_version(void);
+
+/* Create a block of nx x ny interior cells with mesh sizes dx, dy and origin
+ * (ox, oy).  Replaces the constructors
+ *   SWE_DimensionalSplitting(nx, ny, dx, dy, ox, oy)        SWE_DimensionalSplitting.hh:44-45
+ *   SWE_DimensionalSplittingMpi(..., bool localTimestepping) SWE_DimensionalSplittingMpi.hh:55-56
+ * Device memory for the state (h, hu, hv, b), the net-update sums and the
+ * control block is allocated here.  Boundaries start PASSIVE
+ * (SWE_Block.hh:297-300). */
+int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int solver, int flavour,
+                   swe_hip_block** out);
+int swe_hip_destroy(swe_hip_block* blk);
+
+/* Copy the state from / to host Float2D arrays (b may be NULL on download).
+ * Replaces the host-memory accessors getWaterHeight()/getMomentum*()/
+ * getBathymetry() (SWE_Block.hh:600-637): the device owns the state, host
+ * mirrors are refreshed on demand (the lazy synchronisation SWE_Block.hh:57-82
+ * describes). */
+int swe_hip_upload(swe_hip_block* blk, const float* h, const float* hu, const float* hv, const float* b);
+int swe_hip_download(swe_hip_block* blk, float* h, float* hu, float* hv, float* b);
+
+/* setBoundaryType (SWE_Block.hh:640-642). */
+int swe_hip_set_boundary(swe_hip_block* blk, int side, int type);
+
+/* applyBoundaryConditions (SWE_Block.hh:757-911): the body of setGhostLayer
+ * for OUTFLOW / WALL sides and the corners. */
+int swe_hip_apply_bc(swe_hip_block* blk);
+
+/* applyBoundaryBathymetry (SWE_Block.hh:724-749). */
+int swe_hip_apply_bc_bathymetry(swe_hip_block* blk);
+
+/* computeNumericalFluxes (flavour U: SWE_DimensionalSplittingMpi.cpp:383-456;
+ * flavour S: SWE_DimensionalSplitting.cpp:105-179).  Solves every edge,
+ * keeps the net updates on the device and returns the block's CFL time step
+ * (the value the reference stores in maxTimestep) in *max_timestep. */
+int swe_hip_fluxes(swe_hip_block* blk, float* max_timestep);
+
+/* updateUnknowns (flavour U: SWE_DimensionalSplittingMpi.cpp:484-518, dry
+ * clamp included; flavour S: SWE_DimensionalSplitting.cpp:187-210) with time
+ * step dt (the reference's U blocks use their maxTimestep member, which the
+ * driver may have lowered to the global minimum). */
+int swe_hip_update(swe_hip_block* blk, float dt);
+
+/* nsteps of the single-block global-time-stepping loop of swe_simple.cpp:
+ * 215-252 (setGhostLayer, computeNumericalFluxes, getMaxTimestep,
+ * updateUnknowns, t += dt) without leaving the device: dt is reduced and
+ * applied on the GPU.  *t_inout is the simulated time (advanced in float as
+ * the driver does); dt_log (may be NULL) receives nsteps step sizes. */
+int swe_hip_run(swe_hip_block* blk, int nsteps, float* t_inout, float* dt_log);
+
+/* swe_hip_run with HIP-event timing on the block's stream.  *total_ms is
+ * the elapsed time of the whole loop.  If kernel_ms is not NULL, events are
+ * also recorded between launches and kernel_ms[0..2] receive the summed
+ * durations of the boundary, first-pass (flux / max) and second-pass
+ * (update / apply) launches; kernel_launches[0..2] their launch counts.
+ * No reference counterpart: measurement hook for the bench. */
+int swe_hip_run_timed(swe_hip_block* blk, int nsteps, float* t_inout, float* total_ms, float* kernel_ms,
+                      int* kernel_launches);
+
+/* Wait for all work queued on the block's stream. */
+int swe_hip_synchronize(swe_hip_block* blk);
+
+/* Device pointers of the current state (for zero-copy interop) and the pitch
+ * (floats between consecutive x columns; equal to ny+2). */
+int swe_hip_device_pointers(swe_hip_block* blk, float** h, float** hu, float** hv, float** b, int* pitch);
+
+/* Batch form of the per-edge solver interface
+ *   solver::X<float>::computeNetUpdates(hL, hR, huL, huR, bL, bR,
+ *                                       hNetL, hNetR, huNetL, huNetR, maxEdgeSpeed)
+ * (call sites SWE_DimensionalSplittingMpi.cpp:403-410, 432-439; the solver
+ * bodies live in the un-vendored TUM-I5/swe_solvers submodule).  Evaluates n
+ * independent edges on the GPU; all arrays are host arrays of n floats. */
+int swe_hip_solve_edges(int solver, int64_t n, const float* hL, const float* hR, const float* huL, const float* huR,
+                        const float* bL, const float* bR, float* hNetL, float* hNetR, float* huNetL, float* huNetR,
+                        float* maxSpeed);
+
+/* Tuning: strategy 0 = recompute (max pass + update pass, 28+16 B/cell),
+ * 1 = store (flux pass writing 4 net-update sums + apply pass); xseg = x
+ * columns per wavefront strip (8, 16 or 32); -1 keeps the current value. */
+int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
+int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE_HIP_H */

@@ -1,0 +1,465 @@
+// oracle_solvers.hpp -- CPU ORACLE (test infrastructure only).
+//
+// This file is the checker, never the product: only tests/, the smoke check
+// and the bench's cpu_baseline leg may load what is built from it.
+//
+// Per-edge Riemann solvers of the SWE dimensional-splitting step, restated on
+// the CPU, templated on the scalar type so that the same code gives the FP32
+// oracle and the FP64 reference run.
+//
+// The reference includes these solvers from the git submodule
+// TUM-I5/swe_solvers (reference .gitmodules:1-3), which is not vendored:
+// submodules/swe_solvers/ is empty and its pinned commit is unknown.  The call
+// interface is taken from the reference's call sites
+//   solver.computeNetUpdates(hL, hR, huL, huR, bL, bR,
+//                            hNetL, hNetR, huNetL, huNetR, maxEdgeSpeed)
+// (src/blocks/SWE_DimensionalSplittingMpi.cpp:403-410, 432-439;
+//  src/blocks/SWE_DimensionalSplitting.cpp:122-130), and the arithmetic is
+// restated from the published methods:
+//   * f-wave:  Bale, LeVeque, Mitran, Rossmanith, SIAM J. Sci. Comput. 24
+//     (2002) 955-978, with Einfeldt (HLLE) speeds and the bathymetry source
+//     folded into the flux jump, dry neighbours treated as reflecting walls;
+//   * augmented Riemann solver:  D. L. George, J. Comput. Phys. 227 (2008)
+//     3089-3113 ("Augmented Riemann solvers for the shallow water equations
+//     over variable topography with steady states and inundation"), with the
+//     wet/dry wall and inundation treatment of that paper and a Newton
+//     middle-state computation for the shock cases.
+//
+// Stated constants (the un-vendored submodule's own values cannot be read):
+//   g = 9.81, dryTol = 0.01, zeroTol = 1e-7, newtonTol = 1e-6 (relative),
+//   maxNewton = 10, criticalTol = 1e-6, hybridTol = 1.0 (Hybrid dispatch:
+//   f-wave when both depths >= hybridTol, augmented solver otherwise).
+//
+// Parity status of each function is stated in DESIGN.md and checked by the
+// oracle tests: conservation of the wave decomposition, lake at rest, mirror
+// symmetry, exact Riemann middle states (FP64 bisection), closed forms.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+
+namespace oracle {
+
+enum SolverKind { SOLVER_HLLE = 0, SOLVER_FWAVE = 1, SOLVER_AUGRIE = 2, SOLVER_HYBRID = 3 };
+
+template <typename T>
+struct SolverConsts {
+    static constexpr T g = T(9.81);
+    static constexpr T dryTol = T(0.01);
+    static constexpr T zeroTol = T(1e-7);
+    static constexpr T newtonTol = T(1e-6);
+    static constexpr int maxNewton = 10;
+    static constexpr T criticalTol = T(1e-6);
+    static constexpr T hybridTol = T(1.0);
+};
+
+// Add one f-wave (its h and hu components) to the side its speed points to;
+// a wave of (near) zero speed is shared equally.
+template <typename T>
+inline void accumulate_wave(T speed, T zh, T zhu, T& hNetL, T& hNetR, T& huNetL, T& huNetR) {
+    const T zt = SolverConsts<T>::zeroTol;
+    if (speed < -zt) {
+        hNetL += zh;
+        huNetL += zhu;
+    } else if (speed > zt) {
+        hNetR += zh;
+        huNetR += zhu;
+    } else {
+        hNetL += T(0.5) * zh;
+        huNetL += T(0.5) * zhu;
+        hNetR += T(0.5) * zh;
+        huNetR += T(0.5) * zhu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// f-wave solver (Bale et al. 2002) with Einfeldt speeds.
+// ---------------------------------------------------------------------------
+template <typename T>
+void fwave(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
+           T& maxSpeed) {
+    typedef SolverConsts<T> C;
+    hNetL = hNetR = huNetL = huNetR = T(0);
+    maxSpeed = T(0);
+    const bool dryL = hL < C::dryTol;
+    const bool dryR = hR < C::dryTol;
+    if (dryL && dryR) return;
+    // a dry neighbour acts as a reflecting wall: mirror the wet state
+    if (dryL) {
+        hL = hR;
+        huL = -huR;
+        bL = bR;
+    } else if (dryR) {
+        hR = hL;
+        huR = -huL;
+        bR = bL;
+    }
+    const T uL = huL / hL;
+    const T uR = huR / hR;
+    const T sqhL = std::sqrt(hL);
+    const T sqhR = std::sqrt(hR);
+    const T cL = std::sqrt(C::g * hL);
+    const T cR = std::sqrt(C::g * hR);
+    // Roe averages
+    const T hRoe = T(0.5) * (hL + hR);
+    const T uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
+    const T cRoe = std::sqrt(C::g * hRoe);
+    // Einfeldt speeds
+    const T s1 = std::min(uL - cL, uRoe - cRoe);
+    const T s2 = std::max(uR + cR, uRoe + cRoe);
+    // flux jump with the bathymetry source folded in
+    const T df1 = huR - huL;
+    const T fR = huR * uR + T(0.5) * C::g * hR * hR;
+    const T fL = huL * uL + T(0.5) * C::g * hL * hL;
+    const T df2 = (fR - fL) + T(0.5) * C::g * (hL + hR) * (bR - bL);
+    // eigen-coefficients of R = [[1,1],[s1,s2]]
+    const T ds = s2 - s1;
+    const T beta1 = (s2 * df1 - df2) / ds;
+    const T beta2 = (df2 - s1 * df1) / ds;
+    accumulate_wave(s1, beta1, beta1 * s1, hNetL, hNetR, huNetL, huNetR);
+    accumulate_wave(s2, beta2, beta2 * s2, hNetL, hNetR, huNetL, huNetR);
+    // the dry side receives nothing
+    if (dryL) {
+        hNetL = T(0);
+        huNetL = T(0);
+    } else if (dryR) {
+        hNetR = T(0);
+        huNetR = T(0);
+    }
+    maxSpeed = std::max(std::fabs(s1), std::fabs(s2));
+}
+
+// ---------------------------------------------------------------------------
+// Middle state of the Riemann problem (George 2008, sec. 3; Toro's exact
+// solver for the shallow water equations).  Returns the middle depth hm and
+// the speeds s1m (1-wave tail/shock at the middle state) and s2m.
+// kind: 0 dry side, 1 two rarefactions, 2 two shocks, 3 shock+rarefaction.
+// ---------------------------------------------------------------------------
+template <typename T>
+int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nullptr) {
+    typedef SolverConsts<T> C;
+    const T hMin = std::min(hL, hR);
+    const T hMax = std::max(hL, hR);
+    const T du = uR - uL;
+    const T cL = std::sqrt(C::g * hL);
+    const T cR = std::sqrt(C::g * hR);
+    int it = 0;
+    int kind;
+    if (hMin < C::dryTol) {
+        hm = T(0);
+        s1m = uR + uL - T(2) * cR + T(2) * cL;
+        s2m = s1m;
+        kind = 0;
+    } else {
+        const T cMin = std::sqrt(C::g * hMin);
+        const T cMax = std::sqrt(C::g * hMax);
+        const T fMin = du + T(2) * (cMin - cMax);
+        const T fMax = du + (hMax - hMin) * std::sqrt(T(0.5) * C::g * (hMax + hMin) / (hMax * hMin));
+        if (fMin > T(0)) {
+            // two rarefactions: closed form
+            const T a = std::max(T(0), -du + T(2) * (cL + cR));
+            hm = a * a / (T(16) * C::g);
+            const T cm = std::sqrt(C::g * hm);
+            s1m = uL + T(2) * cL - T(3) * cm;
+            s2m = uR - T(2) * cR + T(3) * cm;
+            kind = 1;
+        } else if (fMax <= T(0)) {
+            // two shocks: Newton on phi(h) = du + (h-hL) gL(h) + (h-hR) gR(h), from h = hMax.
+            // phi is increasing and concave for h >= hMax, so the iterates increase monotonically.
+            T h = hMax;
+            for (it = 0; it < C::maxNewton; ++it) {
+                const T gL = std::sqrt(T(0.5) * C::g * (T(1) / h + T(1) / hL));
+                const T gR = std::sqrt(T(0.5) * C::g * (T(1) / h + T(1) / hR));
+                const T phi = du + (h - hL) * gL + (h - hR) * gR;
+                const T dphi = gL - C::g * (h - hL) / (T(4) * h * h * gL) + gR -
+                               C::g * (h - hR) / (T(4) * h * h * gR);
+                const T step = phi / dphi;
+                h = h - step;
+                if (std::fabs(step) <= C::newtonTol * h) {
+                    ++it;
+                    break;
+                }
+            }
+            hm = h;
+            const T gmL = std::sqrt(T(0.5) * C::g * (T(1) / hm + T(1) / hL));
+            const T gmR = std::sqrt(T(0.5) * C::g * (T(1) / hm + T(1) / hR));
+            const T u1m = uL - (hm - hL) * gmL;
+            const T u2m = uR + (hm - hR) * gmR;
+            const T cm = std::sqrt(C::g * hm);
+            s1m = u1m - cm;
+            s2m = u2m + cm;
+            kind = 2;
+        } else {
+            // one shock, one rarefaction: Newton on
+            // phi(h) = du + 2 (sqrt(g h) - sqrt(g hMax)) + (h - hMin) gMin(h), from h = hMin.
+            T h = hMin;
+            for (it = 0; it < C::maxNewton; ++it) {
+                const T gm = std::sqrt(T(0.5) * C::g * (T(1) / h + T(1) / hMin));
+                const T ch = std::sqrt(C::g * h);
+                const T phi = du + T(2) * (ch - cMax) + (h - hMin) * gm;
+                const T dphi = ch / h + gm - C::g * (h - hMin) / (T(4) * h * h * gm);
+                const T step = phi / dphi;
+                h = h - step;
+                if (std::fabs(step) <= C::newtonTol * h) {
+                    ++it;
+                    break;
+                }
+            }
+            hm = h;
+            const T cm = std::sqrt(C::g * hm);
+            if (hL > hR) {
+                s1m = uL + T(2) * cL - T(3) * cm;
+                s2m = uL + T(2) * cL - cm;
+            } else {
+                s1m = uR - T(2) * cR + cm;
+                s2m = uR - T(2) * cR + T(3) * cm;
+            }
+            kind = 3;
+        }
+    }
+    if (iters) *iters = it;
+    return kind;
+}
+
+// ---------------------------------------------------------------------------
+// Augmented Riemann solver (George 2008): three f-waves (two acoustic waves
+// at modified Einfeldt speeds plus a middle wave carrying the momentum-flux
+// deficit), the steady-state wave subtracted for well-balancing, wet/dry
+// walls and inundation.
+// ---------------------------------------------------------------------------
+template <typename T>
+void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
+            T& maxSpeed) {
+    typedef SolverConsts<T> C;
+    hNetL = hNetR = huNetL = huNetR = T(0);
+    maxSpeed = T(0);
+    if (hL < C::dryTol && hR < C::dryTol) return;
+
+    T uL, uR;
+    if (hL < C::dryTol) {
+        hL = T(0);
+        huL = T(0);
+        uL = T(0);
+    } else {
+        uL = huL / hL;
+    }
+    if (hR < C::dryTol) {
+        hR = T(0);
+        huR = T(0);
+        uR = T(0);
+    } else {
+        uR = huR / hR;
+    }
+
+    // wet/dry interface: wall (the dry cell's bathymetry is above anything the
+    // wet state can reach) or inundation (limit the step to the water surface)
+    bool wallL = false, wallR = false;
+    if (hR < C::dryTol) {
+        T hs, s1, s2;
+        middle_state(hL, hL, uL, -uL, hs, s1, s2);
+        const T hTest = std::max(hL, hs);
+        if (hTest + bL < bR) {
+            wallR = true;
+            hR = hL;
+            huR = -huL;
+            uR = -uL;
+            bR = bL;
+        } else if (hL + bL < bR) {
+            bR = hL + bL;
+        }
+    } else if (hL < C::dryTol) {
+        T hs, s1, s2;
+        middle_state(hR, hR, -uR, uR, hs, s1, s2);
+        const T hTest = std::max(hR, hs);
+        if (hTest + bR < bL) {
+            wallL = true;
+            hL = hR;
+            huL = -huR;
+            uL = -uR;
+            bL = bR;
+        } else if (hR + bR < bL) {
+            bL = hR + bR;
+        }
+    }
+
+    // Einfeldt speeds from the Roe average weighted by the celerities
+    const T cL = std::sqrt(C::g * hL);
+    const T cR = std::sqrt(C::g * hR);
+    const T uHat = (cL * uL + cR * uR) / (cL + cR);
+    const T cHat = std::sqrt(C::g * T(0.5) * (hR + hL));
+    const T sE1 = std::min(uL - cL, uHat - cHat);
+    const T sE2 = std::max(uR + cR, uHat + cHat);
+
+    T hm, s1m, s2m;
+    middle_state(hL, hR, uL, uR, hm, s1m, s2m);
+
+    // modified Einfeldt speeds; middle wave halfway between
+    const T l1 = std::min(sE1, s2m);
+    const T l3 = std::max(sE2, s1m);
+    const T l2 = T(0.5) * (l1 + l3);
+    const T dl = l3 - l1;
+
+    const T hStarHLL = std::max((huL - huR + l3 * hR - l1 * hL) / dl, T(0));
+
+    // steady-state wave (one pass)
+    const T dh = hR - hL;
+    const T dhu = huR - huL;
+    const T phiL = huL * uL + T(0.5) * C::g * hL * hL;
+    const T phiR = huR * uR + T(0.5) * C::g * hR * hR;
+    const T dphi = phiR - phiL;
+    const T db = bR - bL;
+
+    const T hBar = std::max(T(0.5) * (hL + hR), T(0));
+    const T uSum = uL + uR;
+    const T s1s2bar = T(0.25) * uSum * uSum - C::g * hBar;
+    const T s1s2tilde = std::max(T(0), uL * uR) - C::g * hBar;
+    const T ct = C::criticalTol;
+
+    bool sonic = false;
+    if (std::fabs(s1s2bar) <= ct) sonic = true;
+    if (s1s2bar * s1s2tilde <= ct) sonic = true;
+    if (s1s2bar * l1 * l3 <= ct) sonic = true;
+    if (std::min(std::fabs(l1), std::fabs(l3)) < ct) sonic = true;
+    if (l1 < T(0) && s1m > T(0)) sonic = true;
+    if (l3 > T(0) && s2m < T(0)) sonic = true;
+    if ((uL + cL) * (uR + cR) < T(0)) sonic = true;
+    if ((uL - cL) * (uR - cR) < T(0)) sonic = true;
+
+    T ddh = sonic ? -db : db * C::g * hBar / s1s2bar;
+    if (l1 < -ct && l3 > ct) {
+        ddh = std::min(ddh, hStarHLL * dl / l3);
+        ddh = std::max(ddh, hStarHLL * dl / l1);
+    } else if (l1 >= ct) {
+        ddh = std::min(ddh, hStarHLL * dl / l1);
+        ddh = std::max(ddh, -hL);
+    } else if (l3 <= -ct) {
+        ddh = std::min(ddh, hR);
+        ddh = std::max(ddh, hStarHLL * dl / l3);
+    }
+
+    T ddphi = sonic ? -C::g * hBar * db : -db * C::g * hBar * s1s2tilde / s1s2bar;
+    const T pL = -hL * db;
+    const T pR = -hR * db;
+    ddphi = std::min(ddphi, C::g * std::max(pL, pR));
+    ddphi = std::max(ddphi, C::g * std::min(pL, pR));
+
+    const T del1 = dh - ddh;
+    const T del2 = dhu;
+    const T del3 = dphi - ddphi;
+
+    // eigenvectors r1 = (1, l1, l1^2), r2 = (0, 0, 1), r3 = (1, l3, l3^2)
+    const T l1sq = l1 * l1;
+    const T l3sq = l3 * l3;
+    const T beta1 = (l3 * del1 - del2) / dl;
+    const T beta3 = (del2 - l1 * del1) / dl;
+    const T beta2 = del3 - (beta1 * l1sq + beta3 * l3sq);
+
+    T z1h = beta1 * l1, z1hu = beta1 * l1sq, s1 = l1;
+    T z2h = T(0), z2hu = beta2, s2 = l2;
+    T z3h = beta3 * l3, z3hu = beta3 * l3sq, s3 = l3;
+    if (wallR) {
+        z2h = z2hu = s2 = T(0);
+        z3h = z3hu = s3 = T(0);
+    }
+    if (wallL) {
+        z1h = z1hu = s1 = T(0);
+        z2h = z2hu = s2 = T(0);
+    }
+    accumulate_wave(s1, z1h, z1hu, hNetL, hNetR, huNetL, huNetR);
+    accumulate_wave(s2, z2h, z2hu, hNetL, hNetR, huNetL, huNetR);
+    accumulate_wave(s3, z3h, z3hu, hNetL, hNetR, huNetL, huNetR);
+    maxSpeed = std::max(std::fabs(s1), std::fabs(s3));
+}
+
+// ---------------------------------------------------------------------------
+// HLLE solver in f-wave form (Einfeldt 1988): two waves at the Einfeldt
+// speeds, the HLL middle state splitting the flux jump.  Same dry handling
+// as the f-wave solver.
+// ---------------------------------------------------------------------------
+template <typename T>
+void hlle(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
+          T& maxSpeed) {
+    typedef SolverConsts<T> C;
+    hNetL = hNetR = huNetL = huNetR = T(0);
+    maxSpeed = T(0);
+    const bool dryL = hL < C::dryTol;
+    const bool dryR = hR < C::dryTol;
+    if (dryL && dryR) return;
+    if (dryL) {
+        hL = hR;
+        huL = -huR;
+        bL = bR;
+    } else if (dryR) {
+        hR = hL;
+        huR = -huL;
+        bR = bL;
+    }
+    const T uL = huL / hL;
+    const T uR = huR / hR;
+    const T cL = std::sqrt(C::g * hL);
+    const T cR = std::sqrt(C::g * hR);
+    const T sqhL = std::sqrt(hL);
+    const T sqhR = std::sqrt(hR);
+    const T uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
+    const T cRoe = std::sqrt(C::g * T(0.5) * (hL + hR));
+    // HLLE speeds bounded by zero so that each side gets one wave
+    const T sL = std::min(std::min(uL - cL, uRoe - cRoe), T(0));
+    const T sR = std::max(std::max(uR + cR, uRoe + cRoe), T(0));
+    const T df1 = huR - huL;
+    const T fR = huR * uR + T(0.5) * C::g * hR * hR;
+    const T fL = huL * uL + T(0.5) * C::g * hL * hL;
+    const T df2 = (fR - fL) + T(0.5) * C::g * (hL + hR) * (bR - bL);
+    const T dq1 = hR - hL;
+    const T dq2 = huR - huL;
+    const T ds = sR - sL;
+    // HLL flux F* minus the left flux, in f-wave form:
+    //   A- dQ = sL (Q* - QL),  A+ dQ = sR (QR - Q*),  sum = dF
+    // with Q* chosen so that the sum is the (source-balanced) flux jump.
+    const T q1 = (sR * dq1 - df1) / ds;  // Q* - QL (h)
+    const T q2 = (sR * dq2 - df2) / ds;  // Q* - QL (hu)
+    hNetL = sL * q1;
+    huNetL = sL * q2;
+    hNetR = df1 - hNetL;
+    huNetR = df2 - huNetL;
+    if (dryL) {
+        hNetL = T(0);
+        huNetL = T(0);
+    } else if (dryR) {
+        hNetR = T(0);
+        huNetR = T(0);
+    }
+    maxSpeed = std::max(std::fabs(sL), std::fabs(sR));
+}
+
+// Hybrid: f-wave for edges that are well wet on both sides, augmented
+// solver otherwise (near-dry cells, wet/dry fronts).
+template <typename T>
+void hybrid(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
+            T& maxSpeed) {
+    typedef SolverConsts<T> C;
+    if (hL >= C::hybridTol && hR >= C::hybridTol)
+        fwave(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+    else
+        augrie(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+}
+
+template <typename T>
+inline void solve_edge(int solver, T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR,
+                       T& huNetL, T& huNetR, T& maxSpeed) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            hlle(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+            break;
+        case SOLVER_FWAVE:
+            fwave(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+            break;
+        case SOLVER_AUGRIE:
+            augrie(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+            break;
+        default:
+            hybrid(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+            break;
+    }
+}
+
+}  // namespace oracle

@@ -1,0 +1,188 @@
+// launch.hip -- solver-independent launches and the solver dispatch.
+#include <hip/hip_runtime.h>
+
+#include "kernels_s.hpp"
+#include "kernels_u.hpp"
+#include "swe_internal.h"
+
+namespace swe {
+
+// setGhostLayer for OUTFLOW/WALL sides plus the corners (SWE_Block.hh:757-911).
+__global__ void apply_bc_kernel(StateView v, BoundaryInfo bi) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = v.nx, ny = v.ny;
+    if (k < ny) {
+        const int j = k + 1;
+        for (int side = BND_LEFT; side <= BND_RIGHT; ++side) {
+            const int t = bi.type[side];
+            if (t != OUTFLOW && t != WALL) continue;
+            const int gx = side == BND_LEFT ? 0 : nx + 1;
+            const int sx = side == BND_LEFT ? 1 : nx;
+            const size_t g = idx(v, gx, j), s = idx(v, sx, j);
+            v.h[g] = v.h[s];
+            v.hu[g] = (t == WALL) ? -v.hu[s] : v.hu[s];
+            v.hv[g] = v.hv[s];
+        }
+    } else if (k < ny + nx) {
+        const int i = k - ny + 1;
+        for (int side = BND_BOTTOM; side <= BND_TOP; ++side) {
+            const int t = bi.type[side];
+            if (t != OUTFLOW && t != WALL) continue;
+            const int gy = side == BND_BOTTOM ? 0 : ny + 1;
+            const int sy = side == BND_BOTTOM ? 1 : ny;
+            const size_t g = idx(v, i, gy), s = idx(v, i, sy);
+            v.h[g] = v.h[s];
+            v.hu[g] = v.hu[s];
+            v.hv[g] = (t == WALL) ? -v.hv[s] : v.hv[s];
+        }
+    } else if (k < ny + nx + 4) {
+        const int c = k - ny - nx;
+        const int gx = (c < 2) ? 0 : nx + 1;
+        const int gy = (c & 1) ? ny + 1 : 0;
+        const int sx = (c < 2) ? 1 : nx;
+        const int sy = (c & 1) ? ny : 1;
+        const size_t g = idx(v, gx, gy), s = idx(v, sx, sy);
+        v.h[g] = v.h[s];
+        v.hu[g] = v.hu[s];
+        v.hv[g] = v.hv[s];
+    }
+}
+
+// applyBoundaryBathymetry (SWE_Block.hh:724-749): whole ghost lines incl.
+// corners for OUTFLOW/WALL sides, then the four corners.
+__global__ void apply_bc_bathymetry_kernel(float* b, int nx, int ny, int pitch, BoundaryInfo bi) {
+    // single thread block; sequential order matters only for the corners,
+    // which are written last
+    const int tid = threadIdx.x;
+    auto ow = [](int t) { return t == OUTFLOW || t == WALL; };
+    if (ow(bi.type[BND_LEFT]))
+        for (int j = tid; j < ny + 2; j += blockDim.x) b[j] = b[(size_t)pitch + j];
+    if (ow(bi.type[BND_RIGHT]))
+        for (int j = tid; j < ny + 2; j += blockDim.x) b[(size_t)(nx + 1) * pitch + j] = b[(size_t)nx * pitch + j];
+    __syncthreads();
+    if (ow(bi.type[BND_BOTTOM]))
+        for (int i = tid; i <= nx + 1; i += blockDim.x) b[(size_t)i * pitch] = b[(size_t)i * pitch + 1];
+    if (ow(bi.type[BND_TOP]))
+        for (int i = tid; i <= nx + 1; i += blockDim.x) b[(size_t)i * pitch + ny + 1] = b[(size_t)i * pitch + ny];
+    __syncthreads();
+    if (tid == 0) {
+        b[0] = b[(size_t)pitch + 1];
+        b[ny + 1] = b[(size_t)pitch + ny];
+        b[(size_t)(nx + 1) * pitch] = b[(size_t)nx * pitch + 1];
+        b[(size_t)(nx + 1) * pitch + ny + 1] = b[(size_t)nx * pitch + ny];
+    }
+}
+
+__global__ void ctl_reset_kernel(DeviceCtl* ctl, int par) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->smax[par] = 0u;
+}
+
+__global__ __launch_bounds__(256) void apply_s_kernel(StateView v, SumsView sums, float dt) {
+    const int y = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
+    const int x0 = 1 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * kSplitXSeg;
+    if (y > v.ny || x0 > v.nx) return;
+    const int x1 = min(x0 + kSplitXSeg - 1, v.nx);
+    const float dtdx = dt / v.dx;
+    for (int x = x0; x <= x1; ++x) {
+        const size_t c = idx(v, x, y);
+        const float hStar = v.h[c] - dtdx * sums.sx[c];
+        const float huStar = v.hu[c] - dtdx * sums.su[c];
+        v.h[c] = hStar - dtdx * sums.sy[c];
+        v.hu[c] = huStar;
+        v.hv[c] = v.hv[c] - dtdx * sums.sv[c];
+    }
+}
+
+
+int launch_sweep_u(int solver, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
+                   BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx,
+                   hipStream_t stream) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            return launch_sweep_u_solver<SOLVER_HLLE>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                      fuseBc, dtLog, logIdx, stream);
+        case SOLVER_FWAVE:
+            return launch_sweep_u_solver<SOLVER_FWAVE>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                       fuseBc, dtLog, logIdx, stream);
+        case SOLVER_AUGRIE:
+            return launch_sweep_u_solver<SOLVER_AUGRIE>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                        fuseBc, dtLog, logIdx, stream);
+        case SOLVER_HYBRID:
+            return launch_sweep_u_solver<SOLVER_HYBRID>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                        fuseBc, dtLog, logIdx, stream);
+        default:
+            set_error("unknown solver %d", solver);
+            return -1;
+    }
+}
+
+int launch_apply_u(int xseg, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+                   int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
+    constexpr int XS = 16;
+    (void)xseg;
+    const int nxs = (v.nx + XS - 1) / XS;
+    dim3 grid((v.ny + 63) / 64, (nxs + 3) / 4);
+    hipLaunchKernelGGL((apply_u_kernel<XS>), grid, dim3(256), 0, stream, v, sums, bi, ctl, par, dtFixed, fuseBc,
+                       dtLog, logIdx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "apply_u_kernel launch");
+    return 0;
+}
+
+static int dispatch_s(int solver, int dir, StateView v, SumsView sums, DeviceCtl* ctl, int par,
+                      hipStream_t stream) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            return launch_sweep_s_solver<SOLVER_HLLE>(dir, v, sums, ctl, par, stream);
+        case SOLVER_FWAVE:
+            return launch_sweep_s_solver<SOLVER_FWAVE>(dir, v, sums, ctl, par, stream);
+        case SOLVER_AUGRIE:
+            return launch_sweep_s_solver<SOLVER_AUGRIE>(dir, v, sums, ctl, par, stream);
+        case SOLVER_HYBRID:
+            return launch_sweep_s_solver<SOLVER_HYBRID>(dir, v, sums, ctl, par, stream);
+        default:
+            set_error("unknown solver %d", solver);
+            return -1;
+    }
+}
+
+int launch_sweep_s_x(int solver, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream) {
+    return dispatch_s(solver, 0, v, sums, ctl, par, stream);
+}
+
+int launch_sweep_s_y(int solver, StateView v, SumsView sums, hipStream_t stream) {
+    return dispatch_s(solver, 1, v, sums, nullptr, 0, stream);
+}
+
+int launch_apply_bc(StateView v, BoundaryInfo bi, hipStream_t stream) {
+    const int n = v.nx + v.ny + 4;
+    hipLaunchKernelGGL(apply_bc_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, v, bi);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "apply_bc_kernel launch");
+    return 0;
+}
+
+int launch_apply_bc_bathymetry(float* b, int nx, int ny, int pitch, BoundaryInfo bi, hipStream_t stream) {
+    hipLaunchKernelGGL(apply_bc_bathymetry_kernel, dim3(1), dim3(256), 0, stream, b, nx, ny, pitch, bi);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "apply_bc_bathymetry_kernel launch");
+    return 0;
+}
+
+int launch_ctl_reset(DeviceCtl* ctl, int par, hipStream_t stream) {
+    hipLaunchKernelGGL(ctl_reset_kernel, dim3(1), dim3(64), 0, stream, ctl, par);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "ctl_reset_kernel launch");
+    return 0;
+}
+
+int launch_apply_s(StateView v, SumsView sums, float dt, hipStream_t stream) {
+    const int nxs = (v.nx + kSplitXSeg - 1) / kSplitXSeg;
+    dim3 grid((v.ny + 63) / 64, (nxs + 3) / 4);
+    hipLaunchKernelGGL(apply_s_kernel, grid, dim3(256), 0, stream, v, sums, dt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "apply_s_kernel launch");
+    return 0;
+}
+
+}  // namespace swe

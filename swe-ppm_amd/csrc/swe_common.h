@@ -1,0 +1,38 @@
+// swe_common.h -- constants and small helpers shared by the HIP kernels and
+// the host side of the MI355X SWE block library.
+#pragma once
+
+#include <cstdint>
+
+namespace swe {
+
+// Solver ids follow the reference's WAVE_PROPAGATION_SOLVER switch
+// (src/blocks/SWE_DimensionalSplittingMpi.hh:42-50: 0 HLLE, 1 f-wave,
+// 2 augmented Riemann), with 3 = Hybrid (src/blocks/SWE_DimensionalSplitting.hh:39).
+enum Solver : int { SOLVER_HLLE = 0, SOLVER_FWAVE = 1, SOLVER_AUGRIE = 2, SOLVER_HYBRID = 3 };
+
+// Step flavours: U = every parallel block of the reference (all edges from
+// one state, unsplit update with dry clamp), S = swe_simple's split sweep.
+enum Flavour : int { FLAVOUR_U = 0, FLAVOUR_S = 1 };
+
+// src/types/Boundary.hh:5-28
+enum Boundary : int { BND_LEFT = 0, BND_RIGHT = 1, BND_BOTTOM = 2, BND_TOP = 3 };
+enum BoundaryType : int { OUTFLOW = 0, WALL = 1, INFLOW = 2, CONNECT = 3, CONNECT_WITHIN_RANK = 4, PASSIVE = 5 };
+enum GhostlayerState : int { GL_DONE = 0, GL_UNVALID = 1, GL_INTER = 2, GL_SYNC = 3, GL_NEXT = 4 };
+
+// Solver constants (the reference's solver submodule is not vendored; these
+// are the values this library states and its CPU oracle uses).
+constexpr float kGravity = 9.81f;
+constexpr float kDryTol = 0.01f;
+constexpr float kZeroTol = 1e-7f;
+constexpr float kNewtonTol = 1e-6f;
+constexpr int kMaxNewton = 10;
+constexpr float kCriticalTol = 1e-6f;
+constexpr float kHybridTol = 1.0f;
+
+// Block-level constants (src/Constants.hh:4-6, SWE_DimensionalSplittingMpi.cpp:448-456, 502-515).
+constexpr float kCfl = 0.4f;
+constexpr float kMinSpeed = 0.00001f;
+constexpr float kClampTol = 0.1f;
+
+}  // namespace swe

@@ -1,0 +1,513 @@
+// swe_hip.hip -- C ABI of the MI355X SWE block library: handle management,
+// host<->device transfer and the step entry points (include/swe_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/swe_hip.h"
+#include "kernels_s.hpp"
+#include "kernels_u.hpp"
+#include "swe_internal.h"
+
+using namespace swe;
+
+namespace swe {
+
+thread_local char g_last_error[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return SWE_HIP_EDEVICE;
+}
+
+}  // namespace swe
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) return hip_fail(_e, #expr); \
+    } while (0)
+
+struct swe_hip_block {
+    int nx = 0, ny = 0, pitch = 0;
+    float dx = 0, dy = 0, ox = 0, oy = 0;
+    int solver = SOLVER_AUGRIE;
+    int flavour = FLAVOUR_U;
+    int strategy = 1;  // 0 recompute, 1 store
+    int xseg = 16;
+    int cur = 0;       // index of the current state buffer
+    BoundaryInfo bi{{PASSIVE, PASSIVE, PASSIVE, PASSIVE}};
+    float* h[2] = {nullptr, nullptr};
+    float* hu[2] = {nullptr, nullptr};
+    float* hv[2] = {nullptr, nullptr};
+    float* b = nullptr;
+    float* sums[4] = {nullptr, nullptr, nullptr, nullptr};
+    DeviceCtl* ctl = nullptr;
+    float* dtlog = nullptr;
+    int dtlogCap = 0;
+    hipStream_t stream = nullptr;
+    float fluxDt = -1.f;  // dt of the last flavour-S flux pass (its Q* depends on it)
+};
+
+namespace {
+
+size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pitch; }
+
+StateView view(swe_hip_block* k, int buf) {
+    StateView v;
+    v.h = k->h[buf];
+    v.hu = k->hu[buf];
+    v.hv = k->hv[buf];
+    v.b = k->b;
+    v.nx = k->nx;
+    v.ny = k->ny;
+    v.pitch = k->pitch;
+    v.dx = k->dx;
+    v.dy = k->dy;
+    return v;
+}
+
+SumsView sums_view(swe_hip_block* k) { return SumsView{k->sums[0], k->sums[1], k->sums[2], k->sums[3]}; }
+
+int check(swe_hip_block* k) {
+    if (!k) {
+        set_error("null handle");
+        return SWE_HIP_EINVAL;
+    }
+    return 0;
+}
+
+int ensure_second_buffer(swe_hip_block* k) {
+    const int o = k->cur ^ 1;
+    if (k->h[o]) return 0;
+    const size_t bytes = field_elems(k) * sizeof(float);
+    HIP_TRY(hipMalloc(&k->h[o], bytes));
+    HIP_TRY(hipMalloc(&k->hu[o], bytes));
+    HIP_TRY(hipMalloc(&k->hv[o], bytes));
+    return 0;
+}
+
+int ensure_dtlog(swe_hip_block* k, int n) {
+    if (n <= k->dtlogCap) return 0;
+    if (k->dtlog) HIP_TRY(hipFree(k->dtlog));
+    k->dtlog = nullptr;
+    HIP_TRY(hipMalloc(&k->dtlog, sizeof(float) * n));
+    k->dtlogCap = n;
+    return 0;
+}
+
+// Optional HIP-event instrumentation of the device loop.  Event i is
+// recorded before launch i; cls[i] is the class of the work between event i
+// and i+1 (0 boundary, 1 first pass, 2 second pass).
+struct LoopTiming {
+    bool perKernel = false;
+    std::vector<hipEvent_t> events;
+    std::vector<int> cls;
+    int used = 0;
+    hipError_t mark(hipStream_t s, int nextClass) {
+        if (used >= (int)events.size()) return hipSuccess;
+        hipError_t e = hipEventRecord(events[used], s);
+        cls.push_back(nextClass);
+        ++used;
+        return e;
+    }
+};
+
+// Global-time-stepping loop of flavour U on the device: dt is reduced and
+// applied on the GPU, the next step's boundary conditions are fused into the
+// update (except after the last step, so ghosts end as in the reference).
+int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log, LoopTiming* tm) {
+    if (int r = ensure_dtlog(k, nsteps)) return r;
+    if (k->strategy == 0)
+        if (int r = ensure_second_buffer(k)) return r;
+    float t0 = t_inout ? *t_inout : 0.f;
+    HIP_TRY(hipMemcpyAsync(&k->ctl->t, &t0, sizeof(float), hipMemcpyHostToDevice, k->stream));
+    if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
+    if (tm) HIP_TRY(tm->mark(k->stream, 0));
+    if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
+    for (int s = 0; s < nsteps; ++s) {
+        const int par = s & 1;
+        const int fuse = (s + 1 < nsteps) ? 1 : 0;  // the next step's setGhostLayer, fused
+        if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
+        if (k->strategy == 0) {
+            const int o = k->cur ^ 1;
+            if (int r = launch_sweep_u(k->solver, MODE_MAX, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
+                                       nullptr, k->bi, k->ctl, par, -1.f, 0, nullptr, 0, k->stream))
+                return r;
+            if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
+            if (int r = launch_sweep_u(k->solver, MODE_UPDATE, k->xseg, view(k, k->cur), sums_view(k), k->h[o],
+                                       k->hu[o], k->hv[o], k->bi, k->ctl, par, -1.f, fuse, k->dtlog, s, k->stream))
+                return r;
+            k->cur = o;
+        } else {
+            if (int r = launch_sweep_u(k->solver, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr,
+                                       nullptr, nullptr, k->bi, k->ctl, par, -1.f, 0, nullptr, 0, k->stream))
+                return r;
+            if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
+            if (int r = launch_apply_u(k->xseg, view(k, k->cur), sums_view(k), k->bi, k->ctl, par, -1.f, fuse,
+                                       k->dtlog, s, k->stream))
+                return r;
+        }
+    }
+    if (tm) HIP_TRY(tm->mark(k->stream, 0));
+    if (dt_log) HIP_TRY(hipMemcpyAsync(dt_log, k->dtlog, sizeof(float) * nsteps, hipMemcpyDeviceToHost, k->stream));
+    float t1 = 0.f;
+    HIP_TRY(hipMemcpyAsync(&t1, &k->ctl->t, sizeof(float), hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    if (t_inout) *t_inout = t1;
+    return SWE_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* swe_hip_last_error(void) { return g_last_error; }
+
+const char* swe_hip_version(void) { return "swe_hip 0.1 gfx950"; }
+
+int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int solver, int flavour,
+                   swe_hip_block** out) {
+    if (!out || nx < 1 || ny < 1 || !(dx > 0.f) || !(dy > 0.f) || solver < 0 || solver > 3 ||
+        (flavour != FLAVOUR_U && flavour != FLAVOUR_S)) {
+        set_error("swe_hip_create: invalid argument (nx=%d ny=%d dx=%g dy=%g solver=%d flavour=%d)", nx, ny, dx,
+                  dy, solver, flavour);
+        return SWE_HIP_EINVAL;
+    }
+    if ((size_t)(nx + 2) * (size_t)(ny + 2) > ((size_t)1 << 31)) {
+        set_error("swe_hip_create: block too large");
+        return SWE_HIP_EINVAL;
+    }
+    swe_hip_block* k = new (std::nothrow) swe_hip_block();
+    if (!k) {
+        set_error("swe_hip_create: out of host memory");
+        return SWE_HIP_EINVAL;
+    }
+    k->nx = nx;
+    k->ny = ny;
+    k->pitch = ny + 2;
+    k->dx = dx;
+    k->dy = dy;
+    k->ox = ox;
+    k->oy = oy;
+    k->solver = solver;
+    k->flavour = flavour;
+    const size_t bytes = field_elems(k) * sizeof(float);
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&k->h[0], bytes);
+    if (e == hipSuccess) e = hipMalloc(&k->hu[0], bytes);
+    if (e == hipSuccess) e = hipMalloc(&k->hv[0], bytes);
+    if (e == hipSuccess) e = hipMalloc(&k->b, bytes);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipMalloc(&k->sums[i], bytes);
+    if (e == hipSuccess) e = hipMalloc(&k->ctl, sizeof(DeviceCtl));
+    if (e == hipSuccess) e = hipMemsetAsync(k->ctl, 0, sizeof(DeviceCtl), k->stream);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipMemsetAsync(k->sums[i], 0, bytes, k->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(k->h[0], 0, bytes, k->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(k->hu[0], 0, bytes, k->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(k->hv[0], 0, bytes, k->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(k->b, 0, bytes, k->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
+    if (e != hipSuccess) {
+        hip_fail(e, "swe_hip_create");
+        swe_hip_destroy(k);
+        return SWE_HIP_EDEVICE;
+    }
+    *out = k;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_destroy(swe_hip_block* k) {
+    if (!k) return SWE_HIP_OK;
+    if (k->stream) (void)hipStreamSynchronize(k->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (k->h[i]) (void)hipFree(k->h[i]);
+        if (k->hu[i]) (void)hipFree(k->hu[i]);
+        if (k->hv[i]) (void)hipFree(k->hv[i]);
+    }
+    if (k->b) (void)hipFree(k->b);
+    for (int i = 0; i < 4; ++i)
+        if (k->sums[i]) (void)hipFree(k->sums[i]);
+    if (k->ctl) (void)hipFree(k->ctl);
+    if (k->dtlog) (void)hipFree(k->dtlog);
+    if (k->stream) (void)hipStreamDestroy(k->stream);
+    delete k;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_upload(swe_hip_block* k, const float* h, const float* hu, const float* hv, const float* b) {
+    if (int r = check(k)) return r;
+    if (!h || !hu || !hv || !b) {
+        set_error("swe_hip_upload: null array");
+        return SWE_HIP_EINVAL;
+    }
+    const size_t bytes = field_elems(k) * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(k->h[k->cur], h, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->hu[k->cur], hu, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->hv[k->cur], hv, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->b, b, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_download(swe_hip_block* k, float* h, float* hu, float* hv, float* b) {
+    if (int r = check(k)) return r;
+    const size_t bytes = field_elems(k) * sizeof(float);
+    if (h) HIP_TRY(hipMemcpyAsync(h, k->h[k->cur], bytes, hipMemcpyDeviceToHost, k->stream));
+    if (hu) HIP_TRY(hipMemcpyAsync(hu, k->hu[k->cur], bytes, hipMemcpyDeviceToHost, k->stream));
+    if (hv) HIP_TRY(hipMemcpyAsync(hv, k->hv[k->cur], bytes, hipMemcpyDeviceToHost, k->stream));
+    if (b) HIP_TRY(hipMemcpyAsync(b, k->b, bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_set_boundary(swe_hip_block* k, int side, int type) {
+    if (int r = check(k)) return r;
+    if (side < BND_LEFT || side > BND_TOP || type < OUTFLOW || type > PASSIVE) {
+        set_error("swe_hip_set_boundary: invalid side %d or type %d", side, type);
+        return SWE_HIP_EINVAL;
+    }
+    if (type == INFLOW) {
+        // the reference asserts on INFLOW in applyBoundaryConditions (SWE_Block.hh:784-786)
+        set_error("swe_hip_set_boundary: INFLOW boundaries are not supported (the reference asserts on them)");
+        return SWE_HIP_EINVAL;
+    }
+    k->bi.type[side] = type;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_apply_bc(swe_hip_block* k) {
+    if (int r = check(k)) return r;
+    return launch_apply_bc(view(k, k->cur), k->bi, k->stream);
+}
+
+int swe_hip_apply_bc_bathymetry(swe_hip_block* k) {
+    if (int r = check(k)) return r;
+    if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_fluxes(swe_hip_block* k, float* max_timestep) {
+    if (int r = check(k)) return r;
+    if (!max_timestep) {
+        set_error("swe_hip_fluxes: null output");
+        return SWE_HIP_EINVAL;
+    }
+    unsigned int bits = 0;
+    if (k->flavour == FLAVOUR_U) {
+        if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
+        if (int r = launch_sweep_u(k->solver, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
+                                   nullptr, k->bi, k->ctl, 0, -1.f, 0, nullptr, 0, k->stream))
+            return r;
+        HIP_TRY(hipMemcpyAsync(&bits, &k->ctl->smax[0], sizeof(bits), hipMemcpyDeviceToHost, k->stream));
+        HIP_TRY(hipStreamSynchronize(k->stream));
+        float s;
+        std::memcpy(&s, &bits, sizeof(s));
+        *max_timestep = host_timestep_u(s, k->dx, k->dy);
+    } else {
+        if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
+        if (int r = launch_sweep_s_x(k->solver, view(k, k->cur), sums_view(k), k->ctl, 0, k->stream)) return r;
+        HIP_TRY(hipMemcpyAsync(&bits, &k->ctl->smax[0], sizeof(bits), hipMemcpyDeviceToHost, k->stream));
+        HIP_TRY(hipStreamSynchronize(k->stream));
+        float s;
+        std::memcpy(&s, &bits, sizeof(s));
+        const float dt = (float).4 * (k->dx / s);  // SWE_DimensionalSplitting.cpp:221, no zero guard
+        if (int r = launch_sweep_s_y(k->solver, view(k, k->cur), sums_view(k), k->stream)) return r;
+        k->fluxDt = dt;
+        *max_timestep = dt;
+    }
+    return SWE_HIP_OK;
+}
+
+int swe_hip_update(swe_hip_block* k, float dt) {
+    if (int r = check(k)) return r;
+    if (k->flavour == FLAVOUR_U) {
+        if (!(dt >= 0.f)) {
+            set_error("swe_hip_update: dt must be >= 0");
+            return SWE_HIP_EINVAL;
+        }
+        return launch_apply_u(k->xseg, view(k, k->cur), sums_view(k), k->bi, k->ctl, 0, dt, 0, nullptr, 0,
+                              k->stream);
+    }
+    // flavour S: Q* was formed with the flux pass's own dt (the reference
+    // asserts that dt equals it, SWE_DimensionalSplitting.cpp:278)
+    if (k->fluxDt < 0.f) {
+        set_error("swe_hip_update: flavour S needs swe_hip_fluxes first");
+        return SWE_HIP_ESTATE;
+    }
+    return launch_apply_s(view(k, k->cur), sums_view(k), k->fluxDt, k->stream);
+}
+
+int swe_hip_run(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log) {
+    if (int r = check(k)) return r;
+    if (nsteps < 0) {
+        set_error("swe_hip_run: nsteps < 0");
+        return SWE_HIP_EINVAL;
+    }
+    if (nsteps == 0) return SWE_HIP_OK;
+    if (k->flavour != FLAVOUR_U) {
+        // flavour S keeps its host-visible dt (swe_simple reads it every step)
+        float t = t_inout ? *t_inout : 0.f;
+        for (int s = 0; s < nsteps; ++s) {
+            float dt;
+            if (int r = swe_hip_apply_bc(k)) return r;
+            if (int r = swe_hip_fluxes(k, &dt)) return r;
+            if (int r = swe_hip_update(k, dt)) return r;
+            if (dt_log) dt_log[s] = dt;
+            t += dt;
+        }
+        HIP_TRY(hipStreamSynchronize(k->stream));
+        if (t_inout) *t_inout = t;
+        return SWE_HIP_OK;
+    }
+    return run_u_device_loop(k, nsteps, t_inout, dt_log, nullptr);
+}
+
+int swe_hip_run_timed(swe_hip_block* k, int nsteps, float* t_inout, float* total_ms, float* kernel_ms,
+                      int* kernel_launches) {
+    if (int r = check(k)) return r;
+    if (nsteps < 1 || k->flavour != FLAVOUR_U) {
+        set_error("swe_hip_run_timed: needs nsteps >= 1 and flavour U");
+        return SWE_HIP_EINVAL;
+    }
+    LoopTiming tm;
+    tm.perKernel = kernel_ms != nullptr;
+    const int nev = tm.perKernel ? 2 * nsteps + 3 : 2;
+    tm.events.resize(nev);
+    for (auto& e : tm.events) HIP_TRY(hipEventCreate(&e));
+    int rc = run_u_device_loop(k, nsteps, t_inout, nullptr, &tm);
+    if (rc == 0) {
+        float ms = 0.f;
+        hipError_t e = hipEventElapsedTime(&ms, tm.events[0], tm.events[tm.used - 1]);
+        if (e != hipSuccess) rc = hip_fail(e, "hipEventElapsedTime");
+        if (total_ms) *total_ms = ms;
+        if (rc == 0 && tm.perKernel) {
+            float acc[3] = {0.f, 0.f, 0.f};
+            int cnt[3] = {0, 0, 0};
+            for (int i = 0; i + 1 < tm.used; ++i) {
+                float d = 0.f;
+                e = hipEventElapsedTime(&d, tm.events[i], tm.events[i + 1]);
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "hipEventElapsedTime");
+                    break;
+                }
+                acc[tm.cls[i]] += d;
+                cnt[tm.cls[i]] += 1;
+            }
+            for (int c = 0; c < 3; ++c) {
+                kernel_ms[c] = acc[c];
+                if (kernel_launches) kernel_launches[c] = cnt[c];
+            }
+        }
+    }
+    for (auto& e : tm.events) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int swe_hip_synchronize(swe_hip_block* k) {
+    if (int r = check(k)) return r;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_device_pointers(swe_hip_block* k, float** h, float** hu, float** hv, float** b, int* pitch) {
+    if (int r = check(k)) return r;
+    if (h) *h = k->h[k->cur];
+    if (hu) *hu = k->hu[k->cur];
+    if (hv) *hv = k->hv[k->cur];
+    if (b) *b = k->b;
+    if (pitch) *pitch = k->pitch;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_solve_edges(int solver, int64_t n, const float* hL, const float* hR, const float* huL,
+                        const float* huR, const float* bL, const float* bR, float* hNetL, float* hNetR,
+                        float* huNetL, float* huNetR, float* maxSpeed) {
+    if (n < 0 || n > ((int64_t)1 << 28) || solver < 0 || solver > 3) {
+        set_error("swe_hip_solve_edges: invalid n=%lld or solver=%d", (long long)n, solver);
+        return SWE_HIP_EINVAL;
+    }
+    if (n == 0) return SWE_HIP_OK;
+    const float* ins[6] = {hL, hR, huL, huR, bL, bR};
+    float* outs[5] = {hNetL, hNetR, huNetL, huNetR, maxSpeed};
+    for (auto p : ins)
+        if (!p) {
+            set_error("swe_hip_solve_edges: null input");
+            return SWE_HIP_EINVAL;
+        }
+    for (auto p : outs)
+        if (!p) {
+            set_error("swe_hip_solve_edges: null output");
+            return SWE_HIP_EINVAL;
+        }
+    float* din = nullptr;
+    float* dout = nullptr;
+    const size_t nb = sizeof(float) * (size_t)n;
+    HIP_TRY(hipMalloc(&din, 6 * nb));
+    hipError_t e = hipMalloc(&dout, 5 * nb);
+    if (e != hipSuccess) {
+        (void)hipFree(din);
+        return hip_fail(e, "hipMalloc");
+    }
+    int rc = 0;
+    for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMemcpy(din + i * n, ins[i], nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        switch (solver) {
+            case SOLVER_HLLE:
+                rc = launch_solve_edges_solver<SOLVER_HLLE>(n, din, dout, nullptr);
+                break;
+            case SOLVER_FWAVE:
+                rc = launch_solve_edges_solver<SOLVER_FWAVE>(n, din, dout, nullptr);
+                break;
+            case SOLVER_AUGRIE:
+                rc = launch_solve_edges_solver<SOLVER_AUGRIE>(n, din, dout, nullptr);
+                break;
+            default:
+                rc = launch_solve_edges_solver<SOLVER_HYBRID>(n, din, dout, nullptr);
+                break;
+        }
+    }
+    if (e == hipSuccess && rc == 0) e = hipDeviceSynchronize();
+    for (int i = 0; i < 5 && e == hipSuccess && rc == 0; ++i)
+        e = hipMemcpy(outs[i], dout + i * n, nb, hipMemcpyDeviceToHost);
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return hip_fail(e, "swe_hip_solve_edges");
+    return rc;
+}
+
+int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
+    if (int r = check(k)) return r;
+    if (strategy != -1 && strategy != 0 && strategy != 1) {
+        set_error("swe_hip_set_tuning: strategy must be 0 or 1");
+        return SWE_HIP_EINVAL;
+    }
+    if (xseg != -1 && xseg != 8 && xseg != 16 && xseg != 32) {
+        set_error("swe_hip_set_tuning: xseg must be 8, 16 or 32");
+        return SWE_HIP_EINVAL;
+    }
+    if (strategy != -1) k->strategy = strategy;
+    if (xseg != -1) k->xseg = xseg;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_get_tuning(swe_hip_block* k, int* strategy, int* xseg) {
+    if (int r = check(k)) return r;
+    if (strategy) *strategy = k->strategy;
+    if (xseg) *xseg = k->xseg;
+    return SWE_HIP_OK;
+}
+
+}  // extern "C"

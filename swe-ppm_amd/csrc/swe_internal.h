@@ -1,0 +1,53 @@
+// swe_internal.h -- declarations shared between the translation units of
+// libswe_hip.so (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "kernels_u.hpp"
+
+namespace swe {
+
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+// Host copy of the flavour-U dt rule (SWE_DimensionalSplittingMpi.cpp:448-456);
+// same float operations as the device version.
+inline float host_timestep_u(float s, float dx, float dy) {
+    if (s > kMinSpeed) {
+        float dt = (dy / s < dx / s) ? dy / s : dx / s;
+        dt *= kCfl;
+        return dt;
+    }
+    return FLT_MAX;
+}
+
+// Strip-sweep launch (flavour U), dispatched on solver, mode and segment length.
+int launch_sweep_u(int solver, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
+                   BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx,
+                   hipStream_t stream);
+int launch_apply_u(int xseg, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+                   int fuseBc, float* dtLog, int logIdx, hipStream_t stream);
+
+int launch_apply_bc(StateView v, BoundaryInfo bi, hipStream_t stream);
+int launch_apply_bc_bathymetry(float* b, int nx, int ny, int pitch, BoundaryInfo bi, hipStream_t stream);
+int launch_ctl_reset(DeviceCtl* ctl, int par, hipStream_t stream);
+
+// Flavour S.
+int launch_sweep_s_x(int solver, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream);
+int launch_sweep_s_y(int solver, StateView v, SumsView sums, hipStream_t stream);
+int launch_apply_s(StateView v, SumsView sums, float dt, hipStream_t stream);
+
+// Per-solver instantiations (one translation unit each, compiled in parallel).
+template <int S>
+int launch_sweep_u_solver(int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
+                          BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog,
+                          int logIdx, hipStream_t stream);
+template <int S>
+int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_t stream);
+template <int S>
+int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream);
+
+}  // namespace swe

@@ -1,0 +1,185 @@
+"""Python binding of libswe_hip.so, the MI355X SWE block library.
+
+The library's C ABI (include/swe_hip.h) is the drop-in boundary; this module
+binds it with ctypes for the tests and the bench.  There is no CPU fallback:
+if the HIP library is missing or fails to load, import raises.
+
+Host arrays use the reference's Float2D layout (src/tools/Float2D.hh:74-80):
+numpy float32 arrays of shape (nx+2, ny+2) in C order, so that a[x, y] sits at
+x*(ny+2) + y with ghost cells at index 0 and n+1.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_ROOT / "libswe_hip.so"
+
+# solvers (WAVE_PROPAGATION_SOLVER ids of SWE_DimensionalSplittingMpi.hh:42-50, + Hybrid)
+HLLE, FWAVE, AUGRIE, HYBRID = 0, 1, 2, 3
+SOLVER_NAMES = {"hlle": HLLE, "fwave": FWAVE, "augrie": AUGRIE, "hybrid": HYBRID}
+# step flavours
+FLAVOUR_U, FLAVOUR_S = 0, 1
+# BoundaryType / Boundary (src/types/Boundary.hh:5-28)
+OUTFLOW, WALL, INFLOW, CONNECT, CONNECT_WITHIN_RANK, PASSIVE = range(6)
+BND_LEFT, BND_RIGHT, BND_BOTTOM, BND_TOP = range(4)
+
+
+class SweHipError(RuntimeError):
+    pass
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise SweHipError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(LIB_PATH))
+    fp = C.POINTER(C.c_float)
+    vp = C.c_void_p
+    sig = {
+        "swe_hip_last_error": (C.c_char_p, []),
+        "swe_hip_version": (C.c_char_p, []),
+        "swe_hip_create": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                     C.c_int, C.POINTER(vp)]),
+        "swe_hip_destroy": (C.c_int, [vp]),
+        "swe_hip_upload": (C.c_int, [vp, fp, fp, fp, fp]),
+        "swe_hip_download": (C.c_int, [vp, fp, fp, fp, fp]),
+        "swe_hip_set_boundary": (C.c_int, [vp, C.c_int, C.c_int]),
+        "swe_hip_apply_bc": (C.c_int, [vp]),
+        "swe_hip_apply_bc_bathymetry": (C.c_int, [vp]),
+        "swe_hip_fluxes": (C.c_int, [vp, fp]),
+        "swe_hip_update": (C.c_int, [vp, C.c_float]),
+        "swe_hip_run": (C.c_int, [vp, C.c_int, fp, fp]),
+        "swe_hip_run_timed": (C.c_int, [vp, C.c_int, fp, fp, fp, C.POINTER(C.c_int)]),
+        "swe_hip_synchronize": (C.c_int, [vp]),
+        "swe_hip_device_pointers": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                              C.POINTER(C.c_int)]),
+        "swe_hip_solve_edges": (C.c_int, [C.c_int, C.c_int64] + [fp] * 11),
+        "swe_hip_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+        "swe_hip_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise SweHipError(f"{what} failed ({rc}): {lib.swe_hip_last_error().decode()}")
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class SweBlock:
+    """One SWE block on the GPU (the state lives in HBM)."""
+
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, ox: float = 0.0, oy: float = 0.0,
+                 solver: int = AUGRIE, flavour: int = FLAVOUR_U):
+        self.nx, self.ny, self.dx, self.dy = int(nx), int(ny), float(dx), float(dy)
+        self.solver, self.flavour = solver, flavour
+        h = C.c_void_p()
+        _check(lib.swe_hip_create(self.nx, self.ny, dx, dy, ox, oy, solver, flavour, C.byref(h)), "create")
+        self._h = h
+
+    @property
+    def shape(self):
+        return (self.nx + 2, self.ny + 2)
+
+    def close(self) -> None:
+        if self._h:
+            lib.swe_hip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_boundaries(self, types) -> None:
+        for side, t in enumerate(types):
+            _check(lib.swe_hip_set_boundary(self._h, side, int(t)), "set_boundary")
+
+    def upload(self, h, hu, hv, b) -> None:
+        arrs = [_f32(a, self.shape) for a in (h, hu, hv, b)]
+        _check(lib.swe_hip_upload(self._h, *[_fptr(a) for a in arrs]), "upload")
+
+    def download(self, with_b: bool = False):
+        out = [np.empty(self.shape, np.float32) for _ in range(4 if with_b else 3)]
+        ptrs = [_fptr(a) for a in out] + ([None] if not with_b else [])
+        _check(lib.swe_hip_download(self._h, *ptrs), "download")
+        return tuple(out)
+
+    def apply_bc(self) -> None:
+        _check(lib.swe_hip_apply_bc(self._h), "apply_bc")
+
+    def apply_bc_bathymetry(self) -> None:
+        _check(lib.swe_hip_apply_bc_bathymetry(self._h), "apply_bc_bathymetry")
+
+    def fluxes(self) -> float:
+        dt = C.c_float()
+        _check(lib.swe_hip_fluxes(self._h, C.byref(dt)), "fluxes")
+        return dt.value
+
+    def update(self, dt: float) -> None:
+        _check(lib.swe_hip_update(self._h, float(dt)), "update")
+
+    def run(self, nsteps: int, t: float = 0.0, want_dts: bool = True):
+        tt = C.c_float(t)
+        dts = np.empty(max(nsteps, 1), np.float32) if want_dts else None
+        _check(lib.swe_hip_run(self._h, int(nsteps), C.byref(tt), _fptr(dts) if want_dts else None), "run")
+        return tt.value, (dts[:nsteps] if want_dts else None)
+
+    def synchronize(self) -> None:
+        _check(lib.swe_hip_synchronize(self._h), "synchronize")
+
+    def device_pointers(self):
+        ps = [C.c_void_p() for _ in range(4)]
+        pitch = C.c_int()
+        _check(lib.swe_hip_device_pointers(self._h, *[C.byref(p) for p in ps], C.byref(pitch)), "device_pointers")
+        return [p.value for p in ps], pitch.value
+
+    def set_tuning(self, strategy: int = -1, xseg: int = -1) -> None:
+        _check(lib.swe_hip_set_tuning(self._h, strategy, xseg), "set_tuning")
+
+    def get_tuning(self):
+        s, x = C.c_int(), C.c_int()
+        _check(lib.swe_hip_get_tuning(self._h, C.byref(s), C.byref(x)), "get_tuning")
+        return s.value, x.value
+
+
+def solve_edges(solver: int, hL, hR, huL, huR, bL, bR):
+    """Per-edge net updates on the GPU: (hNetL, hNetR, huNetL, huNetR, maxSpeed)."""
+    ins = [np.ascontiguousarray(np.atleast_1d(a), dtype=np.float32) for a in (hL, hR, huL, huR, bL, bR)]
+    n = ins[0].size
+    outs = [np.empty(n, np.float32) for _ in range(5)]
+    _check(lib.swe_hip_solve_edges(solver, n, *[_fptr(a) for a in ins], *[_fptr(a) for a in outs]), "solve_edges")
+    return tuple(outs)
+
+
+def version() -> str:
+    return lib.swe_hip_version().decode()

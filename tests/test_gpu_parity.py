@@ -1,0 +1,167 @@
+"""GPU parity of the HIP step against the CPU oracle, through the C ABI.
+
+Bar: the kernels keep the oracle's single-precision operation order with
+no contraction and correctly rounded division/square root, so per-edge
+results, time steps and states after N steps are required to be identical
+(relative L-infinity 0); the north-star tolerance 1e-5 is reported beside.
+"""
+import numpy as np
+import pytest
+
+from helpers import random_edges, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = [(0, "hlle"), (1, "fwave"), (2, "augrie"), (3, "hybrid")]
+
+
+@pytest.mark.parametrize("solver,name", SOLVERS)
+def test_solve_edges_bitwise(swe, oracle, solver, name):
+    ins = random_edges(200_000, seed=11 + solver)
+    g = swe.solve_edges(solver, *ins)
+    o = oracle.solve_edges(solver, *ins)
+    for k, (a, b) in enumerate(zip(g, o)):
+        assert np.array_equal(a, b), f"{name} output {k}: max diff {np.nanmax(np.abs(a - b))}"
+
+
+def _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, solver, nsteps, strategy=1, xseg=16, flavour=0):
+    nx, ny = h.shape[0] - 2, h.shape[1] - 2
+    with swe.SweBlock(nx, ny, dx, dy, solver=solver, flavour=flavour) as blk:
+        blk.set_boundaries(bnd)
+        blk.set_tuning(strategy, xseg)
+        blk.upload(h, hu, hv, b)
+        t, dts = blk.run(nsteps)
+        out = blk.download()
+    return out, t, dts
+
+
+def _assert_parity(g, o, gdts, odts, tol=1e-5):
+    for name, a, b in zip(("h", "hu", "hv"), g, o):
+        e = rel_linf(a, b)
+        assert e <= tol, f"{name}: rel Linf {e:.3e}"
+        assert np.array_equal(a[1:-1, 1:-1], b[1:-1, 1:-1]), f"{name}: not bit-identical (rel Linf {e:.3e})"
+    assert np.array_equal(gdts, odts)
+
+
+def test_c1_fwave_256_200_steps(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_c1(256)
+    (gh, ghu, ghv), t, gdts = _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, swe.FWAVE, 200)
+    oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.FWAVE, 200)
+    _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
+    # the front stays inside the domain: mass conserved to round-off
+    m0 = scenarios.interior_mass(h, dx, dy)
+    m1 = scenarios.interior_mass(gh, dx, dy)
+    assert abs(m1 - m0) / m0 < 1e-6
+    assert abs(t - float(np.sum(gdts, dtype=np.float32))) <= 1e-3 * t
+
+
+@pytest.mark.parametrize("strategy,xseg", [(1, 16), (0, 16), (1, 8), (0, 32)])
+def test_islands_augrie_parity(swe, oracle, strategy, xseg):
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=192, seed=5)
+    (gh, ghu, ghv), t, gdts = _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, 60, strategy, xseg)
+    oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, 60)
+    _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
+    assert np.any(gh[1:-1, 1:-1] == 0.0), "the islands must stay dry"
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (1, 70), (70, 1), (63, 65), (130, 17), (61, 62), (17, 125)])
+@pytest.mark.parametrize("solver", [1, 3])
+def test_ragged_sizes_and_walls(swe, oracle, nx, ny, solver):
+    rng = np.random.default_rng(nx * 1000 + ny)
+    h = rng.uniform(0.0, 8.0, (nx + 2, ny + 2)).astype(np.float32)
+    h[rng.random(h.shape) < 0.1] = 0.0
+    hu = (h * rng.normal(0, 1, h.shape)).astype(np.float32)
+    hv = (h * rng.normal(0, 1, h.shape)).astype(np.float32)
+    b = rng.normal(0, 1, h.shape).astype(np.float32)
+    bnd = [1, 0, 0, 1]  # WALL, OUTFLOW, OUTFLOW, WALL
+    from swe_ppm_amd import scenarios
+
+    scenarios.apply_bc(h, hu, hv, bnd)
+    scenarios.apply_bc_bathymetry(b, bnd)
+    for strategy in (0, 1):
+        (gh, ghu, ghv), t, gdts = _run_gpu(swe, h, hu, hv, b, 1.0, 1.5, bnd, solver, 12, strategy)
+        oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, 1.0, 1.5, bnd, solver, 12)
+        _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
+
+
+def test_per_call_api_matches_oracle_step_by_step(swe, oracle):
+    """setGhostLayer / computeNumericalFluxes / getMaxTimestep / updateUnknowns
+    one call at a time, ghosts included, against the oracle's step."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=96, seed=9)
+    oh, ohu, ohv = h.copy(), hu.copy(), hv.copy()
+    with swe.SweBlock(96, 96, dx, dy, solver=swe.AUGRIE) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        for step in range(15):
+            blk.apply_bc()
+            dt = blk.fluxes()
+            blk.update(dt)
+            oh, ohu, ohv, odts = oracle.run_u(oh, ohu, ohv, b, dx, dy, bnd, swe.AUGRIE, 1)
+            assert dt == odts[0]
+            gh, ghu, ghv = blk.download()
+            for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
+                assert np.array_equal(a, c), f"step {step}"
+
+
+def test_run_ghosts_match_reference_sequence(swe, oracle):
+    """After run(), ghost cells hold the boundary values set at the start of
+    the last step, as after the reference's loop."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_c1(64)
+    for strategy in (0, 1):
+        (gh, ghu, ghv), t, _ = _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, swe.FWAVE, 7, strategy)
+        oh, ohu, ohv, _ = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.FWAVE, 7)
+        for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
+            assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("solver", [1, 3])
+def test_flavour_s_parity(swe, oracle, solver):
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_c1(96)
+    nsteps = 25
+    with swe.SweBlock(96, 96, dx, dy, solver=solver, flavour=swe.FLAVOUR_S) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        t, gdts = blk.run(nsteps)
+        gh, ghu, ghv = blk.download()
+    oh, ohu, ohv, odts = oracle.run_s(h, hu, hv, b, dx, bnd, solver, nsteps)
+    _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
+
+
+def test_apply_bc_matches_oracle(swe, oracle):
+    rng = np.random.default_rng(3)
+    nx, ny = 37, 29
+    h, hu, hv, b = (rng.normal(5, 1, (nx + 2, ny + 2)).astype(np.float32) for _ in range(4))
+    for bnd in ([0, 0, 0, 0], [1, 1, 1, 1], [1, 0, 5, 3], [5, 5, 5, 5]):
+        with swe.SweBlock(nx, ny, 1.0, 1.0) as blk:
+            blk.set_boundaries(bnd)
+            blk.upload(h, hu, hv, b)
+            blk.apply_bc()
+            blk.apply_bc_bathymetry()
+            gh, ghu, ghv, gb = blk.download(with_b=True)
+        oh, ohu, ohv, ob = h.copy(), hu.copy(), hv.copy(), b.copy()
+        oracle.apply_bc(bnd, oh, ohu, ohv)
+        oracle.apply_bc_bathymetry(bnd, ob)
+        for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv), (gb, ob)):
+            assert np.array_equal(a, c), bnd
+
+
+def test_invalid_arguments_fail_loudly(swe):
+    with pytest.raises(swe.SweHipError):
+        swe.SweBlock(0, 10, 1.0, 1.0)
+    with pytest.raises(swe.SweHipError):
+        swe.SweBlock(10, 10, -1.0, 1.0)
+    with swe.SweBlock(8, 8, 1.0, 1.0) as blk:
+        with pytest.raises(swe.SweHipError):
+            blk.set_boundaries([2, 0, 0, 0])  # INFLOW: the reference asserts on it
+        with pytest.raises(swe.SweHipError):
+            blk.set_tuning(7, -1)
