@@ -127,13 +127,38 @@ int swe_hip_synchronize(swe_hip_block* blk);
  * (floats between consecutive x columns; equal to ny+2). */
 int swe_hip_device_pointers(swe_hip_block* blk, float** h, float** hu, float** hv, float** b, int* pitch);
 
+/* Field masks for the layer functions below. */
+#define SWE_FIELD_H 1
+#define SWE_FIELD_HU 2
+#define SWE_FIELD_HV 4
+#define SWE_FIELD_B 8
+
+/* In-rank neighbour exchange: dst's ghost line on `side` <- src's copy layer
+ * (the interior line next to the shared edge), for the fields in `fields`.
+ * side RIGHT: dst[nx+1][1..ny] <- src[1][1..ny]; LEFT: dst[0][j] <- src[src.nx][j];
+ * TOP: dst[i][ny+1] <- src[i][1]; BOTTOM: dst[i][0] <- src[i][src.ny].
+ * Replaces the CONNECT_WITHIN_RANK copies of
+ * SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304 (h, hu, hv) and :208-235
+ * (bathymetry).  Ordered after src's queued work and before src's later
+ * work; the shared edge lengths must match. */
+int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fields);
+
+/* Host transfer of one line of length ny (LEFT/RIGHT) or nx (BOTTOM/TOP),
+ * indices 1..n: ghost = 1 selects the ghost line on `side`, ghost = 0 the
+ * copy layer next to it.  Packed per field in the order h, hu, hv, b for the
+ * bits set in `fields`.  Used by the host-side LTS state machine
+ * (SWE_Block.hh:375-524), which interpolates ghost lines on the host. */
+int swe_hip_read_layer(swe_hip_block* blk, int side, int ghost, int fields, float* out);
+int swe_hip_write_layer(swe_hip_block* blk, int side, int ghost, int fields, const float* in);
+
 /* Batch form of the per-edge solver interface
  *   solver::X<float>::computeNetUpdates(hL, hR, huL, huR, bL, bR,
  *                                       hNetL, hNetR, huNetL, huNetR, maxEdgeSpeed)
  * (call sites SWE_DimensionalSplittingMpi.cpp:403-410, 432-439; the solver
  * bodies live in the un-vendored TUM-I5/swe_solvers submodule).  Evaluates n
- * independent edges on the GPU; all arrays are host arrays of n floats. */
-int swe_hip_solve_edges(int solver, int64_t n, const float* hL, const float* hR, const float* huL, const float* huR,
+ * independent edges on the GPU; all arrays are host arrays of n floats.
+ * math selects the arithmetic variant (see swe_hip_set_math). */
+int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const float* hR, const float* huL, const float* huR,
                         const float* bL, const float* bR, float* hNetL, float* hNetR, float* huNetL, float* huNetR,
                         float* maxSpeed);
 
@@ -142,6 +167,14 @@ int swe_hip_solve_edges(int solver, int64_t n, const float* hL, const float* hR,
  * columns per wavefront strip (8, 16 or 32); -1 keeps the current value. */
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
+
+/* Arithmetic variant of the flavour-U kernels:
+ *   0 = IEEE: no contraction, correctly rounded division and square root;
+ *       results are bit-identical to an IEEE single-precision CPU evaluation
+ *       of the same formulas (the default);
+ *   1 = fast: FMA contraction, hardware reciprocal and square root; results
+ *       agree with the IEEE variant to the tolerance stated in DESIGN.md. */
+int swe_hip_set_math(swe_hip_block* blk, int math);
 
 #ifdef __cplusplus
 }

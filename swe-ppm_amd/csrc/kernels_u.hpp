@@ -198,7 +198,7 @@ __device__ __forceinline__ Cell update_cell(float h, float hu, float hv, float d
 // Strip sweep.  One wave per (y-tile, x-segment); 4 waves per workgroup.
 // dtFixed >= 0 forces dt (the per-call API passes the host's maxTimestep);
 // otherwise dt comes from ctl->smax[par].
-template <int S, int MODE, int XS>
+template <int S, int MODE, int XS, int V>
 __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums, float* hn, float* hun, float* hvn,
                                                        BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
                                                        int fuseBc, float* dtLog, int logIdx) {
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
 }
 
 // Apply stored net-update sums in place (second half of the store strategy).
-template <int XS>
+template <int XS, int V>
 __global__ __launch_bounds__(256) void apply_u_kernel(StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl,
                                                        int par, float dtFixed, int fuseBc, float* dtLog,
                                                        int logIdx) {

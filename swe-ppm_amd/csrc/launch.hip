@@ -94,39 +94,61 @@ __global__ __launch_bounds__(256) void apply_s_kernel(StateView v, SumsView sums
 }
 
 
-int launch_sweep_u(int solver, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                   BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx,
-                   hipStream_t stream) {
+template <int V>
+static int sweep_u_v(int solver, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
+                     BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx,
+                     hipStream_t stream) {
     switch (solver) {
         case SOLVER_HLLE:
-            return launch_sweep_u_solver<SOLVER_HLLE>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
-                                                      fuseBc, dtLog, logIdx, stream);
+            return launch_sweep_u_solver<SOLVER_HLLE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                         fuseBc, dtLog, logIdx, stream);
         case SOLVER_FWAVE:
-            return launch_sweep_u_solver<SOLVER_FWAVE>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
-                                                       fuseBc, dtLog, logIdx, stream);
+            return launch_sweep_u_solver<SOLVER_FWAVE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                          fuseBc, dtLog, logIdx, stream);
         case SOLVER_AUGRIE:
-            return launch_sweep_u_solver<SOLVER_AUGRIE>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
-                                                        fuseBc, dtLog, logIdx, stream);
+            return launch_sweep_u_solver<SOLVER_AUGRIE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                           fuseBc, dtLog, logIdx, stream);
         case SOLVER_HYBRID:
-            return launch_sweep_u_solver<SOLVER_HYBRID>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
-                                                        fuseBc, dtLog, logIdx, stream);
+            return launch_sweep_u_solver<SOLVER_HYBRID, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+                                                           fuseBc, dtLog, logIdx, stream);
         default:
             set_error("unknown solver %d", solver);
             return -1;
     }
 }
 
-int launch_apply_u(int xseg, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+int launch_sweep_u(int solver, int variant, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun,
+                   float* hvn, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog,
+                   int logIdx, hipStream_t stream) {
+    if (variant == 1)
+        return sweep_u_v<1>(solver, mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
+                            stream);
+    return sweep_u_v<0>(solver, mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
+                        stream);
+}
+
+int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
                    int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
-    constexpr int XS = 16;
-    (void)xseg;
-    const int nxs = (v.nx + XS - 1) / XS;
-    dim3 grid((v.ny + 63) / 64, (nxs + 3) / 4);
-    hipLaunchKernelGGL((apply_u_kernel<XS>), grid, dim3(256), 0, stream, v, sums, bi, ctl, par, dtFixed, fuseBc,
-                       dtLog, logIdx);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "apply_u_kernel launch");
-    return 0;
+    if (variant == 1) return launch_apply_u_variant<1>(v, sums, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+    return launch_apply_u_variant<0>(v, sums, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+}
+
+template <int V>
+static int solve_edges_v(int solver, int64_t n, const float* in, float* out, hipStream_t stream) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            return launch_solve_edges_solver<SOLVER_HLLE, V>(n, in, out, stream);
+        case SOLVER_FWAVE:
+            return launch_solve_edges_solver<SOLVER_FWAVE, V>(n, in, out, stream);
+        case SOLVER_AUGRIE:
+            return launch_solve_edges_solver<SOLVER_AUGRIE, V>(n, in, out, stream);
+        default:
+            return launch_solve_edges_solver<SOLVER_HYBRID, V>(n, in, out, stream);
+    }
+}
+
+int launch_solve_edges(int solver, int variant, int64_t n, const float* in, float* out, hipStream_t stream) {
+    return variant == 1 ? solve_edges_v<1>(solver, n, in, out, stream) : solve_edges_v<0>(solver, n, in, out, stream);
 }
 
 static int dispatch_s(int solver, int dir, StateView v, SumsView sums, DeviceCtl* ctl, int par,

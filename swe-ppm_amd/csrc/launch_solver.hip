@@ -1,6 +1,13 @@
-// launch_solver.hip -- per-solver kernel instantiations.  Compiled once per
-// solver with -DSWE_SOLVER_ID=<id> so the big device solvers build in
-// parallel translation units.
+// launch_solver.hip -- per-solver, per-math-variant kernel instantiations.
+// Compiled once per (solver, variant) with -DSWE_SOLVER_ID=<id> and
+// -DSWE_VARIANT=<0|1>, so that the big device solvers build in parallel
+// translation units and the two arithmetic variants get their own compile
+// flags:
+//   variant 0 (IEEE): no contraction, correctly rounded division and square
+//                     root -- bit-compatible with an IEEE CPU evaluation;
+//   variant 1 (fast): FMA contraction, hardware reciprocal / square root.
+// The variant is also a template parameter of every kernel, so the two
+// instantiations never share a symbol.
 #include <hip/hip_runtime.h>
 
 #include "kernels_s.hpp"
@@ -10,57 +17,89 @@
 #ifndef SWE_SOLVER_ID
 #error "SWE_SOLVER_ID must be defined"
 #endif
+#ifndef SWE_VARIANT
+#define SWE_VARIANT 0
+#endif
 
 namespace swe {
 
 namespace {
 
-template <int S, int MODE, int XS>
+template <int S, int MODE, int XS, int V>
 int go_u(StateView v, SumsView sums, float* hn, float* hun, float* hvn, BoundaryInfo bi, DeviceCtl* ctl, int par,
          float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
     const int nyt = (v.ny + kStripRows - 1) / kStripRows;
     const int nxs = (v.nx + XS - 1) / XS;
     const int waves = nyt * nxs;
     const int groups = (waves + kWavesPerGroup - 1) / kWavesPerGroup;
-    hipLaunchKernelGGL((sweep_u_kernel<S, MODE, XS>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v, sums,
-                       hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx);
+    hipLaunchKernelGGL((sweep_u_kernel<S, MODE, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v,
+                       sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "sweep_u_kernel launch");
     return 0;
 }
 
-template <int S, int MODE>
+template <int S, int MODE, int V>
 int go_u_xs(int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn, BoundaryInfo bi,
             DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
     switch (xseg) {
         case 8:
-            return go_u<S, MODE, 8>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+            return go_u<S, MODE, 8, V>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
         case 32:
-            return go_u<S, MODE, 32>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+            return go_u<S, MODE, 32, V>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
         default:
-            return go_u<S, MODE, 16>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+            return go_u<S, MODE, 16, V>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
     }
+}
+
+template <int S, int V>
+__global__ __launch_bounds__(256) void solve_edges_kernel(int64_t n, const float* __restrict__ in,
+                                                          float* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const dev::EdgeOut e =
+        dev::EdgeSolver<S>::solve(in[k], in[n + k], in[2 * n + k], in[3 * n + k], in[4 * n + k], in[5 * n + k]);
+    out[k] = e.hL;
+    out[n + k] = e.hR;
+    out[2 * n + k] = e.huL;
+    out[3 * n + k] = e.huR;
+    out[4 * n + k] = e.speed;
 }
 
 }  // namespace
 
-template <int S>
+template <int S, int V>
 int launch_sweep_u_solver(int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
                           BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog,
                           int logIdx, hipStream_t stream) {
     switch (mode) {
         case MODE_MAX:
-            return go_u_xs<S, MODE_MAX>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
-                                        stream);
+            return go_u_xs<S, MODE_MAX, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
+                                           stream);
         case MODE_STORE:
-            return go_u_xs<S, MODE_STORE>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
-                                          stream);
+            return go_u_xs<S, MODE_STORE, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog,
+                                             logIdx, stream);
         default:
-            return go_u_xs<S, MODE_UPDATE>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog,
-                                           logIdx, stream);
+            return go_u_xs<S, MODE_UPDATE, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog,
+                                              logIdx, stream);
     }
 }
 
+template <int S, int V>
+int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_t stream) {
+    const int64_t groups = (n + 255) / 256;
+    hipLaunchKernelGGL((solve_edges_kernel<S, V>), dim3((unsigned)groups), dim3(256), 0, stream, n, in, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "solve_edges_kernel launch");
+    return 0;
+}
+
+template int launch_sweep_u_solver<SWE_SOLVER_ID, SWE_VARIANT>(int, int, StateView, SumsView, float*, float*,
+                                                               float*, BoundaryInfo, DeviceCtl*, int, float, int,
+                                                               float*, int, hipStream_t);
+template int launch_solve_edges_solver<SWE_SOLVER_ID, SWE_VARIANT>(int64_t, const float*, float*, hipStream_t);
+
+#if SWE_VARIANT == 0
 template <int S>
 int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream) {
     if (dir == 0) {
@@ -78,34 +117,7 @@ int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, i
     if (e != hipSuccess) return hip_fail(e, "sweep_s kernel launch");
     return 0;
 }
-
-template <int S>
-__global__ __launch_bounds__(256) void solve_edges_kernel(int64_t n, const float* __restrict__ in,
-                                                          float* __restrict__ out) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const dev::EdgeOut e =
-        dev::EdgeSolver<S>::solve(in[k], in[n + k], in[2 * n + k], in[3 * n + k], in[4 * n + k], in[5 * n + k]);
-    out[k] = e.hL;
-    out[n + k] = e.hR;
-    out[2 * n + k] = e.huL;
-    out[3 * n + k] = e.huR;
-    out[4 * n + k] = e.speed;
-}
-
-template <int S>
-int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_t stream) {
-    const int64_t groups = (n + 255) / 256;
-    hipLaunchKernelGGL((solve_edges_kernel<S>), dim3((unsigned)groups), dim3(256), 0, stream, n, in, out);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "solve_edges_kernel launch");
-    return 0;
-}
-
-template int launch_solve_edges_solver<SWE_SOLVER_ID>(int64_t, const float*, float*, hipStream_t);
-template int launch_sweep_u_solver<SWE_SOLVER_ID>(int, int, StateView, SumsView, float*, float*, float*,
-                                                  BoundaryInfo, DeviceCtl*, int, float, int, float*, int,
-                                                  hipStream_t);
 template int launch_sweep_s_solver<SWE_SOLVER_ID>(int, StateView, SumsView, DeviceCtl*, int, hipStream_t);
+#endif
 
 }  // namespace swe

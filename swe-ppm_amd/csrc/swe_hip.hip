@@ -45,7 +45,8 @@ struct swe_hip_block {
     int solver = SOLVER_AUGRIE;
     int flavour = FLAVOUR_U;
     int strategy = 1;  // 0 recompute, 1 store
-    int xseg = 16;
+    int math = 0;      // 0 IEEE (bit-compatible), 1 fast
+    int xseg = 8;      // 8 columns per strip: most waves in flight for the VALU-bound solvers
     int cur = 0;       // index of the current state buffer
     BoundaryInfo bi{{PASSIVE, PASSIVE, PASSIVE, PASSIVE}};
     float* h[2] = {nullptr, nullptr};
@@ -142,20 +143,20 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
         if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
         if (k->strategy == 0) {
             const int o = k->cur ^ 1;
-            if (int r = launch_sweep_u(k->solver, MODE_MAX, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
+            if (int r = launch_sweep_u(k->solver, k->math, MODE_MAX, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
                                        nullptr, k->bi, k->ctl, par, -1.f, 0, nullptr, 0, k->stream))
                 return r;
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
-            if (int r = launch_sweep_u(k->solver, MODE_UPDATE, k->xseg, view(k, k->cur), sums_view(k), k->h[o],
+            if (int r = launch_sweep_u(k->solver, k->math, MODE_UPDATE, k->xseg, view(k, k->cur), sums_view(k), k->h[o],
                                        k->hu[o], k->hv[o], k->bi, k->ctl, par, -1.f, fuse, k->dtlog, s, k->stream))
                 return r;
             k->cur = o;
         } else {
-            if (int r = launch_sweep_u(k->solver, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr,
+            if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr,
                                        nullptr, nullptr, k->bi, k->ctl, par, -1.f, 0, nullptr, 0, k->stream))
                 return r;
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
-            if (int r = launch_apply_u(k->xseg, view(k, k->cur), sums_view(k), k->bi, k->ctl, par, -1.f, fuse,
+            if (int r = launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, par, -1.f, fuse,
                                        k->dtlog, s, k->stream))
                 return r;
         }
@@ -308,7 +309,7 @@ int swe_hip_fluxes(swe_hip_block* k, float* max_timestep) {
     unsigned int bits = 0;
     if (k->flavour == FLAVOUR_U) {
         if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
-        if (int r = launch_sweep_u(k->solver, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
+        if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
                                    nullptr, k->bi, k->ctl, 0, -1.f, 0, nullptr, 0, k->stream))
             return r;
         HIP_TRY(hipMemcpyAsync(&bits, &k->ctl->smax[0], sizeof(bits), hipMemcpyDeviceToHost, k->stream));
@@ -338,7 +339,7 @@ int swe_hip_update(swe_hip_block* k, float dt) {
             set_error("swe_hip_update: dt must be >= 0");
             return SWE_HIP_EINVAL;
         }
-        return launch_apply_u(k->xseg, view(k, k->cur), sums_view(k), k->bi, k->ctl, 0, dt, 0, nullptr, 0,
+        return launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, 0, dt, 0, nullptr, 0,
                               k->stream);
     }
     // flavour S: Q* was formed with the flux pass's own dt (the reference
@@ -432,10 +433,140 @@ int swe_hip_device_pointers(swe_hip_block* k, float** h, float** hu, float** hv,
     return SWE_HIP_OK;
 }
 
-int swe_hip_solve_edges(int solver, int64_t n, const float* hL, const float* hR, const float* huL,
+}  // extern "C"
+
+namespace {
+
+// Geometry of one boundary line: element 1..n of the ghost line (ghost=1)
+// or of the adjacent copy layer (ghost=0), as (offset, stride) in floats.
+struct LineGeom {
+    size_t off;
+    size_t stride;
+    int n;
+};
+
+LineGeom line_geom(const swe_hip_block* k, int side, int ghost) {
+    LineGeom g;
+    switch (side) {
+        case BND_LEFT:
+            g.off = (size_t)(ghost ? 0 : 1) * k->pitch + 1;
+            g.stride = 1;
+            g.n = k->ny;
+            break;
+        case BND_RIGHT:
+            g.off = (size_t)(ghost ? k->nx + 1 : k->nx) * k->pitch + 1;
+            g.stride = 1;
+            g.n = k->ny;
+            break;
+        case BND_BOTTOM:
+            g.off = (size_t)1 * k->pitch + (ghost ? 0 : 1);
+            g.stride = k->pitch;
+            g.n = k->nx;
+            break;
+        default:
+            g.off = (size_t)1 * k->pitch + (ghost ? k->ny + 1 : k->ny);
+            g.stride = k->pitch;
+            g.n = k->nx;
+            break;
+    }
+    return g;
+}
+
+float* field_ptr(swe_hip_block* k, int bit) {
+    switch (bit) {
+        case SWE_FIELD_H:
+            return k->h[k->cur];
+        case SWE_FIELD_HU:
+            return k->hu[k->cur];
+        case SWE_FIELD_HV:
+            return k->hv[k->cur];
+        default:
+            return k->b;
+    }
+}
+
+int opposite(int side) { return side ^ 1; }
+
+}  // namespace
+
+extern "C" {
+
+int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fields) {
+    if (int r = check(dst)) return r;
+    if (int r = check(src)) return r;
+    if (side < BND_LEFT || side > BND_TOP || fields <= 0 || fields > 15) {
+        set_error("swe_hip_copy_layer: invalid side %d or fields %d", side, fields);
+        return SWE_HIP_EINVAL;
+    }
+    const LineGeom gd = line_geom(dst, side, 1);
+    const LineGeom gs = line_geom(src, opposite(side), 0);
+    if (gd.n != gs.n) {
+        set_error("swe_hip_copy_layer: edge lengths differ (%d vs %d)", gd.n, gs.n);
+        return SWE_HIP_EINVAL;
+    }
+    hipEvent_t e1 = nullptr, e2 = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    hipError_t e = hipEventCreateWithFlags(&e2, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(e1, src->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(dst->stream, e1, 0);
+    for (int bit = 1; bit <= 8 && e == hipSuccess; bit <<= 1) {
+        if (!(fields & bit)) continue;
+        float* d = field_ptr(dst, bit) + gd.off;
+        const float* s = field_ptr(src, bit) + gs.off;
+        e = hipMemcpy2DAsync(d, gd.stride * sizeof(float), s, gs.stride * sizeof(float), sizeof(float), gd.n,
+                             hipMemcpyDeviceToDevice, dst->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e2, dst->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(src->stream, e2, 0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (e != hipSuccess) return hip_fail(e, "swe_hip_copy_layer");
+    return SWE_HIP_OK;
+}
+
+static int layer_host_transfer(swe_hip_block* k, int side, int ghost, int fields, float* out, const float* in) {
+    if (int r = check(k)) return r;
+    if (side < BND_LEFT || side > BND_TOP || fields <= 0 || fields > 15 || (!out && !in)) {
+        set_error("layer transfer: invalid side %d, fields %d or buffer", side, fields);
+        return SWE_HIP_EINVAL;
+    }
+    const LineGeom g = line_geom(k, side, ghost ? 1 : 0);
+    size_t pos = 0;
+    for (int bit = 1; bit <= 8; bit <<= 1) {
+        if (!(fields & bit)) continue;
+        float* d = field_ptr(k, bit) + g.off;
+        if (out)
+            HIP_TRY(hipMemcpy2DAsync(out + pos, sizeof(float), d, g.stride * sizeof(float), sizeof(float), g.n,
+                                     hipMemcpyDeviceToHost, k->stream));
+        else
+            HIP_TRY(hipMemcpy2DAsync(d, g.stride * sizeof(float), in + pos, sizeof(float), sizeof(float), g.n,
+                                     hipMemcpyHostToDevice, k->stream));
+        pos += g.n;
+    }
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_read_layer(swe_hip_block* k, int side, int ghost, int fields, float* out) {
+    if (!out) {
+        set_error("swe_hip_read_layer: null output");
+        return SWE_HIP_EINVAL;
+    }
+    return layer_host_transfer(k, side, ghost, fields, out, nullptr);
+}
+
+int swe_hip_write_layer(swe_hip_block* k, int side, int ghost, int fields, const float* in) {
+    if (!in) {
+        set_error("swe_hip_write_layer: null input");
+        return SWE_HIP_EINVAL;
+    }
+    return layer_host_transfer(k, side, ghost, fields, nullptr, in);
+}
+
+int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const float* hR, const float* huL,
                         const float* huR, const float* bL, const float* bR, float* hNetL, float* hNetR,
                         float* huNetL, float* huNetR, float* maxSpeed) {
-    if (n < 0 || n > ((int64_t)1 << 28) || solver < 0 || solver > 3) {
+    if (n < 0 || n > ((int64_t)1 << 28) || solver < 0 || solver > 3 || (math != 0 && math != 1)) {
         set_error("swe_hip_solve_edges: invalid n=%lld or solver=%d", (long long)n, solver);
         return SWE_HIP_EINVAL;
     }
@@ -464,20 +595,7 @@ int swe_hip_solve_edges(int solver, int64_t n, const float* hL, const float* hR,
     int rc = 0;
     for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMemcpy(din + i * n, ins[i], nb, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        switch (solver) {
-            case SOLVER_HLLE:
-                rc = launch_solve_edges_solver<SOLVER_HLLE>(n, din, dout, nullptr);
-                break;
-            case SOLVER_FWAVE:
-                rc = launch_solve_edges_solver<SOLVER_FWAVE>(n, din, dout, nullptr);
-                break;
-            case SOLVER_AUGRIE:
-                rc = launch_solve_edges_solver<SOLVER_AUGRIE>(n, din, dout, nullptr);
-                break;
-            default:
-                rc = launch_solve_edges_solver<SOLVER_HYBRID>(n, din, dout, nullptr);
-                break;
-        }
+        rc = launch_solve_edges(solver, math, n, din, dout, nullptr);
     }
     if (e == hipSuccess && rc == 0) e = hipDeviceSynchronize();
     for (int i = 0; i < 5 && e == hipSuccess && rc == 0; ++i)
@@ -500,6 +618,20 @@ int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
     }
     if (strategy != -1) k->strategy = strategy;
     if (xseg != -1) k->xseg = xseg;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_set_math(swe_hip_block* k, int math) {
+    if (int r = check(k)) return r;
+    if (math != 0 && math != 1) {
+        set_error("swe_hip_set_math: math must be 0 (IEEE) or 1 (fast)");
+        return SWE_HIP_EINVAL;
+    }
+    if (math == 1 && k->flavour != FLAVOUR_U) {
+        set_error("swe_hip_set_math: the fast variant exists for flavour U only");
+        return SWE_HIP_EINVAL;
+    }
+    k->math = math;
     return SWE_HIP_OK;
 }
 

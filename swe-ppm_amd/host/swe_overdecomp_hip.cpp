@@ -1,0 +1,220 @@
+// swe_overdecomp_hip.cpp -- many blocks in one process, neighbours connected
+// in memory (CONNECT_WITHIN_RANK), global or local time stepping: the loop
+// structure of the reference's overdecomposed driver
+// (src/examples/swe_mpi_overdecomp.cpp:147-343) with every block's step on
+// the GPU and no MPI (one rank).
+//
+//   swe_overdecomp_hip -t 80 -n 20 -x 2048 -y 2048 -b 28 [-l 1] [--rows R]
+//                      [--solver augrie] [--input blocks.bin] [--dump out.bin]
+//
+// Block k sits at column k / rows, row k % rows; neighbours follow
+// getNeighbours (swe_mpi_overdecomp.cpp:41-49).  Without --input the blocks
+// are an even split of the scenario (remainder cells to the last row/column,
+// swe_mpi_overdecomp.cpp:164-178); with --input the block sizes and initial
+// states come from a snapshot file.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+
+#include "blocks/SWE_DimensionalSplittingHip.hh"
+#include "scenarios/SWE_simple_scenarios.hh"
+#include "swe_drivers_common.hh"
+
+namespace {
+
+std::array<int, 4> get_neighbours(int px, int py, int cols, int rows, int k) {
+    std::array<int, 4> n;
+    n[BND_LEFT] = (px > 0) ? k - rows : -1;
+    n[BND_RIGHT] = (px < cols - 1) ? k + rows : -1;
+    n[BND_BOTTOM] = (py > 0) ? k - 1 : -1;
+    n[BND_TOP] = (py < rows - 1) ? k + 1 : -1;
+    return n;
+}
+
+// initial state copied from a snapshot instead of a scenario
+struct SnapshotScenarioLoader {
+    static void load(SWE_DimensionalSplittingHip& blk, const BlockState& s, BoundaryType bnd[4]) {
+        const size_t m = (size_t)(s.nx + 2) * (s.ny + 2);
+        std::copy(s.h.begin(), s.h.end(), blk.h.getRawPointer());
+        std::copy(s.hu.begin(), s.hu.end(), blk.hu.getRawPointer());
+        std::copy(s.hv.begin(), s.hv.end(), blk.hv.getRawPointer());
+        std::copy(s.b.begin(), s.b.end(), blk.b.getRawPointer());
+        (void)m;
+        for (int i = 0; i < 4; i++) blk.setBoundaryType(static_cast<Boundary>(i), bnd[i]);
+        blk.synchAfterWrite();
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        // --rows is read here (not a reference flag)
+        int rowsArg = -1;
+        std::vector<char*> rest;
+        for (int i = 0; i < argc; i++) {
+            if (std::string(argv[i]) == "--rows" && i + 1 < argc) {
+                rowsArg = std::atoi(argv[++i]);
+                continue;
+            }
+            rest.push_back(argv[i]);
+        }
+        DriverArgs a = parse_args((int)rest.size(), rest.data(), "radial-bathymetry");
+        const bool lts = a.lts > 0.f;
+        std::unique_ptr<SWE_Scenario> scenario(new SWE_RadialBathymetryDamBreakScenario());
+        std::vector<float> cp = checkpoint_instants(a.duration, a.checkpoints);
+
+        std::vector<BlockState> init;
+        if (!a.input.empty()) init = read_snapshot(a.input);
+        const int total = init.empty() ? a.blocks : (int)init.size();
+        int rows = rowsArg;
+        if (rows <= 0) {
+            rows = (int)std::sqrt((double)total);
+            while (total % rows != 0) rows--;
+        }
+        if (total % rows != 0) throw std::invalid_argument("block count must be a multiple of --rows");
+        const int cols = total / rows;
+
+        const int width = (int)(scenario->getBoundaryPos(BND_RIGHT) - scenario->getBoundaryPos(BND_LEFT));
+        const int height = (int)(scenario->getBoundaryPos(BND_TOP) - scenario->getBoundaryPos(BND_BOTTOM));
+        float dx = (float)width / a.nx;
+        float dy = (float)height / a.ny;
+        if (!init.empty()) {
+            int nxTot = 0, nyTot = 0;
+            for (int c = 0; c < cols; c++) nxTot += init[c * rows].nx;
+            for (int r = 0; r < rows; r++) nyTot += init[r].ny;
+            dx = (float)width / nxTot;
+            dy = (float)height / nyTot;
+        }
+
+        std::vector<std::unique_ptr<SWE_DimensionalSplittingHip>> blocks(total);
+        const int nxBlock = a.nx / cols, nyBlock = a.ny / rows;
+        for (int k = 0; k < total; k++) {
+            const int px = k / rows, py = k % rows;
+            int nxl, nyl;
+            float ox, oy;
+            if (init.empty()) {
+                nxl = (px < cols - 1) ? nxBlock : a.nx - (cols - 1) * nxBlock;
+                nyl = (py < rows - 1) ? nyBlock : a.ny - (rows - 1) * nyBlock;
+                ox = scenario->getBoundaryPos(BND_LEFT) + px * dx * nxBlock;
+                oy = scenario->getBoundaryPos(BND_BOTTOM) + py * dy * nyBlock;
+            } else {
+                nxl = init[k].nx;
+                nyl = init[k].ny;
+                int xo = 0, yo = 0;
+                for (int c = 0; c < px; c++) xo += init[c * rows].nx;
+                for (int r = 0; r < py; r++) yo += init[r].ny;
+                ox = xo * dx;
+                oy = yo * dy;
+            }
+            blocks[k].reset(new SWE_DimensionalSplittingHip(nxl, nyl, dx, dy, ox, oy, lts, a.solver, a.flavour));
+        }
+        for (int k = 0; k < total; k++) {
+            const int px = k / rows, py = k % rows;
+            std::array<int, 4> nb = get_neighbours(px, py, cols, rows, k);
+            BoundaryType bnd[4];
+            std::array<SWE_DimensionalSplittingHip*, 4> nbb{};
+            for (int s = 0; s < 4; s++) {
+                if (nb[s] >= 0) {
+                    bnd[s] = CONNECT_WITHIN_RANK;
+                    nbb[s] = blocks[nb[s]].get();
+                } else {
+                    bnd[s] = scenario->getBoundaryType(static_cast<Boundary>(s));
+                }
+            }
+            if (init.empty())
+                blocks[k]->initScenario(*scenario, bnd);
+            else
+                SnapshotScenarioLoader::load(*blocks[k], init[k], bnd);
+            blocks[k]->connectLocalNeighbours(nbb);
+            blocks[k]->setRank(k);
+            blocks[k]->setDuration(a.duration);
+        }
+        for (auto& b : blocks) b->sendBathymetry();
+        if (!a.dump0.empty()) {
+            std::vector<BlockState> s0;
+            for (auto& b : blocks) s0.push_back(snapshot_of(*b));
+            write_snapshot(a.dump0, s0);
+        }
+
+        float maxLocalTimestep = 0.f;
+        if (lts) {
+            // swe_mpi_overdecomp.cpp:235-245: the CFL estimate is computed and
+            // then replaced by the command-line macro step
+            for (auto& b : blocks) b->computeMaxTimestep(0.01f, 0.4f);
+            maxLocalTimestep = a.lts;
+            for (auto& b : blocks) b->setMaxLocalTimestep(maxLocalTimestep);
+        }
+
+        std::vector<long> updates(total, 0);
+        float t = 0.f, timestep = 0.f;
+        long iterations = 0, macro = 0;
+        double wall = 0.0;
+        bool stop = false;
+        for (int i = 0; i < a.checkpoints && !stop; i++) {
+            while (t < cp[i]) {
+                auto t0 = std::chrono::steady_clock::now();
+                bool synchronized = true;
+                do {
+#pragma omp parallel for schedule(dynamic, 1)
+                    for (int k = 0; k < total; k++) blocks[k]->setGhostLayer();
+#pragma omp parallel for schedule(dynamic, 1)
+                    for (int k = 0; k < total; k++) blocks[k]->receiveGhostLayer();
+#pragma omp parallel for schedule(dynamic, 1)
+                    for (int k = 0; k < total; k++) blocks[k]->computeNumericalFluxes();
+                    if (!lts) {
+                        timestep = blocks[0]->maxTimestep;
+                        for (auto& b : blocks) timestep = std::min(timestep, b->maxTimestep);
+                        for (auto& b : blocks) b->maxTimestep = timestep;
+                    } else {
+                        for (auto& b : blocks)
+                            if (b->allGhostlayersInSync()) b->maxTimestep = b->getRoundTimestep(b->maxTimestep);
+                    }
+#pragma omp parallel for schedule(dynamic, 1)
+                    for (int k = 0; k < total; k++) {
+                        if (blocks[k]->allGhostlayersInSync()) updates[k]++;
+                        blocks[k]->updateUnknowns(timestep);
+                    }
+                    iterations++;
+                    if (lts) {
+                        synchronized = true;
+                        for (auto& b : blocks)
+                            if (!b->hasMaxLocalTimestep()) synchronized = false;
+                    }
+                    if (iterations > 100000000L) throw std::runtime_error("LTS made no progress");
+                } while (lts && !synchronized);
+                t += lts ? maxLocalTimestep : timestep;
+                macro++;
+                if (lts)
+                    for (auto& b : blocks) b->resetStepSizeCounter();
+                for (auto& b : blocks) swe_hip_synchronize(b->handle());
+                wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (a.max_steps > 0 && macro >= a.max_steps) {
+                    stop = true;
+                    break;
+                }
+            }
+            std::printf("Write timestep (%fs)\n", t);
+        }
+        long cellUpdates = 0;
+        for (int k = 0; k < total; k++) cellUpdates += updates[k] * (long)blocks[k]->nx * blocks[k]->ny;
+        if (!a.dump.empty()) {
+            std::vector<BlockState> s;
+            for (auto& b : blocks) s.push_back(snapshot_of(*b));
+            write_snapshot(a.dump, s);
+        }
+        std::printf("blocks %d (%d x %d) macro-steps %ld iterations %ld simulated %.6f s wall %.3f s "
+                    "%.2f MCell-updates/s\n",
+                    total, cols, rows, macro, iterations, t, wall, cellUpdates / wall / 1e6);
+        std::printf("updates");
+        for (long u : updates) std::printf(" %ld", u);
+        std::printf("\n");
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "swe_overdecomp_hip: %s\n", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,84 @@
+// swe_simple_hip.cpp -- single-block driver with swe_simple's structure
+// (reference src/examples/swe_simple.cpp:56-274): scenario, checkpoints,
+// and the loop setGhostLayer -> computeNumericalFluxes -> getMaxTimestep ->
+// updateUnknowns -> t += dt, with the block's step on the GPU.
+//
+//   swe_simple_hip -t 25 -n 5 -x 256 -y 256 -o out [--solver fwave]
+//                  [--flavour u|s] [--scenario radial|centred|...] [--dump f]
+#include <chrono>
+#include <cstdio>
+#include <memory>
+
+#include "blocks/SWE_DimensionalSplittingHip.hh"
+#include "scenarios/SWE_simple_scenarios.hh"
+#include "swe_drivers_common.hh"
+#include "writer/VtkWriter.hh"
+
+static std::unique_ptr<SWE_Scenario> make_scenario(const std::string& name) {
+    if (name == "radial") return std::unique_ptr<SWE_Scenario>(new SWE_RadialDamBreakScenario());
+    if (name == "centred") return std::unique_ptr<SWE_Scenario>(new SWE_CentredDamBreakScenario());
+    if (name == "radial-bathymetry") return std::unique_ptr<SWE_Scenario>(new SWE_RadialBathymetryDamBreakScenario());
+    if (name == "bathymetry") return std::unique_ptr<SWE_Scenario>(new SWE_BathymetryDamBreakScenario());
+    if (name == "half-dry") return std::unique_ptr<SWE_Scenario>(new SWE_HalfDomainDry());
+    if (name == "splashing-pool") return std::unique_ptr<SWE_Scenario>(new SWE_SplashingPoolScenario());
+    throw std::invalid_argument("unknown scenario " + name);
+}
+
+int main(int argc, char** argv) {
+    try {
+        DriverArgs a = parse_args(argc, argv, "radial");
+        std::unique_ptr<SWE_Scenario> scenario = make_scenario(a.scenario);
+        std::vector<float> cp = checkpoint_instants(a.duration, a.checkpoints);
+
+        const int width = (int)(scenario->getBoundaryPos(BND_RIGHT) - scenario->getBoundaryPos(BND_LEFT));
+        const int height = (int)(scenario->getBoundaryPos(BND_TOP) - scenario->getBoundaryPos(BND_BOTTOM));
+        const float dx = (float)width / a.nx;
+        const float dy = (float)height / a.ny;
+        const float ox = scenario->getBoundaryPos(BND_LEFT);
+        const float oy = scenario->getBoundaryPos(BND_BOTTOM);
+        BoundaryType boundaries[4];
+        for (int s = 0; s < 4; s++) boundaries[s] = scenario->getBoundaryType(static_cast<Boundary>(s));
+
+        SWE_DimensionalSplittingHip sim(a.nx, a.ny, dx, dy, ox, oy, false, a.solver, a.flavour);
+        sim.initScenario(*scenario, boundaries);
+        if (!a.dump0.empty()) write_snapshot(a.dump0, {snapshot_of(sim)});
+
+        std::unique_ptr<VtkWriter> writer;
+        if (a.write) {
+            writer.reset(new VtkWriter(a.out, sim.getBathymetry(), {{1, 1, 1, 1}}, a.nx, a.ny, dx, dy, ox, oy));
+            writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(), 0.f);
+        }
+
+        float t = 0.f;
+        long iterations = 0;
+        double wall = 0.0;
+        bool stop = false;
+        for (int i = 0; i < a.checkpoints && !stop; i++) {
+            while (t < cp[i]) {
+                auto t0 = std::chrono::steady_clock::now();
+                sim.setGhostLayer();
+                sim.computeNumericalFluxes();
+                const float timestep = sim.getMaxTimestep();
+                sim.updateUnknowns(timestep);
+                swe_hip_synchronize(sim.handle());
+                wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                t += timestep;
+                iterations++;
+                if (a.max_steps > 0 && iterations >= a.max_steps) {
+                    stop = true;
+                    break;
+                }
+            }
+            std::printf("Write timestep (%fs)\n", t);
+            if (writer)
+                writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(), t);
+        }
+        if (!a.dump.empty()) write_snapshot(a.dump, {snapshot_of(sim)});
+        std::printf("iterations %ld simulated %.6f s wall %.4f s  %.2f MCell-updates/s\n", iterations, t, wall,
+                    (double)a.nx * a.ny * iterations / wall / 1e6);
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "swe_simple_hip: %s\n", e.what());
+        return 1;
+    }
+}

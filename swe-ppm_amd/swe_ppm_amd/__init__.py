@@ -58,7 +58,8 @@ def _load() -> C.CDLL:
         "swe_hip_synchronize": (C.c_int, [vp]),
         "swe_hip_device_pointers": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                               C.POINTER(C.c_int)]),
-        "swe_hip_solve_edges": (C.c_int, [C.c_int, C.c_int64] + [fp] * 11),
+        "swe_hip_solve_edges": (C.c_int, [C.c_int, C.c_int, C.c_int64] + [fp] * 11),
+        "swe_hip_set_math": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
         "swe_hip_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     }
@@ -166,18 +167,22 @@ class SweBlock:
     def set_tuning(self, strategy: int = -1, xseg: int = -1) -> None:
         _check(lib.swe_hip_set_tuning(self._h, strategy, xseg), "set_tuning")
 
+    def set_math(self, math: int) -> None:
+        """0 = IEEE (bit-compatible with the CPU oracle), 1 = fast."""
+        _check(lib.swe_hip_set_math(self._h, int(math)), "set_math")
+
     def get_tuning(self):
         s, x = C.c_int(), C.c_int()
         _check(lib.swe_hip_get_tuning(self._h, C.byref(s), C.byref(x)), "get_tuning")
         return s.value, x.value
 
 
-def solve_edges(solver: int, hL, hR, huL, huR, bL, bR):
+def solve_edges(solver: int, hL, hR, huL, huR, bL, bR, math: int = 0):
     """Per-edge net updates on the GPU: (hNetL, hNetR, huNetL, huNetR, maxSpeed)."""
     ins = [np.ascontiguousarray(np.atleast_1d(a), dtype=np.float32) for a in (hL, hR, huL, huR, bL, bR)]
     n = ins[0].size
     outs = [np.empty(n, np.float32) for _ in range(5)]
-    _check(lib.swe_hip_solve_edges(solver, n, *[_fptr(a) for a in ins], *[_fptr(a) for a in outs]), "solve_edges")
+    _check(lib.swe_hip_solve_edges(solver, math, n, *[_fptr(a) for a in ins], *[_fptr(a) for a in outs]), "solve_edges")
     return tuple(outs)
 
 
