@@ -89,7 +89,22 @@ def main() -> int:
     ap.add_argument("--xseg", type=int, default=-1)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--math", type=int, default=0, help="0 IEEE (bit-compatible), 1 fast")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n,
+                          "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
@@ -103,6 +118,7 @@ def main() -> int:
     blk = swe.SweBlock(n, n, dx, dy, solver=swe.AUGRIE)
     blk.set_boundaries(bnd)
     blk.set_tuning(args.strategy, args.xseg)
+    blk.set_math(args.math)
     strategy, xseg = blk.get_tuning()
 
     # warmup from the initial state, then the timed K steps from the initial state again

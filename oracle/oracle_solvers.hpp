@@ -409,7 +409,9 @@ void hlle(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T
     const T fR = huR * uR + T(0.5) * C::g * hR * hR;
     const T fL = huL * uL + T(0.5) * C::g * hL * hL;
     const T df2 = (fR - fL) + T(0.5) * C::g * (hL + hR) * (bR - bL);
-    const T dq1 = hR - hL;
+    // surface-gradient form (Zhou et al. 2001): the HLL diffusion acts on the
+    // jump of the water surface h + b, so a lake at rest gives no update
+    const T dq1 = (hR + bR) - (hL + bL);
     const T dq2 = huR - huL;
     const T ds = sR - sL;
     // HLL flux F* minus the left flux, in f-wave form:

@@ -347,7 +347,7 @@ __device__ __forceinline__ EdgeOut hlle(float hL, float hR, float huL, float huR
     const float fR = hur * uR + 0.5f * kGravity * hr * hr;
     const float fL = hul * uL + 0.5f * kGravity * hl * hl;
     const float df2 = (fR - fL) + 0.5f * kGravity * (hl + hr) * (br - bl);
-    const float dq1 = hr - hl;
+    const float dq1 = (hr + br) - (hl + bl);  // surface-gradient form: lake at rest stays at rest
     const float dq2 = hur - hul;
     const float ds = sR - sL;
     const float q1 = (sR * dq1 - df1) / ds;

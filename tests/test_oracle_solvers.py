@@ -1,0 +1,217 @@
+"""Pinning the oracle's per-edge solvers without the (absent) solver
+submodule: every check is a property the mathematics fixes, chosen so that a
+plausible mistake (dropped term, wrong sign, swapped operand, wrong
+derivative) fails one of them.
+
+  * zero jump: no update, speed |u| + sqrt(g h) (closed form)
+  * mass: hNetL + hNetR = huR - huL (f-waves sum to the flux jump)
+  * momentum: huNetL + huNetR = flux jump + source, the source lying between
+    the left- and right-depth discretisations of -g h db
+  * well-balancing: lake at rest over a bathymetry step, wet/dry shoreline
+  * mirror symmetry x -> -x
+  * middle state = exact Riemann solution (independent bisection, FP64),
+    two-rarefaction closed form
+  * Einfeldt speeds bound the characteristic speeds
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import random_edges
+
+G = 9.81
+SOLVERS = {"hlle": 0, "fwave": 1, "augrie": 2, "hybrid": 3}
+
+
+def wet_edges(n, seed, bathy=True):
+    rng = np.random.default_rng(seed)
+    hL = rng.uniform(0.5, 30.0, n)
+    hR = rng.uniform(0.5, 30.0, n)
+    uL = rng.normal(0, 2.0, n)
+    uR = rng.normal(0, 2.0, n)
+    bL = rng.normal(0, 1.0, n) if bathy else np.zeros(n)
+    bR = bL + (rng.normal(0, 0.3, n) if bathy else 0.0)
+    return hL, hR, hL * uL, hR * uR, bL, bR
+
+
+@pytest.mark.parametrize("name", list(SOLVERS))
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_zero_jump_closed_form(oracle, name, dtype):
+    rng = np.random.default_rng(1)
+    h = rng.uniform(0.02, 50.0, 5000)
+    hu = h * rng.normal(0, 3.0, 5000)
+    b = rng.normal(0, 5.0, 5000)
+    out = oracle.solve_edges(SOLVERS[name], h, h, hu, hu, b, b, dtype=dtype)
+    for a in out[:4]:
+        assert np.all(a == 0.0)
+    hh = h.astype(dtype)
+    u = (hu.astype(dtype) / hh).astype(np.float64)
+    c = np.sqrt(G * hh.astype(np.float64))
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    np.testing.assert_allclose(out[4], np.abs(u) + c, rtol=tol)
+
+
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_mass_flux_split(oracle, name):
+    """h-components of the f-waves add up to the mass-flux jump huR - huL."""
+    hL, hR, huL, huR, bL, bR = wet_edges(20000, 2)
+    o = oracle.solve_edges(SOLVERS[name], hL, hR, huL, huR, bL, bR, dtype=np.float64)
+    np.testing.assert_allclose(o[0] + o[1], huR - huL, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["fwave", "hlle"])
+def test_momentum_flux_split_fwave(oracle, name):
+    """f-wave / HLLE: momentum updates add up to the flux jump plus the
+    source g/2 (hL + hR)(bR - bL)."""
+    hL, hR, huL, huR, bL, bR = wet_edges(20000, 3)
+    o = oracle.solve_edges(SOLVERS[name], hL, hR, huL, huR, bL, bR, dtype=np.float64)
+    jump = (huR ** 2 / hR + 0.5 * G * hR ** 2) - (huL ** 2 / hL + 0.5 * G * hL ** 2)
+    src = 0.5 * G * (hL + hR) * (bR - bL)
+    np.testing.assert_allclose(o[2] + o[3], jump + src, rtol=1e-9, atol=1e-9)
+
+
+def test_momentum_flux_split_augrie(oracle):
+    """AugRie: the momentum updates equal the flux jump minus the steady-state
+    part, which lies between -g hL db and -g hR db (George 2008, eq. 4.6)."""
+    hL, hR, huL, huR, bL, bR = wet_edges(20000, 4)
+    o = oracle.solve_edges(2, hL, hR, huL, huR, bL, bR, dtype=np.float64)
+    jump = (huR ** 2 / hR + 0.5 * G * hR ** 2) - (huL ** 2 / hL + 0.5 * G * hL ** 2)
+    ddphi = jump - (o[2] + o[3])  # the subtracted steady-state momentum jump
+    db = bR - bL
+    lo = G * np.minimum(-hL * db, -hR * db)
+    hi = G * np.maximum(-hL * db, -hR * db)
+    assert np.all(ddphi >= lo - 1e-8 * (1 + np.abs(lo)))
+    assert np.all(ddphi <= hi + 1e-8 * (1 + np.abs(hi)))
+
+
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_lake_at_rest_over_step(oracle, name):
+    """h + b constant, u = 0, both cells wet: no updates (well-balanced)."""
+    rng = np.random.default_rng(5)
+    eta = rng.uniform(5.0, 40.0, 5000)
+    bL = rng.uniform(-3.0, 3.0, 5000)
+    bR = bL + rng.uniform(-2.0, 2.0, 5000)
+    hL, hR = eta - bL, eta - bR
+    z = np.zeros(5000)
+    o = oracle.solve_edges(SOLVERS[name], hL, hR, z, z, bL, bR, dtype=np.float64)
+    scale = G * eta ** 2
+    for a in o[:4]:
+        assert np.max(np.abs(a) / scale) < 1e-12
+
+
+def test_shoreline_at_rest_augrie(oracle):
+    """Wet cell at rest next to a dry cell whose bed stands above the water
+    surface: a wall, no flux either way (George 2008, sec. 5)."""
+    rng = np.random.default_rng(6)
+    h = rng.uniform(0.05, 10.0, 3000)
+    bL = rng.uniform(-5.0, 0.0, 3000)
+    bR = bL + h + rng.uniform(0.01, 5.0, 3000)
+    z = np.zeros(3000)
+    for args in ((h, z, z, z, bL, bR), (z, h, z, z, bR, bL)):
+        o = oracle.solve_edges(2, *args, dtype=np.float64)
+        for a in o[:4]:
+            assert np.max(np.abs(a)) == 0.0
+
+
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_mirror_symmetry(oracle, name):
+    """Reflecting x -> -x swaps the cells and negates the momentum: the
+    update of each physical cell is unchanged."""
+    ins = random_edges(20000, 7)
+    hL, hR, huL, huR, bL, bR = [a.astype(np.float64) for a in ins]
+    o = oracle.solve_edges(SOLVERS[name], hL, hR, huL, huR, bL, bR, dtype=np.float64)
+    m = oracle.solve_edges(SOLVERS[name], hR, hL, -huR, -huL, bR, bL, dtype=np.float64)
+    scale = 1.0 + np.abs(o[0]) + np.abs(o[1]) + np.abs(o[2]) + np.abs(o[3])
+    for a, b in ((m[0], o[1]), (m[1], o[0]), (m[2], -o[3]), (m[3], -o[2]), (m[4], o[4])):
+        assert np.max(np.abs(a - b) / scale) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["fwave", "augrie", "hlle"])
+def test_speed_bounds_characteristics(oracle, name):
+    hL, hR, huL, huR, bL, bR = wet_edges(20000, 8, bathy=False)
+    o = oracle.solve_edges(SOLVERS[name], hL, hR, huL, huR, bL, bR, dtype=np.float64)
+    uL, uR = huL / hL, huR / hR
+    bound = np.maximum(np.abs(uL - np.sqrt(G * hL)), np.abs(uR + np.sqrt(G * hR)))
+    assert np.all(o[4] >= bound * (1 - 1e-12))
+
+
+def exact_middle_depth(hL, hR, uL, uR):
+    """Root of du + f(h; hL) + f(h; hR) = 0 (Toro, shallow water) by bisection."""
+
+    def f(h, hk):
+        if h > hk:
+            return (h - hk) * math.sqrt(0.5 * G * (1.0 / h + 1.0 / hk))
+        return 2.0 * (math.sqrt(G * h) - math.sqrt(G * hk))
+
+    du = uR - uL
+    lo, hi = 1e-12, 10.0 * max(hL, hR) + (du * du if du < 0 else 0.0) / G + 1.0
+    while du + f(hi, hL) + f(hi, hR) < 0:
+        hi *= 2
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if du + f(mid, hL) + f(mid, hR) < 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def test_middle_state_matches_exact_riemann_solution(oracle):
+    rng = np.random.default_rng(9)
+    n = 3000
+    hL = rng.uniform(0.05, 50.0, n)
+    hR = np.where(rng.random(n) < 0.3, hL * rng.uniform(0.9, 1.1, n), rng.uniform(0.05, 50.0, n))
+    uL = rng.normal(0, 4.0, n)
+    uR = rng.normal(0, 4.0, n)
+    keep = (uR - uL) < 2 * (np.sqrt(G * hL) + np.sqrt(G * hR)) * 0.95  # no vacuum
+    hL, hR, uL, uR = hL[keep], hR[keep], uL[keep], uR[keep]
+    hm, s1m, s2m, kind, its = oracle.middle_state(hL, hR, uL, uR)
+    exact = np.array([exact_middle_depth(*v) for v in zip(hL, hR, uL, uR)])
+    np.testing.assert_allclose(hm, exact, rtol=1e-9)
+    assert set(np.unique(kind)) == {1, 2, 3}
+    assert np.all(its <= 10)
+    # wave-curve consistency: both families meet at one middle velocity
+    cm = np.sqrt(G * hm)
+    shock = kind == 2
+    u_from_s1 = s1m + cm
+    u_from_s2 = s2m - cm
+    np.testing.assert_allclose(u_from_s1[shock], u_from_s2[shock], rtol=1e-8, atol=1e-8)
+
+
+def test_two_rarefaction_closed_form(oracle):
+    rng = np.random.default_rng(10)
+    hL = rng.uniform(0.1, 20.0, 2000)
+    hR = rng.uniform(0.1, 20.0, 2000)
+    cL, cR = np.sqrt(G * hL), np.sqrt(G * hR)
+    du = rng.uniform(0.0, 1.0, 2000) * 2 * (cL + cR)
+    # make sure both waves are rarefactions: du > 2 |cL - cR|
+    du = np.maximum(du, 2 * np.abs(cL - cR) * 1.01)
+    uL, uR = -0.5 * du, 0.5 * du
+    hm, s1m, s2m, kind, its = oracle.middle_state(hL, hR, uL, uR)
+    assert np.all(kind == 1)
+    np.testing.assert_allclose(hm, (2 * (cL + cR) - du) ** 2 / (16 * G), rtol=1e-13)
+    cm = np.sqrt(G * hm)
+    np.testing.assert_allclose(s1m, uL + 2 * cL - 3 * cm, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(s2m, uR - 2 * cR + 3 * cm, rtol=1e-12, atol=1e-12)
+
+
+def test_dry_dry_and_dry_limits(oracle):
+    z = np.zeros(4)
+    h = np.array([0.0, 0.005, 0.0099, 0.0])
+    for s in SOLVERS.values():
+        o = oracle.solve_edges(s, h, h[::-1], z, z, z, z, dtype=np.float64)
+        for a in o:
+            assert np.all(a == 0.0)
+
+
+def test_float32_close_to_float64(oracle):
+    ins = random_edges(50000, 12)
+    for name, s in SOLVERS.items():
+        o32 = oracle.solve_edges(s, *ins, dtype=np.float32)
+        o64 = oracle.solve_edges(s, *[a.astype(np.float64) for a in ins], dtype=np.float64)
+        for a, b in zip(o32, o64):
+            scale = 1.0 + np.abs(b)
+            err = np.abs(a - b) / scale
+            # single-precision round-off, except edges next to a branch threshold
+            assert np.quantile(err, 0.99) < 1e-4, name
