@@ -113,9 +113,9 @@ void fwave(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, 
     const T fL = huL * uL + T(0.5) * C::g * hL * hL;
     const T df2 = (fR - fL) + T(0.5) * C::g * (hL + hR) * (bR - bL);
     // eigen-coefficients of R = [[1,1],[s1,s2]]
-    const T ds = s2 - s1;
-    const T beta1 = (s2 * df1 - df2) / ds;
-    const T beta2 = (df2 - s1 * df1) / ds;
+    const T invDs = T(1) / (s2 - s1);
+    const T beta1 = (s2 * df1 - df2) * invDs;
+    const T beta2 = (df2 - s1 * df1) * invDs;
     accumulate_wave(s1, beta1, beta1 * s1, hNetL, hNetR, huNetL, huNetR);
     accumulate_wave(s2, beta2, beta2 * s2, hNetL, hNetR, huNetL, huNetR);
     // the dry side receives nothing
@@ -134,6 +134,19 @@ void fwave(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, 
 // solver for the shallow water equations).  Returns the middle depth hm and
 // the speeds s1m (1-wave tail/shock at the middle state) and s2m.
 // kind: 0 dry side, 1 two rarefactions, 2 two shocks, 3 shock+rarefaction.
+//
+// The depth solves phi(h) = du + f(h; hMin) + f(h; hMax) = 0 with the wave
+// curves f(h; k) = (h - k) gk(h), gk(h) = sqrt(g/2 (1/h + 1/k)) (shock,
+// h > k) and f(h; k) = 2 (sqrt(g h) - sqrt(g k)) (rarefaction).
+//  * two rarefactions: closed form;
+//  * two shocks: Newton from h = hMax, both curves shocks;
+//  * shock + rarefaction: Newton from h = hMin, shock towards hMin.
+// phi is increasing and concave on each search interval, so Newton from the
+// left end increases monotonically to the root.  The derivative's two
+// quotients share one division:
+//   (h-hMin)/gMin + (h-hMax)/gMax = ((h-hMin) gMax + (h-hMax) gMin) / (gMin gMax).
+// After convergence (|step| <= newtonTol h) the middle velocities use the
+// last iteration's gk.
 // ---------------------------------------------------------------------------
 template <typename T>
 int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nullptr) {
@@ -153,8 +166,12 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
     } else {
         const T cMin = std::sqrt(C::g * hMin);
         const T cMax = std::sqrt(C::g * hMax);
+        const T invMin = T(1) / hMin;
+        const T invMax = T(1) / hMax;
+        const T halfG = T(0.5) * C::g;
+        const T quarterG = T(0.25) * C::g;
         const T fMin = du + T(2) * (cMin - cMax);
-        const T fMax = du + (hMax - hMin) * std::sqrt(T(0.5) * C::g * (hMax + hMin) / (hMax * hMin));
+        const T fMax = du + (hMax - hMin) * std::sqrt(halfG * (invMax + invMin));
         if (fMin > T(0)) {
             // two rarefactions: closed form
             const T a = std::max(T(0), -du + T(2) * (cL + cR));
@@ -164,15 +181,16 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
             s2m = uR - T(2) * cR + T(3) * cm;
             kind = 1;
         } else if (fMax <= T(0)) {
-            // two shocks: Newton on phi(h) = du + (h-hL) gL(h) + (h-hR) gR(h), from h = hMax.
-            // phi is increasing and concave for h >= hMax, so the iterates increase monotonically.
-            T h = hMax;
+            // two shocks
+            T h = hMax, gMin = 0, gMax = 0;
             for (it = 0; it < C::maxNewton; ++it) {
-                const T gL = std::sqrt(T(0.5) * C::g * (T(1) / h + T(1) / hL));
-                const T gR = std::sqrt(T(0.5) * C::g * (T(1) / h + T(1) / hR));
-                const T phi = du + (h - hL) * gL + (h - hR) * gR;
-                const T dphi = gL - C::g * (h - hL) / (T(4) * h * h * gL) + gR -
-                               C::g * (h - hR) / (T(4) * h * h * gR);
+                const T invH = T(1) / h;
+                gMin = std::sqrt(halfG * (invH + invMin));
+                gMax = std::sqrt(halfG * (invH + invMax));
+                const T phi = (du + (h - hMin) * gMin) + (h - hMax) * gMax;
+                const T num = (h - hMin) * gMax + (h - hMax) * gMin;
+                const T den = gMin * gMax;
+                const T dphi = (gMin + gMax) - quarterG * invH * invH * (num / den);
                 const T step = phi / dphi;
                 h = h - step;
                 if (std::fabs(step) <= C::newtonTol * h) {
@@ -181,23 +199,26 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
                 }
             }
             hm = h;
-            const T gmL = std::sqrt(T(0.5) * C::g * (T(1) / hm + T(1) / hL));
-            const T gmR = std::sqrt(T(0.5) * C::g * (T(1) / hm + T(1) / hR));
-            const T u1m = uL - (hm - hL) * gmL;
-            const T u2m = uR + (hm - hR) * gmR;
+            const bool leftIsMin = !(hR < hL);
+            const T gL = leftIsMin ? gMin : gMax;
+            const T gR = leftIsMin ? gMax : gMin;
+            const T u1m = uL - (hm - hL) * gL;
+            const T u2m = uR + (hm - hR) * gR;
             const T cm = std::sqrt(C::g * hm);
             s1m = u1m - cm;
             s2m = u2m + cm;
             kind = 2;
         } else {
-            // one shock, one rarefaction: Newton on
-            // phi(h) = du + 2 (sqrt(g h) - sqrt(g hMax)) + (h - hMin) gMin(h), from h = hMin.
+            // shock towards hMin, rarefaction towards hMax
             T h = hMin;
             for (it = 0; it < C::maxNewton; ++it) {
-                const T gm = std::sqrt(T(0.5) * C::g * (T(1) / h + T(1) / hMin));
+                const T invH = T(1) / h;
+                const T gMin = std::sqrt(halfG * (invH + invMin));
                 const T ch = std::sqrt(C::g * h);
-                const T phi = du + T(2) * (ch - cMax) + (h - hMin) * gm;
-                const T dphi = ch / h + gm - C::g * (h - hMin) / (T(4) * h * h * gm);
+                const T phi = (du + (h - hMin) * gMin) + T(2) * (ch - cMax);
+                const T num = h - hMin;
+                const T den = gMin;
+                const T dphi = (gMin + ch * invH) - quarterG * invH * invH * (num / den);
                 const T step = phi / dphi;
                 h = h - step;
                 if (std::fabs(step) <= C::newtonTol * h) {
@@ -298,8 +319,9 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
     const T l3 = std::max(sE2, s1m);
     const T l2 = T(0.5) * (l1 + l3);
     const T dl = l3 - l1;
+    const T invDl = T(1) / dl;
 
-    const T hStarHLL = std::max((huL - huR + l3 * hR - l1 * hL) / dl, T(0));
+    const T hStarHLL = std::max((huL - huR + l3 * hR - l1 * hL) * invDl, T(0));
 
     // steady-state wave (one pass)
     const T dh = hR - hL;
@@ -325,19 +347,26 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
     if ((uL + cL) * (uR + cR) < T(0)) sonic = true;
     if ((uL - cL) * (uR - cR) < T(0)) sonic = true;
 
-    T ddh = sonic ? -db : db * C::g * hBar / s1s2bar;
+    // the steady-state jumps share one reciprocal of s1s2bar; the depth
+    // bounds hStarHLL dl / l3 and hStarHLL dl / l1 share one of l1 l3
+    const T invBar = T(1) / s1s2bar;
+    T ddh = sonic ? -db : db * C::g * hBar * invBar;
+    const T hsd = hStarHLL * dl;
     if (l1 < -ct && l3 > ct) {
-        ddh = std::min(ddh, hStarHLL * dl / l3);
-        ddh = std::max(ddh, hStarHLL * dl / l1);
+        const T inv13 = T(1) / (l1 * l3);
+        ddh = std::min(ddh, hsd * l1 * inv13);
+        ddh = std::max(ddh, hsd * l3 * inv13);
     } else if (l1 >= ct) {
-        ddh = std::min(ddh, hStarHLL * dl / l1);
+        const T inv13 = T(1) / (l1 * l3);
+        ddh = std::min(ddh, hsd * l3 * inv13);
         ddh = std::max(ddh, -hL);
     } else if (l3 <= -ct) {
+        const T inv13 = T(1) / (l1 * l3);
         ddh = std::min(ddh, hR);
-        ddh = std::max(ddh, hStarHLL * dl / l3);
+        ddh = std::max(ddh, hsd * l1 * inv13);
     }
 
-    T ddphi = sonic ? -C::g * hBar * db : -db * C::g * hBar * s1s2tilde / s1s2bar;
+    T ddphi = sonic ? -C::g * hBar * db : -db * C::g * hBar * s1s2tilde * invBar;
     const T pL = -hL * db;
     const T pR = -hR * db;
     ddphi = std::min(ddphi, C::g * std::max(pL, pR));
@@ -350,8 +379,8 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
     // eigenvectors r1 = (1, l1, l1^2), r2 = (0, 0, 1), r3 = (1, l3, l3^2)
     const T l1sq = l1 * l1;
     const T l3sq = l3 * l3;
-    const T beta1 = (l3 * del1 - del2) / dl;
-    const T beta3 = (del2 - l1 * del1) / dl;
+    const T beta1 = (l3 * del1 - del2) * invDl;
+    const T beta3 = (del2 - l1 * del1) * invDl;
     const T beta2 = del3 - (beta1 * l1sq + beta3 * l3sq);
 
     T z1h = beta1 * l1, z1hu = beta1 * l1sq, s1 = l1;
@@ -413,12 +442,12 @@ void hlle(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T
     // jump of the water surface h + b, so a lake at rest gives no update
     const T dq1 = (hR + bR) - (hL + bL);
     const T dq2 = huR - huL;
-    const T ds = sR - sL;
+    const T invDs = T(1) / (sR - sL);
     // HLL flux F* minus the left flux, in f-wave form:
     //   A- dQ = sL (Q* - QL),  A+ dQ = sR (QR - Q*),  sum = dF
     // with Q* chosen so that the sum is the (source-balanced) flux jump.
-    const T q1 = (sR * dq1 - df1) / ds;  // Q* - QL (h)
-    const T q2 = (sR * dq2 - df2) / ds;  // Q* - QL (hu)
+    const T q1 = (sR * dq1 - df1) * invDs;  // Q* - QL (h)
+    const T q2 = (sR * dq2 - df2) * invDs;  // Q* - QL (hu)
     hNetL = sL * q1;
     huNetL = sL * q2;
     hNetR = df1 - hNetL;

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void sweep_s_x_kernel(StateView v, SumsView su
     float carryH = 0.f, carryHu = 0.f;  // R of edge (e-1 | e), valid from the second edge on
     if (e0 >= 1) {
         const size_t p = o - v.pitch;
-        const dev::EdgeOut e = dev::EdgeSolver<S>::solve(v.h[p], ch, v.hu[p], chu, v.b[p], cb);
+        const dev::EdgeOut e = dev::solve_raw<S>(v.h[p], ch, v.hu[p], chu, v.b[p], cb);
         carryH = e.hR;
         carryHu = e.huR;
     }
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void sweep_s_x_kernel(StateView v, SumsView su
     for (int e = e0; e <= e1; ++e) {
         o += v.pitch;
         const float nh = v.h[o], nhu = v.hu[o], nb = v.b[o];
-        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(ch, nh, chu, nhu, cb, nb);
+        const dev::EdgeOut ev = dev::solve_raw<S>(ch, nh, chu, nhu, cb, nb);
         if (rowValid) smaxv = dev::smax(smaxv, ev.speed);
         // cell e is complete: L from edge (e|e+1), R from edge (e-1|e)
         if (e >= 1 && rowValid && y >= 1 && y <= v.ny) {
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void sweep_s_y_kernel(StateView v, SumsView su
         const float uh = __shfl_down(ch, 1, 64);
         const float uhv = __shfl_down(chv, 1, 64);
         const float ub = __shfl_down(cb, 1, 64);
-        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(ch, uh, chv, uhv, cb, ub);
+        const dev::EdgeOut eh = dev::solve_raw<S>(ch, uh, chv, uhv, cb, ub);
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);
         if (owned) {

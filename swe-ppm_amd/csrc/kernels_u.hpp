@@ -195,6 +195,37 @@ __device__ __forceinline__ Cell update_cell(float h, float hu, float hv, float d
     return c;
 }
 
+// One loaded cell: state plus the quantities its edges share.
+struct CellQ {
+    float h, hu, hv, b, u, v, c, sq;
+    __device__ __forceinline__ dev::Side xside() const { return dev::Side{h, hu, b, u, c, sq}; }
+    __device__ __forceinline__ dev::Side yside() const { return dev::Side{h, hv, b, v, c, sq}; }
+    // the y-side of the cell held by lane + 1 (the cell above), by shuffle
+    __device__ __forceinline__ dev::Side yside_from_lane_above() const {
+        dev::Side s;
+        s.h = __shfl_down(h, 1, 64);
+        s.hu = __shfl_down(hv, 1, 64);
+        s.b = __shfl_down(b, 1, 64);
+        s.u = __shfl_down(v, 1, 64);
+        s.c = __shfl_down(c, 1, 64);
+        s.sq = __shfl_down(sq, 1, 64);
+        return s;
+    }
+};
+
+__device__ __forceinline__ CellQ cell_q(float h, float hu, float hv, float b) {
+    CellQ q;
+    q.h = h;
+    q.hu = hu;
+    q.hv = hv;
+    q.b = b;
+    q.u = hu / h;
+    q.v = hv / h;
+    q.c = sqrtf(kGravity * h);
+    q.sq = sqrtf(h);
+    return q;
+}
+
 // Strip sweep.  One wave per (y-tile, x-segment); 4 waves per workgroup.
 // dtFixed >= 0 forces dt (the per-call API passes the host's maxTimestep);
 // otherwise dt comes from ctl->smax[par].
@@ -246,8 +277,13 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
         hvn[g] = chv;
     }
 
+    // per-cell quantities (velocities, celerity, sqrt h) are computed once
+    // when a cell is loaded and reused by its four edges
+    CellQ cq = cell_q(ch, chu, chv, cb);
+    CellQ nq = cell_q(nh, nhu, nhv, nb);
+
     // halo vertical edge (x0-1 | x0)
-    dev::EdgeOut e = dev::EdgeSolver<S>::solve(ch, nh, chu, nhu, cb, nb);
+    dev::EdgeOut e = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
 
@@ -256,21 +292,20 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
         chu = nhu;
         chv = nhv;
         cb = nb;
+        cq = nq;
         o += v.pitch;  // column x+1 (<= nx+1)
         nh = H[o];
         nhu = HU[o];
         nhv = HV[o];
         nb = B[o];
+        nq = cell_q(nh, nhu, nhv, nb);
 
         // vertical edge (x | x+1)
-        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(ch, nh, chu, nhu, cb, nb);
+        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
 
         // horizontal edge (y | y+1): upper cell from lane+1
-        const float uh = __shfl_down(ch, 1, 64);
-        const float uhv = __shfl_down(chv, 1, 64);
-        const float ub = __shfl_down(cb, 1, 64);
-        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(ch, uh, chv, uhv, cb, ub);
+        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(cq.yside(), cq.yside_from_lane_above());
         if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
         // up-going part of edge (y-1 | y) from lane-1
         const float aH = __shfl_up(eh.hR, 1, 64);

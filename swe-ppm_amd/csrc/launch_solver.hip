@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void solve_edges_kernel(int64_t n, const float
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const dev::EdgeOut e =
-        dev::EdgeSolver<S>::solve(in[k], in[n + k], in[2 * n + k], in[3 * n + k], in[4 * n + k], in[5 * n + k]);
+        dev::solve_raw<S>(in[k], in[n + k], in[2 * n + k], in[3 * n + k], in[4 * n + k], in[5 * n + k]);
     out[k] = e.hL;
     out[n + k] = e.hR;
     out[2 * n + k] = e.huL;

@@ -36,6 +36,28 @@ struct EdgeOut {
     float hL, hR, huL, huR, speed;
 };
 
+// One side of an edge: depth, normal momentum, bathymetry and the per-cell
+// quantities every edge touching the cell needs.  They are computed once per
+// cell (cell_side) and reused by its four edges; the values are the same
+// IEEE results each edge would compute itself.
+struct Side {
+    float h, hu, b;
+    float u;   // hu / h (normal velocity; meaningless for dry cells)
+    float c;   // sqrt(g h)
+    float sq;  // sqrt(h)
+};
+
+__device__ __forceinline__ Side cell_side(float h, float hu, float b) {
+    Side s;
+    s.h = h;
+    s.hu = hu;
+    s.b = b;
+    s.u = hu / h;
+    s.c = sqrtf(kGravity * h);
+    s.sq = sqrtf(h);
+    return s;
+}
+
 // Add one f-wave to the side its speed points to; a wave of (near) zero speed
 // is shared equally.  Expressed as selects of the three branch results.
 __device__ __forceinline__ void acc_wave(float s, float zh, float zhu, EdgeOut& o) {
@@ -53,69 +75,84 @@ __device__ __forceinline__ void acc_wave(float s, float zh, float zhu, EdgeOut& 
     o.huR = huR;
 }
 
+// A dry neighbour of an f-wave / HLLE edge is a reflecting wall: the wet
+// state mirrored (h, -hu, b; the velocity negates exactly).
+struct Mirrored {
+    Side L, R;
+    bool dryL, dryR, mirrorR;
+};
+__device__ __forceinline__ Mirrored mirror_dry(const Side& L, const Side& R) {
+    Mirrored m;
+    m.dryL = L.h < kDryTol;
+    m.dryR = R.h < kDryTol;
+    m.mirrorR = !m.dryL && m.dryR;
+    m.L.h = m.dryL ? R.h : L.h;
+    m.L.hu = m.dryL ? -R.hu : L.hu;
+    m.L.b = m.dryL ? R.b : L.b;
+    m.L.u = m.dryL ? -R.u : L.u;
+    m.L.c = m.dryL ? R.c : L.c;
+    m.L.sq = m.dryL ? R.sq : L.sq;
+    m.R.h = m.mirrorR ? L.h : R.h;
+    m.R.hu = m.mirrorR ? -L.hu : R.hu;
+    m.R.b = m.mirrorR ? L.b : R.b;
+    m.R.u = m.mirrorR ? -L.u : R.u;
+    m.R.c = m.mirrorR ? L.c : R.c;
+    m.R.sq = m.mirrorR ? L.sq : R.sq;
+    return m;
+}
+
 // ---------------------------------------------------------------------------
 // f-wave
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ EdgeOut fwave(float hL, float hR, float huL, float huR, float bL, float bR) {
-    const bool dryL = hL < kDryTol;
-    const bool dryR = hR < kDryTol;
-    const bool mirrorR = !dryL && dryR;
-    // a dry neighbour is a reflecting wall: mirror the wet state
-    const float hl = dryL ? hR : hL;
-    const float hul = dryL ? -huR : huL;
-    const float bl = dryL ? bR : bL;
-    const float hr = mirrorR ? hL : hR;
-    const float hur = mirrorR ? -huL : huR;
-    const float br = mirrorR ? bL : bR;
-
-    const float uL = hul / hl;
-    const float uR = hur / hr;
-    const float sqhL = sqrtf(hl);
-    const float sqhR = sqrtf(hr);
-    const float cL = sqrtf(kGravity * hl);
-    const float cR = sqrtf(kGravity * hr);
-    const float hRoe = 0.5f * (hl + hr);
-    const float uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
+__device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
+    const Mirrored m = mirror_dry(Lin, Rin);
+    const Side& L = m.L;
+    const Side& R = m.R;
+    const float hRoe = 0.5f * (L.h + R.h);
+    const float uRoe = (L.u * L.sq + R.u * R.sq) / (L.sq + R.sq);
     const float cRoe = sqrtf(kGravity * hRoe);
-    const float s1 = smin(uL - cL, uRoe - cRoe);
-    const float s2 = smax(uR + cR, uRoe + cRoe);
+    const float s1 = smin(L.u - L.c, uRoe - cRoe);
+    const float s2 = smax(R.u + R.c, uRoe + cRoe);
 
-    const float df1 = hur - hul;
-    const float fR = hur * uR + 0.5f * kGravity * hr * hr;
-    const float fL = hul * uL + 0.5f * kGravity * hl * hl;
-    const float df2 = (fR - fL) + 0.5f * kGravity * (hl + hr) * (br - bl);
-    const float ds = s2 - s1;
-    const float beta1 = (s2 * df1 - df2) / ds;
-    const float beta2 = (df2 - s1 * df1) / ds;
+    const float df1 = R.hu - L.hu;
+    const float fR = R.hu * R.u + 0.5f * kGravity * R.h * R.h;
+    const float fL = L.hu * L.u + 0.5f * kGravity * L.h * L.h;
+    const float df2 = (fR - fL) + 0.5f * kGravity * (L.h + R.h) * (R.b - L.b);
+    const float invDs = 1.f / (s2 - s1);
+    const float beta1 = (s2 * df1 - df2) * invDs;
+    const float beta2 = (df2 - s1 * df1) * invDs;
 
     EdgeOut o{0.f, 0.f, 0.f, 0.f, 0.f};
     acc_wave(s1, beta1, beta1 * s1, o);
     acc_wave(s2, beta2, beta2 * s2, o);
-    // the dry side receives nothing
-    o.hL = dryL ? 0.f : o.hL;
-    o.huL = dryL ? 0.f : o.huL;
-    o.hR = mirrorR ? 0.f : o.hR;
-    o.huR = mirrorR ? 0.f : o.huR;
+    o.hL = m.dryL ? 0.f : o.hL;
+    o.huL = m.dryL ? 0.f : o.huL;
+    o.hR = m.mirrorR ? 0.f : o.hR;
+    o.huR = m.mirrorR ? 0.f : o.huR;
     o.speed = smax(fabsf(s1), fabsf(s2));
-    if (dryL && dryR) o = EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f};
+    if (m.dryL && m.dryR) o = EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f};
     return o;
 }
 
 // ---------------------------------------------------------------------------
-// Middle state of the Riemann problem: closed form for two rarefactions,
-// Newton (monotone from the left: the wave curves are increasing and concave)
-// for two shocks and for shock + rarefaction.
+// Middle state of the Riemann problem: phi(h) = du + f(h; hMin) + f(h; hMax)
+// with shock curves (h-k) sqrt(g/2 (1/h + 1/k)) and rarefaction curves
+// 2 (sqrt(g h) - sqrt(g k)).  Two rarefactions: closed form.  Otherwise one
+// branch-free Newton loop serves both the two-shock case (from hMax) and the
+// shock+rarefaction case (from hMin): the curve towards hMax is selected per
+// lane, and its two square roots share one instruction slot.  The
+// derivative's two quotients share one division.  phi is increasing and
+// concave on the search interval, so the iterates increase monotonically.
+// cL, cR are sqrt(g hL), sqrt(g hR).
 // ---------------------------------------------------------------------------
 struct Middle {
     float hm, s1m, s2m;
 };
 
-__device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, float uR) {
+__device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, float uR, float cL, float cR) {
     const float hMin = smin(hL, hR);
     const float hMax = smax(hL, hR);
     const float du = uR - uL;
-    const float cL = sqrtf(kGravity * hL);
-    const float cR = sqrtf(kGravity * hR);
     Middle m;
     if (hMin < kDryTol) {
         m.hm = 0.f;
@@ -123,10 +160,14 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
         m.s2m = m.s1m;
         return m;
     }
-    const float cMin = sqrtf(kGravity * hMin);
-    const float cMax = sqrtf(kGravity * hMax);
+    const float cMin = (hR < hL) ? cR : cL;
+    const float cMax = (hL < hR) ? cR : cL;
+    const float invMin = 1.f / hMin;
+    const float invMax = 1.f / hMax;
+    const float halfG = 0.5f * kGravity;
+    const float quarterG = 0.25f * kGravity;
     const float fMin = du + 2.f * (cMin - cMax);
-    const float fMax = du + (hMax - hMin) * sqrtf(0.5f * kGravity * (hMax + hMin) / (hMax * hMin));
+    const float fMax = du + (hMax - hMin) * sqrtf(halfG * (invMax + invMin));
     if (fMin > 0.f) {
         const float a = smax(0.f, -du + 2.f * (cL + cR));
         m.hm = a * a / (16.f * kGravity);
@@ -137,36 +178,31 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
     }
     const bool shocks = fMax <= 0.f;
     float h = shocks ? hMax : hMin;
-    const float invL = 1.f / hL;
-    const float invR = 1.f / hR;
-    const float invMin = 1.f / hMin;
+    float gMin = 0.f, s2 = 0.f;
     for (int it = 0; it < kMaxNewton; ++it) {
         const float invH = 1.f / h;
-        float step;
-        if (shocks) {
-            const float gL = sqrtf(0.5f * kGravity * (invH + invL));
-            const float gR = sqrtf(0.5f * kGravity * (invH + invR));
-            const float phi = du + (h - hL) * gL + (h - hR) * gR;
-            const float dphi =
-                gL - kGravity * (h - hL) / (4.f * h * h * gL) + gR - kGravity * (h - hR) / (4.f * h * h * gR);
-            step = phi / dphi;
-        } else {
-            const float gm = sqrtf(0.5f * kGravity * (invH + invMin));
-            const float ch = sqrtf(kGravity * h);
-            const float phi = du + 2.f * (ch - cMax) + (h - hMin) * gm;
-            const float dphi = ch / h + gm - kGravity * (h - hMin) / (4.f * h * h * gm);
-            step = phi / dphi;
-        }
+        gMin = sqrtf(halfG * (invH + invMin));
+        // shocks: gMax(h); shock+rarefaction: sqrt(g h)
+        s2 = sqrtf(shocks ? halfG * (invH + invMax) : kGravity * h);
+        const float t2 = shocks ? (h - hMax) * s2 : 2.f * (s2 - cMax);
+        const float phi = (du + (h - hMin) * gMin) + t2;
+        const float q = shocks ? s2 : 1.f;
+        const float num = (h - hMin) * q + (shocks ? (h - hMax) * gMin : 0.f);
+        const float den = gMin * q;
+        const float first = gMin + (shocks ? s2 : s2 * invH);
+        const float dphi = first - quarterG * invH * invH * (num / den);
+        const float step = phi / dphi;
         h = h - step;
         if (fabsf(step) <= kNewtonTol * h) break;
     }
     m.hm = h;
     const float cm = sqrtf(kGravity * h);
     if (shocks) {
-        const float gmL = sqrtf(0.5f * kGravity * (1.f / h + invL));
-        const float gmR = sqrtf(0.5f * kGravity * (1.f / h + invR));
-        const float u1m = uL - (h - hL) * gmL;
-        const float u2m = uR + (h - hR) * gmR;
+        const bool leftIsMin = !(hR < hL);
+        const float gL = leftIsMin ? gMin : s2;
+        const float gR = leftIsMin ? s2 : gMin;
+        const float u1m = uL - (h - hL) * gL;
+        const float u2m = uR + (h - hR) * gR;
         m.s1m = u1m - cm;
         m.s2m = u2m + cm;
     } else if (hL > hR) {
@@ -185,47 +221,52 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
 // middle_state is the default, the learned surrogate plugs in here.
 // ---------------------------------------------------------------------------
 struct ExactMiddle {
-    __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR) const {
-        return middle_state(hL, hR, uL, uR);
+    __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR) const {
+        return middle_state(hL, hR, uL, uR, cL, cR);
     }
 };
 
 template <typename MiddleFn>
-__device__ __forceinline__ EdgeOut augrie_with(float hL, float hR, float huL, float huR, float bL, float bR,
-                                               const MiddleFn& middle) {
-    const bool dryBoth = hL < kDryTol && hR < kDryTol;
-    const bool dryLin = hL < kDryTol;
-    const bool dryRin = hR < kDryTol;
-    hL = dryLin ? 0.f : hL;
-    huL = dryLin ? 0.f : huL;
-    hR = dryRin ? 0.f : hR;
-    huR = dryRin ? 0.f : huR;
-    float uL = dryLin ? 0.f : huL / hL;
-    float uR = dryRin ? 0.f : huR / hR;
+__device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin, const MiddleFn& middle) {
+    const bool dryLin = Lin.h < kDryTol;
+    const bool dryRin = Rin.h < kDryTol;
+    const bool dryBoth = dryLin && dryRin;
+    float hL = dryLin ? 0.f : Lin.h;
+    float huL = dryLin ? 0.f : Lin.hu;
+    float uL = dryLin ? 0.f : Lin.u;
+    float cL = dryLin ? 0.f : Lin.c;
+    float bL = Lin.b;
+    float hR = dryRin ? 0.f : Rin.h;
+    float huR = dryRin ? 0.f : Rin.hu;
+    float uR = dryRin ? 0.f : Rin.u;
+    float cR = dryRin ? 0.f : Rin.c;
+    float bR = Rin.b;
 
     // wet/dry interface: wall or inundation
     bool wallL = false, wallR = false;
     if ((dryLin || dryRin) && !dryBoth) {
         if (dryRin) {
-            const Middle t = middle_state(hL, hL, uL, -uL);
+            const Middle t = middle_state(hL, hL, uL, -uL, cL, cL);
             const float hTest = smax(hL, t.hm);
             if (hTest + bL < bR) {
                 wallR = true;
                 hR = hL;
                 huR = -huL;
                 uR = -uL;
+                cR = cL;
                 bR = bL;
             } else if (hL + bL < bR) {
                 bR = hL + bL;
             }
         } else {
-            const Middle t = middle_state(hR, hR, -uR, uR);
+            const Middle t = middle_state(hR, hR, -uR, uR, cR, cR);
             const float hTest = smax(hR, t.hm);
             if (hTest + bR < bL) {
                 wallL = true;
                 hL = hR;
                 huL = -huR;
                 uL = -uR;
+                cL = cR;
                 bL = bR;
             } else if (hR + bR < bL) {
                 bL = hR + bR;
@@ -233,20 +274,19 @@ __device__ __forceinline__ EdgeOut augrie_with(float hL, float hR, float huL, fl
         }
     }
 
-    const float cL = sqrtf(kGravity * hL);
-    const float cR = sqrtf(kGravity * hR);
     const float uHat = (cL * uL + cR * uR) / (cL + cR);
     const float cHat = sqrtf(kGravity * 0.5f * (hR + hL));
     const float sE1 = smin(uL - cL, uHat - cHat);
     const float sE2 = smax(uR + cR, uHat + cHat);
 
-    const Middle mid = middle(hL, hR, uL, uR);
+    const Middle mid = middle(hL, hR, uL, uR, cL, cR);
 
     const float l1 = smin(sE1, mid.s2m);
     const float l3 = smax(sE2, mid.s1m);
     const float l2 = 0.5f * (l1 + l3);
     const float dl = l3 - l1;
-    const float hStarHLL = smax((huL - huR + l3 * hR - l1 * hL) / dl, 0.f);
+    const float invDl = 1.f / dl;
+    const float hStarHLL = smax((huL - huR + l3 * hR - l1 * hL) * invDl, 0.f);
 
     const float dh = hR - hL;
     const float dhu = huR - huL;
@@ -266,20 +306,23 @@ __device__ __forceinline__ EdgeOut augrie_with(float hL, float hR, float huL, fl
                        ((l3 > 0.f) && (mid.s2m < 0.f)) || ((uL + cL) * (uR + cR) < 0.f) ||
                        ((uL - cL) * (uR - cR) < 0.f);
 
-    float ddh = sonic ? -db : db * kGravity * hBar / s1s2bar;
+    const float invBar = 1.f / s1s2bar;
+    float ddh = sonic ? -db : db * kGravity * hBar * invBar;
     const bool caseA = (l1 < -ct) && (l3 > ct);
     const bool caseB = !caseA && (l1 >= ct);
     const bool caseC = !caseA && !caseB && (l3 <= -ct);
     if (caseA || caseB || caseC) {
-        const float dA = hStarHLL * dl / l3;
-        const float dB = hStarHLL * dl / l1;
+        const float hsd = hStarHLL * dl;
+        const float inv13 = 1.f / (l1 * l3);
+        const float dA = hsd * l1 * inv13;  // hStarHLL dl / l3
+        const float dB = hsd * l3 * inv13;  // hStarHLL dl / l1
         const float hiBound = caseA ? dA : (caseB ? dB : hR);
         const float loBound = caseA ? dB : (caseB ? -hL : dA);
         ddh = smin(ddh, hiBound);
         ddh = smax(ddh, loBound);
     }
 
-    float ddphi = sonic ? -kGravity * hBar * db : -db * kGravity * hBar * s1s2tilde / s1s2bar;
+    float ddphi = sonic ? -kGravity * hBar * db : -db * kGravity * hBar * s1s2tilde * invBar;
     const float pL = -hL * db;
     const float pR = -hR * db;
     ddphi = smin(ddphi, kGravity * smax(pL, pR));
@@ -291,8 +334,8 @@ __device__ __forceinline__ EdgeOut augrie_with(float hL, float hR, float huL, fl
 
     const float l1sq = l1 * l1;
     const float l3sq = l3 * l3;
-    const float beta1 = (l3 * del1 - del2) / dl;
-    const float beta3 = (del2 - l1 * del1) / dl;
+    const float beta1 = (l3 * del1 - del2) * invDl;
+    const float beta3 = (del2 - l1 * del1) * invDl;
     const float beta2 = del3 - (beta1 * l1sq + beta3 * l3sq);
 
     const bool kill1 = wallL;
@@ -316,53 +359,39 @@ __device__ __forceinline__ EdgeOut augrie_with(float hL, float hR, float huL, fl
     return o;
 }
 
-__device__ __forceinline__ EdgeOut augrie(float hL, float hR, float huL, float huR, float bL, float bR) {
-    return augrie_with(hL, hR, huL, huR, bL, bR, ExactMiddle());
-}
+__device__ __forceinline__ EdgeOut augrie(const Side& L, const Side& R) { return augrie_with(L, R, ExactMiddle()); }
 
 // ---------------------------------------------------------------------------
-// HLLE in fluctuation form
+// HLLE in fluctuation form, surface-gradient diffusion (well-balanced)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ EdgeOut hlle(float hL, float hR, float huL, float huR, float bL, float bR) {
-    const bool dryL = hL < kDryTol;
-    const bool dryR = hR < kDryTol;
-    const bool mirrorR = !dryL && dryR;
-    const float hl = dryL ? hR : hL;
-    const float hul = dryL ? -huR : huL;
-    const float bl = dryL ? bR : bL;
-    const float hr = mirrorR ? hL : hR;
-    const float hur = mirrorR ? -huL : huR;
-    const float br = mirrorR ? bL : bR;
-    const float uL = hul / hl;
-    const float uR = hur / hr;
-    const float cL = sqrtf(kGravity * hl);
-    const float cR = sqrtf(kGravity * hr);
-    const float sqhL = sqrtf(hl);
-    const float sqhR = sqrtf(hr);
-    const float uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
-    const float cRoe = sqrtf(kGravity * 0.5f * (hl + hr));
-    const float sL = smin(smin(uL - cL, uRoe - cRoe), 0.f);
-    const float sR = smax(smax(uR + cR, uRoe + cRoe), 0.f);
-    const float df1 = hur - hul;
-    const float fR = hur * uR + 0.5f * kGravity * hr * hr;
-    const float fL = hul * uL + 0.5f * kGravity * hl * hl;
-    const float df2 = (fR - fL) + 0.5f * kGravity * (hl + hr) * (br - bl);
-    const float dq1 = (hr + br) - (hl + bl);  // surface-gradient form: lake at rest stays at rest
-    const float dq2 = hur - hul;
-    const float ds = sR - sL;
-    const float q1 = (sR * dq1 - df1) / ds;
-    const float q2 = (sR * dq2 - df2) / ds;
+__device__ __forceinline__ EdgeOut hlle(const Side& Lin, const Side& Rin) {
+    const Mirrored m = mirror_dry(Lin, Rin);
+    const Side& L = m.L;
+    const Side& R = m.R;
+    const float uRoe = (L.u * L.sq + R.u * R.sq) / (L.sq + R.sq);
+    const float cRoe = sqrtf(kGravity * 0.5f * (L.h + R.h));
+    const float sL = smin(smin(L.u - L.c, uRoe - cRoe), 0.f);
+    const float sR = smax(smax(R.u + R.c, uRoe + cRoe), 0.f);
+    const float df1 = R.hu - L.hu;
+    const float fR = R.hu * R.u + 0.5f * kGravity * R.h * R.h;
+    const float fL = L.hu * L.u + 0.5f * kGravity * L.h * L.h;
+    const float df2 = (fR - fL) + 0.5f * kGravity * (L.h + R.h) * (R.b - L.b);
+    const float dq1 = (R.h + R.b) - (L.h + L.b);  // surface-gradient form: lake at rest stays at rest
+    const float dq2 = R.hu - L.hu;
+    const float invDs = 1.f / (sR - sL);
+    const float q1 = (sR * dq1 - df1) * invDs;
+    const float q2 = (sR * dq2 - df2) * invDs;
     EdgeOut o;
     o.hL = sL * q1;
     o.huL = sL * q2;
     o.hR = df1 - o.hL;
     o.huR = df2 - o.huL;
-    o.hL = dryL ? 0.f : o.hL;
-    o.huL = dryL ? 0.f : o.huL;
-    o.hR = mirrorR ? 0.f : o.hR;
-    o.huR = mirrorR ? 0.f : o.huR;
+    o.hL = m.dryL ? 0.f : o.hL;
+    o.huL = m.dryL ? 0.f : o.huL;
+    o.hR = m.mirrorR ? 0.f : o.hR;
+    o.huR = m.mirrorR ? 0.f : o.huR;
     o.speed = smax(fabsf(sL), fabsf(sR));
-    if (dryL && dryR) o = EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f};
+    if (m.dryL && m.dryR) o = EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f};
     return o;
 }
 
@@ -371,29 +400,29 @@ struct EdgeSolver;
 
 template <>
 struct EdgeSolver<SOLVER_FWAVE> {
-    __device__ __forceinline__ static EdgeOut solve(float hL, float hR, float huL, float huR, float bL, float bR) {
-        return fwave(hL, hR, huL, huR, bL, bR);
-    }
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) { return fwave(L, R); }
 };
 template <>
 struct EdgeSolver<SOLVER_AUGRIE> {
-    __device__ __forceinline__ static EdgeOut solve(float hL, float hR, float huL, float huR, float bL, float bR) {
-        return augrie(hL, hR, huL, huR, bL, bR);
-    }
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) { return augrie(L, R); }
 };
 template <>
 struct EdgeSolver<SOLVER_HLLE> {
-    __device__ __forceinline__ static EdgeOut solve(float hL, float hR, float huL, float huR, float bL, float bR) {
-        return hlle(hL, hR, huL, huR, bL, bR);
-    }
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) { return hlle(L, R); }
 };
 template <>
 struct EdgeSolver<SOLVER_HYBRID> {
-    __device__ __forceinline__ static EdgeOut solve(float hL, float hR, float huL, float huR, float bL, float bR) {
-        if (hL >= kHybridTol && hR >= kHybridTol) return fwave(hL, hR, huL, huR, bL, bR);
-        return augrie(hL, hR, huL, huR, bL, bR);
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) {
+        if (L.h >= kHybridTol && R.h >= kHybridTol) return fwave(L, R);
+        return augrie(L, R);
     }
 };
+
+// Convenience for raw edge states (one-off evaluations).
+template <int S>
+__device__ __forceinline__ EdgeOut solve_raw(float hL, float hR, float huL, float huR, float bL, float bR) {
+    return EdgeSolver<S>::solve(cell_side(hL, huL, bL), cell_side(hR, huR, bR));
+}
 
 }  // namespace dev
 }  // namespace swe

@@ -171,12 +171,15 @@ def test_middle_state_matches_exact_riemann_solution(oracle):
     np.testing.assert_allclose(hm, exact, rtol=1e-9)
     assert set(np.unique(kind)) == {1, 2, 3}
     assert np.all(its <= 10)
-    # wave-curve consistency: both families meet at one middle velocity
+    # wave-curve consistency: both families meet at one middle velocity.  The
+    # velocities use the last Newton iterate's gk, |h_k - hm| <= 1e-6 hm, so
+    # they agree to that order (a wrong gk assignment gives O(1) differences)
     cm = np.sqrt(G * hm)
     shock = kind == 2
     u_from_s1 = s1m + cm
     u_from_s2 = s2m - cm
-    np.testing.assert_allclose(u_from_s1[shock], u_from_s2[shock], rtol=1e-8, atol=1e-8)
+    scale = np.abs(uL[shock]) + np.abs(uR[shock]) + cm[shock]
+    assert np.max(np.abs(u_from_s1[shock] - u_from_s2[shock]) / scale) < 2e-6
 
 
 def test_two_rarefaction_closed_form(oracle):
