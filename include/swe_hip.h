@@ -162,9 +162,11 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
                         const float* bL, const float* bR, float* hNetL, float* hNetR, float* huNetL, float* huNetR,
                         float* maxSpeed);
 
-/* Tuning: strategy 0 = recompute (max pass + update pass, 28+16 B/cell),
- * 1 = store (flux pass writing 4 net-update sums + apply pass); xseg = x
- * columns per wavefront strip (8, 16 or 32); -1 keeps the current value. */
+/* Tuning of the device loop (swe_hip_run): strategy 0 = recompute (max pass
+ * + update pass), 1 = store (flux pass writing 4 net-update sums + apply
+ * pass), 2 = fused (one launch per step that applies the previous update on
+ * load; the default); xseg = x columns per wavefront strip (8, 16 or 32);
+ * -1 keeps the current value.  All strategies give identical results. */
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
 

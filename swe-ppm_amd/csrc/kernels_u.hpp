@@ -64,11 +64,18 @@ struct BoundaryInfo {
 // alternately so that a step can clear the next one without a memset),
 // the current dt and simulated time and a per-step dt log.
 struct DeviceCtl {
-    unsigned int smax[2];  // float bits of non-negative speeds: unsigned order == float order
+    unsigned int smax[3];  // float bits of non-negative speeds: unsigned order == float order
     float dt;
     float t;
     int steps;
-    int pad[3];
+    int pad[2];
+};
+
+// Which CFL-maximum slot a launch reads or accumulates into, and which slot
+// it clears for a later step (the last reader of that slot has finished).
+struct Slots {
+    int use;
+    int clear;
 };
 
 __device__ __forceinline__ size_t idx(const StateView& v, int x, int y) { return (size_t)x * v.pitch + y; }
@@ -231,7 +238,7 @@ __device__ __forceinline__ CellQ cell_q(float h, float hu, float hv, float b) {
 // otherwise dt comes from ctl->smax[par].
 template <int S, int MODE, int XS, int V>
 __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                                                       BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+                                                       BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed,
                                                        int fuseBc, float* dtLog, int logIdx) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
@@ -251,7 +258,7 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
 
     float dtdx = 0.f, dtdy = 0.f, dt = 0.f;
     if (MODE == MODE_UPDATE) {
-        dt = dtFixed >= 0.f ? dtFixed : timestep_from_max(__uint_as_float(ctl->smax[par]), v.dx, v.dy);
+        dt = dtFixed >= 0.f ? dtFixed : timestep_from_max(__uint_as_float(ctl->smax[sl.use]), v.dx, v.dy);
         dtdx = dt / v.dx;
         dtdy = dt / v.dy;
     }
@@ -349,10 +356,10 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
 
     if (MODE != MODE_UPDATE) {
         smaxv = wave_max(smaxv);
-        if (lane == 0) atomicMax(&ctl->smax[par], __float_as_uint(smaxv));
+        if (lane == 0) atomicMax(&ctl->smax[sl.use], __float_as_uint(smaxv));
     } else {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            ctl->smax[par ^ 1] = 0u;  // next step's maximum starts from zero
+            ctl->smax[sl.clear] = 0u;  // a later step's maximum starts from zero
             if (dtFixed < 0.f) {
                 ctl->dt = dt;
                 ctl->t += dt;
@@ -366,12 +373,12 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
 // Apply stored net-update sums in place (second half of the store strategy).
 template <int XS, int V>
 __global__ __launch_bounds__(256) void apply_u_kernel(StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl,
-                                                       int par, float dtFixed, int fuseBc, float* dtLog,
+                                                       Slots sl, float dtFixed, int fuseBc, float* dtLog,
                                                        int logIdx) {
     const int y = blockIdx.x * 64 + (threadIdx.x & 63) + 1;
     const int xseg = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int x0 = 1 + xseg * XS;
-    const float dt = dtFixed >= 0.f ? dtFixed : timestep_from_max(__uint_as_float(ctl->smax[par]), v.dx, v.dy);
+    const float dt = dtFixed >= 0.f ? dtFixed : timestep_from_max(__uint_as_float(ctl->smax[sl.use]), v.dx, v.dy);
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;
     if (y <= v.ny && x0 <= v.nx) {
@@ -387,7 +394,7 @@ __global__ __launch_bounds__(256) void apply_u_kernel(StateView v, SumsView sums
         }
     }
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        ctl->smax[par ^ 1] = 0u;
+        ctl->smax[sl.clear] = 0u;
         if (dtFixed < 0.f) {
             ctl->dt = dt;
             ctl->t += dt;

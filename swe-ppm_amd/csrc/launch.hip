@@ -96,20 +96,20 @@ __global__ __launch_bounds__(256) void apply_s_kernel(StateView v, SumsView sums
 
 template <int V>
 static int sweep_u_v(int solver, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                     BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx,
+                     BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog, int logIdx,
                      hipStream_t stream) {
     switch (solver) {
         case SOLVER_HLLE:
-            return launch_sweep_u_solver<SOLVER_HLLE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+            return launch_sweep_u_solver<SOLVER_HLLE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed,
                                                          fuseBc, dtLog, logIdx, stream);
         case SOLVER_FWAVE:
-            return launch_sweep_u_solver<SOLVER_FWAVE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+            return launch_sweep_u_solver<SOLVER_FWAVE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed,
                                                           fuseBc, dtLog, logIdx, stream);
         case SOLVER_AUGRIE:
-            return launch_sweep_u_solver<SOLVER_AUGRIE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+            return launch_sweep_u_solver<SOLVER_AUGRIE, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed,
                                                            fuseBc, dtLog, logIdx, stream);
         case SOLVER_HYBRID:
-            return launch_sweep_u_solver<SOLVER_HYBRID, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed,
+            return launch_sweep_u_solver<SOLVER_HYBRID, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed,
                                                            fuseBc, dtLog, logIdx, stream);
         default:
             set_error("unknown solver %d", solver);
@@ -118,19 +118,19 @@ static int sweep_u_v(int solver, int mode, int xseg, StateView v, SumsView sums,
 }
 
 int launch_sweep_u(int solver, int variant, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun,
-                   float* hvn, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog,
+                   float* hvn, BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog,
                    int logIdx, hipStream_t stream) {
     if (variant == 1)
-        return sweep_u_v<1>(solver, mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
+        return sweep_u_v<1>(solver, mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx,
                             stream);
-    return sweep_u_v<0>(solver, mode, xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
+    return sweep_u_v<0>(solver, mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx,
                         stream);
 }
 
-int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed,
                    int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
-    if (variant == 1) return launch_apply_u_variant<1>(v, sums, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
-    return launch_apply_u_variant<0>(v, sums, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+    if (variant == 1) return launch_apply_u_variant<1>(v, sums, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
+    return launch_apply_u_variant<0>(v, sums, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
 }
 
 template <int V>
@@ -145,6 +145,31 @@ static int solve_edges_v(int solver, int64_t n, const float* in, float* out, hip
         default:
             return launch_solve_edges_solver<SOLVER_HYBRID, V>(n, in, out, stream);
     }
+}
+
+template <int V>
+static int fused_u_v(int solver, int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl, int slotIn,
+                     int slotOut, int slotClear, float* dtLog, int logIdx, hipStream_t stream) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            return launch_fused_u_solver<SOLVER_HLLE, V>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
+                                                         logIdx, stream);
+        case SOLVER_FWAVE:
+            return launch_fused_u_solver<SOLVER_FWAVE, V>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
+                                                          logIdx, stream);
+        case SOLVER_AUGRIE:
+            return launch_fused_u_solver<SOLVER_AUGRIE, V>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
+                                                           logIdx, stream);
+        default:
+            return launch_fused_u_solver<SOLVER_HYBRID, V>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
+                                                           logIdx, stream);
+    }
+}
+
+int launch_fused_u(int solver, int variant, int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl,
+                   int slotIn, int slotOut, int slotClear, float* dtLog, int logIdx, hipStream_t stream) {
+    if (variant == 1) return fused_u_v<1>(solver, xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog, logIdx, stream);
+    return fused_u_v<0>(solver, xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog, logIdx, stream);
 }
 
 int launch_solve_edges(int solver, int variant, int64_t n, const float* in, float* out, hipStream_t stream) {

@@ -10,6 +10,7 @@
 // instantiations never share a symbol.
 #include <hip/hip_runtime.h>
 
+#include "kernels_fused.hpp"
 #include "kernels_s.hpp"
 #include "kernels_u.hpp"
 #include "swe_internal.h"
@@ -26,14 +27,14 @@ namespace swe {
 namespace {
 
 template <int S, int MODE, int XS, int V>
-int go_u(StateView v, SumsView sums, float* hn, float* hun, float* hvn, BoundaryInfo bi, DeviceCtl* ctl, int par,
+int go_u(StateView v, SumsView sums, float* hn, float* hun, float* hvn, BoundaryInfo bi, DeviceCtl* ctl, Slots sl,
          float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
     const int nyt = (v.ny + kStripRows - 1) / kStripRows;
     const int nxs = (v.nx + XS - 1) / XS;
     const int waves = nyt * nxs;
     const int groups = (waves + kWavesPerGroup - 1) / kWavesPerGroup;
     hipLaunchKernelGGL((sweep_u_kernel<S, MODE, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v,
-                       sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx);
+                       sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "sweep_u_kernel launch");
     return 0;
@@ -41,14 +42,14 @@ int go_u(StateView v, SumsView sums, float* hn, float* hun, float* hvn, Boundary
 
 template <int S, int MODE, int V>
 int go_u_xs(int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn, BoundaryInfo bi,
-            DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
+            DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
     switch (xseg) {
         case 8:
-            return go_u<S, MODE, 8, V>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+            return go_u<S, MODE, 8, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
         case 32:
-            return go_u<S, MODE, 32, V>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+            return go_u<S, MODE, 32, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
         default:
-            return go_u<S, MODE, 16, V>(v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx, stream);
+            return go_u<S, MODE, 16, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
     }
 }
 
@@ -70,19 +71,41 @@ __global__ __launch_bounds__(256) void solve_edges_kernel(int64_t n, const float
 
 template <int S, int V>
 int launch_sweep_u_solver(int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                          BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog,
+                          BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog,
                           int logIdx, hipStream_t stream) {
     switch (mode) {
         case MODE_MAX:
-            return go_u_xs<S, MODE_MAX, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog, logIdx,
+            return go_u_xs<S, MODE_MAX, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx,
                                            stream);
         case MODE_STORE:
-            return go_u_xs<S, MODE_STORE, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog,
+            return go_u_xs<S, MODE_STORE, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog,
                                              logIdx, stream);
         default:
-            return go_u_xs<S, MODE_UPDATE, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, par, dtFixed, fuseBc, dtLog,
+            return go_u_xs<S, MODE_UPDATE, V>(xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog,
                                               logIdx, stream);
     }
+}
+
+template <int S, int V>
+int launch_fused_u_solver(int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl, int slotIn, int slotOut,
+                          int slotClear, float* dtLog, int logIdx, hipStream_t stream) {
+    const int nyt = (v.ny + kStripRows - 1) / kStripRows;
+    auto go = [&](auto xsTag) {
+        constexpr int XS = decltype(xsTag)::value;
+        const int nxs = (v.nx + XS - 1) / XS;
+        const int groups = (nyt * nxs + kWavesPerGroup - 1) / kWavesPerGroup;
+        hipLaunchKernelGGL((fused_u_kernel<S, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v, io, bi,
+                           ctl, slotIn, slotOut, slotClear, dtLog, logIdx);
+    };
+    if (xseg == 16)
+        go(std::integral_constant<int, 16>());
+    else if (xseg == 32)
+        go(std::integral_constant<int, 32>());
+    else
+        go(std::integral_constant<int, 8>());
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "fused_u_kernel launch");
+    return 0;
 }
 
 template <int S, int V>
@@ -95,9 +118,11 @@ int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_
 }
 
 template int launch_sweep_u_solver<SWE_SOLVER_ID, SWE_VARIANT>(int, int, StateView, SumsView, float*, float*,
-                                                               float*, BoundaryInfo, DeviceCtl*, int, float, int,
+                                                               float*, BoundaryInfo, DeviceCtl*, Slots, float, int,
                                                                float*, int, hipStream_t);
 template int launch_solve_edges_solver<SWE_SOLVER_ID, SWE_VARIANT>(int64_t, const float*, float*, hipStream_t);
+template int launch_fused_u_solver<SWE_SOLVER_ID, SWE_VARIANT>(int, StateView, FusedIO, BoundaryInfo, DeviceCtl*, int,
+                                                               int, int, float*, int, hipStream_t);
 
 #if SWE_VARIANT == 0
 template <int S>

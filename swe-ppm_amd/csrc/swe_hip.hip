@@ -44,7 +44,7 @@ struct swe_hip_block {
     float dx = 0, dy = 0, ox = 0, oy = 0;
     int solver = SOLVER_AUGRIE;
     int flavour = FLAVOUR_U;
-    int strategy = 1;  // 0 recompute, 1 store
+    int strategy = 2;  // device loop: 0 recompute, 1 store + apply, 2 fused (apply on load)
     int math = 0;      // 0 IEEE (bit-compatible), 1 fast
     int xseg = 8;      // 8 columns per strip: most waves in flight for the VALU-bound solvers
     int cur = 0;       // index of the current state buffer
@@ -54,6 +54,7 @@ struct swe_hip_block {
     float* hv[2] = {nullptr, nullptr};
     float* b = nullptr;
     float* sums[4] = {nullptr, nullptr, nullptr, nullptr};
+    float* sums2[4] = {nullptr, nullptr, nullptr, nullptr};  // second set (fused strategy)
     DeviceCtl* ctl = nullptr;
     float* dtlog = nullptr;
     int dtlogCap = 0;
@@ -80,6 +81,7 @@ StateView view(swe_hip_block* k, int buf) {
 }
 
 SumsView sums_view(swe_hip_block* k) { return SumsView{k->sums[0], k->sums[1], k->sums[2], k->sums[3]}; }
+SumsView sums2_view(swe_hip_block* k) { return SumsView{k->sums2[0], k->sums2[1], k->sums2[2], k->sums2[3]}; }
 
 int check(swe_hip_block* k) {
     if (!k) {
@@ -96,6 +98,16 @@ int ensure_second_buffer(swe_hip_block* k) {
     HIP_TRY(hipMalloc(&k->h[o], bytes));
     HIP_TRY(hipMalloc(&k->hu[o], bytes));
     HIP_TRY(hipMalloc(&k->hv[o], bytes));
+    return 0;
+}
+
+int ensure_second_sums(swe_hip_block* k) {
+    if (k->sums2[0]) return 0;
+    const size_t bytes = field_elems(k) * sizeof(float);
+    for (int i = 0; i < 4; ++i) {
+        HIP_TRY(hipMalloc(&k->sums2[i], bytes));
+        HIP_TRY(hipMemsetAsync(k->sums2[i], 0, bytes, k->stream));
+    }
     return 0;
 }
 
@@ -130,33 +142,60 @@ struct LoopTiming {
 // update (except after the last step, so ghosts end as in the reference).
 int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log, LoopTiming* tm) {
     if (int r = ensure_dtlog(k, nsteps)) return r;
-    if (k->strategy == 0)
+    if (k->strategy == 0 || k->strategy == 2)
         if (int r = ensure_second_buffer(k)) return r;
+    if (k->strategy == 2)
+        if (int r = ensure_second_sums(k)) return r;
     float t0 = t_inout ? *t_inout : 0.f;
     HIP_TRY(hipMemcpyAsync(&k->ctl->t, &t0, sizeof(float), hipMemcpyHostToDevice, k->stream));
     if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
     if (tm) HIP_TRY(tm->mark(k->stream, 0));
     if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
-    for (int s = 0; s < nsteps; ++s) {
+    if (k->strategy == 2) {
+        // fused: sweep(STORE) on Q(0), then one launch per step that applies
+        // the previous update on load, then the last update
+        for (int slot = 1; slot < 3; ++slot)
+            if (int r = launch_ctl_reset(k->ctl, slot, k->stream)) return r;
+        SumsView set[2] = {sums_view(k), sums2_view(k)};
+        if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
+        if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), set[0], nullptr, nullptr,
+                                   nullptr, k->bi, k->ctl, Slots{0, 1}, -1.f, 0, nullptr, 0, k->stream))
+            return r;
+        for (int s = 1; s < nsteps; ++s) {
+            const int o = k->cur ^ 1;
+            FusedIO io{k->h[k->cur], k->hu[k->cur], k->hv[k->cur], k->b, k->h[o], k->hu[o], k->hv[o],
+                       set[(s - 1) & 1], set[s & 1]};
+            if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
+            if (int r = launch_fused_u(k->solver, k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl, (s - 1) % 3,
+                                       s % 3, (s + 1) % 3, k->dtlog, s - 1, k->stream))
+                return r;
+            k->cur = o;
+        }
+        if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
+        if (int r = launch_apply_u(k->math, view(k, k->cur), set[(nsteps - 1) & 1], k->bi, k->ctl,
+                                   Slots{(nsteps - 1) % 3, nsteps % 3}, -1.f, 0, k->dtlog, nsteps - 1, k->stream))
+            return r;
+    }
+    for (int s = 0; s < nsteps && k->strategy != 2; ++s) {
         const int par = s & 1;
         const int fuse = (s + 1 < nsteps) ? 1 : 0;  // the next step's setGhostLayer, fused
         if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
         if (k->strategy == 0) {
             const int o = k->cur ^ 1;
             if (int r = launch_sweep_u(k->solver, k->math, MODE_MAX, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
-                                       nullptr, k->bi, k->ctl, par, -1.f, 0, nullptr, 0, k->stream))
+                                       nullptr, k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, 0, nullptr, 0, k->stream))
                 return r;
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
             if (int r = launch_sweep_u(k->solver, k->math, MODE_UPDATE, k->xseg, view(k, k->cur), sums_view(k), k->h[o],
-                                       k->hu[o], k->hv[o], k->bi, k->ctl, par, -1.f, fuse, k->dtlog, s, k->stream))
+                                       k->hu[o], k->hv[o], k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, fuse, k->dtlog, s, k->stream))
                 return r;
             k->cur = o;
         } else {
             if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr,
-                                       nullptr, nullptr, k->bi, k->ctl, par, -1.f, 0, nullptr, 0, k->stream))
+                                       nullptr, nullptr, k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, 0, nullptr, 0, k->stream))
                 return r;
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
-            if (int r = launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, par, -1.f, fuse,
+            if (int r = launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, fuse,
                                        k->dtlog, s, k->stream))
                 return r;
         }
@@ -238,8 +277,10 @@ int swe_hip_destroy(swe_hip_block* k) {
         if (k->hv[i]) (void)hipFree(k->hv[i]);
     }
     if (k->b) (void)hipFree(k->b);
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
         if (k->sums[i]) (void)hipFree(k->sums[i]);
+        if (k->sums2[i]) (void)hipFree(k->sums2[i]);
+    }
     if (k->ctl) (void)hipFree(k->ctl);
     if (k->dtlog) (void)hipFree(k->dtlog);
     if (k->stream) (void)hipStreamDestroy(k->stream);
@@ -310,7 +351,7 @@ int swe_hip_fluxes(swe_hip_block* k, float* max_timestep) {
     if (k->flavour == FLAVOUR_U) {
         if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
         if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
-                                   nullptr, k->bi, k->ctl, 0, -1.f, 0, nullptr, 0, k->stream))
+                                   nullptr, k->bi, k->ctl, Slots{0, 1}, -1.f, 0, nullptr, 0, k->stream))
             return r;
         HIP_TRY(hipMemcpyAsync(&bits, &k->ctl->smax[0], sizeof(bits), hipMemcpyDeviceToHost, k->stream));
         HIP_TRY(hipStreamSynchronize(k->stream));
@@ -339,7 +380,7 @@ int swe_hip_update(swe_hip_block* k, float dt) {
             set_error("swe_hip_update: dt must be >= 0");
             return SWE_HIP_EINVAL;
         }
-        return launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, 0, dt, 0, nullptr, 0,
+        return launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, Slots{0, 1}, dt, 0, nullptr, 0,
                               k->stream);
     }
     // flavour S: Q* was formed with the flux pass's own dt (the reference
@@ -608,8 +649,8 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
 
 int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
     if (int r = check(k)) return r;
-    if (strategy != -1 && strategy != 0 && strategy != 1) {
-        set_error("swe_hip_set_tuning: strategy must be 0 or 1");
+    if (strategy != -1 && strategy != 0 && strategy != 1 && strategy != 2) {
+        set_error("swe_hip_set_tuning: strategy must be 0, 1 or 2");
         return SWE_HIP_EINVAL;
     }
     if (xseg != -1 && xseg != 8 && xseg != 16 && xseg != 32) {

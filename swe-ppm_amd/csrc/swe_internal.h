@@ -6,6 +6,7 @@
 
 #include <cfloat>
 
+#include "kernels_fused.hpp"
 #include "kernels_u.hpp"
 
 namespace swe {
@@ -26,9 +27,9 @@ inline float host_timestep_u(float s, float dx, float dy) {
 
 // Strip-sweep launch (flavour U), dispatched on solver, mode and segment length.
 int launch_sweep_u(int solver, int variant, int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                   BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog, int logIdx,
+                   BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog, int logIdx,
                    hipStream_t stream);
-int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed,
                    int fuseBc, float* dtLog, int logIdx, hipStream_t stream);
 
 int launch_apply_bc(StateView v, BoundaryInfo bi, hipStream_t stream);
@@ -41,15 +42,20 @@ int launch_sweep_s_y(int solver, StateView v, SumsView sums, hipStream_t stream)
 int launch_apply_s(StateView v, SumsView sums, float dt, hipStream_t stream);
 
 template <int V>
-int launch_apply_u_variant(StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed,
+int launch_apply_u_variant(StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed,
                            int fuseBc, float* dtLog, int logIdx, hipStream_t stream);
 int launch_solve_edges(int solver, int variant, int64_t n, const float* in, float* out, hipStream_t stream);
 
 // Per-solver instantiations (one translation unit each, compiled in parallel).
 template <int S, int V>
 int launch_sweep_u_solver(int mode, int xseg, StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                          BoundaryInfo bi, DeviceCtl* ctl, int par, float dtFixed, int fuseBc, float* dtLog,
+                          BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog,
                           int logIdx, hipStream_t stream);
+template <int S, int V>
+int launch_fused_u_solver(int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl, int slotIn, int slotOut,
+                          int slotClear, float* dtLog, int logIdx, hipStream_t stream);
+int launch_fused_u(int solver, int variant, int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl,
+                   int slotIn, int slotOut, int slotClear, float* dtLog, int logIdx, hipStream_t stream);
 template <int S, int V>
 int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_t stream);
 template <int S>

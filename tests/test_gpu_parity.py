@@ -57,7 +57,7 @@ def test_c1_fwave_256_200_steps(swe, oracle):
     assert abs(t - float(np.sum(gdts, dtype=np.float32))) <= 1e-3 * t
 
 
-@pytest.mark.parametrize("strategy,xseg", [(1, 16), (0, 16), (1, 8), (0, 32)])
+@pytest.mark.parametrize("strategy,xseg", [(1, 16), (0, 16), (1, 8), (0, 32), (2, 8), (2, 16), (2, 32)])
 def test_islands_augrie_parity(swe, oracle, strategy, xseg):
     from swe_ppm_amd import scenarios
 
@@ -82,7 +82,7 @@ def test_ragged_sizes_and_walls(swe, oracle, nx, ny, solver):
 
     scenarios.apply_bc(h, hu, hv, bnd)
     scenarios.apply_bc_bathymetry(b, bnd)
-    for strategy in (0, 1):
+    for strategy in (0, 1, 2):
         (gh, ghu, ghv), t, gdts = _run_gpu(swe, h, hu, hv, b, 1.0, 1.5, bnd, solver, 12, strategy)
         oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, 1.0, 1.5, bnd, solver, 12)
         _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
@@ -115,11 +115,13 @@ def test_run_ghosts_match_reference_sequence(swe, oracle):
     from swe_ppm_amd import scenarios
 
     h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_c1(64)
-    for strategy in (0, 1):
-        (gh, ghu, ghv), t, _ = _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, swe.FWAVE, 7, strategy)
-        oh, ohu, ohv, _ = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.FWAVE, 7)
-        for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
-            assert np.array_equal(a, c)
+    for bnd in ([0, 0, 0, 0], [1, 0, 1, 5], [5, 1, 0, 1]):
+        for strategy in (0, 1, 2):
+            for nsteps in (1, 2, 7):
+                (gh, ghu, ghv), t, _ = _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, swe.FWAVE, nsteps, strategy)
+                oh, ohu, ohv, _ = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.FWAVE, nsteps)
+                for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
+                    assert np.array_equal(a, c), (bnd, strategy, nsteps)
 
 
 @pytest.mark.parametrize("solver", [1, 3])
