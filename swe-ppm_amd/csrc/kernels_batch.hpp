@@ -1,0 +1,296 @@
+// kernels_batch.hpp -- batched flavour-U engine: many blocks of uneven size
+// in one device arena, driven by a device-side work queue.
+//
+// Three uses share it:
+//   * ensemble: independent instances, each with its own CFL dt and time;
+//   * one domain split into connected blocks with a global dt (the
+//     overdecomposed driver without LTS, swe_mpi_overdecomp.cpp:297-307);
+//   * multi-rate local time stepping across connected blocks (the driver's
+//     LTS loop, swe_mpi_overdecomp.cpp:263-343, with the per-block state
+//     machine of SWE_Block.hh:311-562).
+//
+// One driver iteration is five launches.  Every launch reads its parameters
+// from device memory, so a chunk of iterations is captured once in a hipGraph
+// and replayed:
+//   ghost : setGhostLayer of every block: neighbour time stamp and copy layer
+//           of CONNECT_WITHIN_RANK sides
+//           (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304), straight into
+//           the ghost line (global dt) or into a receive line (LTS); resets
+//           the work queue
+//   check : checkAllGhostlayers (SWE_Block.hh:375-436): ghost copy or
+//           interpolation from the receive line; the strip tiles of every
+//           block in sync are appended to the work queue
+//   sweep : waves pull (block, tile) items from the queue and solve the
+//           tile's edges: per-cell net-update sums + per-block CFL maximum
+//   round : per-block dt (CFL; LTS: getRoundTimestep; global: minimum over
+//           blocks), the driver's completion test (hasMaxLocalTimestep) and
+//           resetStepSizeCounter at the end of a macro step
+//   apply : waves pull the same items and update their cells, writing the
+//           OUTFLOW / WALL ghosts that the next setGhostLayer would write
+// The sweep and apply grids are persistent (sized to the machine, not to the
+// work): each wave loops until the queue is drained.  No wave ever waits for
+// another inside a launch; the launch boundaries order the phases.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "kernels_u.hpp"
+
+namespace swe {
+
+constexpr int kTileBits = 20;  // work item = block << kTileBits | tile
+constexpr int kMaxBatchBlocks = 2047;
+constexpr int kMaxDivisor = 1024;  // SWE_Block.hh:538 (maxDivisor)
+
+enum BatchMode : int { BATCH_ENSEMBLE = 0, BATCH_GLOBAL = 1, BATCH_LTS = 2 };
+
+struct BatchBlockDesc {
+    int nx, ny, pitch;
+    float dx, dy;
+    int pad0;
+    long long off;      // offset of the block's fields in the arena (floats)
+    int bnd[4];         // BoundaryType per side
+    int nbr[4];         // neighbour block index or -1
+    int nyt, ntiles;    // strip tiles: nyt * ceil(nx / xs)
+    long long line[4];  // LTS receive lines (h, hu, hv of one side) in the line arena
+};
+
+struct BatchBlockState {
+    unsigned int smax;  // CFL maximum of this iteration (float bits)
+    float maxTimestep;  // dt of this iteration's update
+    int stepSize, stepSizeCounter, timestepCounter;
+    int gl[4];
+    float borderTs[4];
+    int notified;
+    int active;  // in sync this iteration: computes and updates
+    int pad;
+    long long updates;  // updates performed
+    float t;            // simulated time of an independent block
+    int pad2;
+};
+
+struct BatchArena {
+    float *h, *hu, *hv, *b;
+    float *sx, *sy, *su, *sv;
+    float* lines;
+};
+
+struct BatchGlobals {
+    float T;         // LTS macro step (maxTimestepLocal)
+    float duration;  // simulation end (GL_DONE / notifiedLastTimestep)
+    int mode;
+    int nblocks;
+    int count;        // items in the queue
+    int head;         // next item of the sweep
+    int headApply;    // next item of the apply
+    int skip;         // this iteration does nothing (the run's target is reached)
+    int stop;         // target reached: later iterations are skipped
+    int target;       // units of this run: steps (ensemble, global) or macro steps (LTS)
+    int done;         // units completed
+    int itersUnit;    // iterations of the current macro step
+    float gdt;        // global dt of this iteration
+    int logCap;
+    long long iterations;  // iterations that did work
+};
+
+__device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d) {
+    StateView v;
+    v.h = A.h + d.off;
+    v.hu = A.hu + d.off;
+    v.hv = A.hv + d.off;
+    v.b = A.b + d.off;
+    v.nx = d.nx;
+    v.ny = d.ny;
+    v.pitch = d.pitch;
+    v.dx = d.dx;
+    v.dy = d.dy;
+    return v;
+}
+__device__ __forceinline__ SumsView batch_sums(const BatchArena& A, const BatchBlockDesc& d) {
+    return SumsView{A.sx + d.off, A.sy + d.off, A.su + d.off, A.sv + d.off};
+}
+
+// ---- LTS scalars of one block (SWE_Block.hh:311-367, 530-562) -------------
+__device__ __forceinline__ float lts_total(float T, const BatchBlockState& s) {
+    return (float)(T * s.timestepCounter) + (s.stepSize > 0 ? ((float)(T * s.stepSizeCounter) / (float)s.stepSize) : 0.f);
+}
+__device__ __forceinline__ bool lts_in_sync(const BatchBlockState& s) {
+    return s.gl[BND_LEFT] != GL_UNVALID && s.gl[BND_RIGHT] != GL_UNVALID && s.gl[BND_TOP] != GL_UNVALID &&
+           s.gl[BND_BOTTOM] != GL_UNVALID && (s.stepSizeCounter <= s.stepSize);
+}
+__device__ __forceinline__ bool lts_receivable(const BatchBlockState& s, int side) {
+    if (s.notified) return false;
+    return s.gl[side] == GL_NEXT || s.gl[side] == GL_UNVALID || s.gl[side] == GL_SYNC;
+}
+// getRoundTimestep's divisor (SWE_Block.hh:536-555): the clamp, then
+// 2^-ceil(log(ts)/log(2)) with the natural log taken in single precision and
+// the quotient in double, doubled once if 1/divisor still exceeds ts.  The
+// single-precision log is the double log rounded to float.
+__device__ __forceinline__ int lts_step_size(float T, float ts) {
+    if (ts > T) ts = T;
+    if (ts < T / kMaxDivisor) ts = T / kMaxDivisor;
+    const float lf = (float)log((double)ts);
+    int divisor = (int)ldexp(1.0, -(int)ceil((double)lf / log(2.0)));  // pow(2, -ceil(..)), exact
+    if (1.f / divisor > ts) divisor *= 2;
+    if (divisor > kMaxDivisor) divisor = kMaxDivisor;
+    return divisor;
+}
+
+// geometry of one ghost line (ghost = 1) or copy layer (ghost = 0), elements 1..n
+__device__ __forceinline__ void side_line(const BatchBlockDesc& d, int side, int ghost, long long& off, int& stride,
+                                          int& n) {
+    switch (side) {
+        case BND_LEFT:
+            off = (long long)(ghost ? 0 : 1) * d.pitch + 1;
+            stride = 1;
+            n = d.ny;
+            break;
+        case BND_RIGHT:
+            off = (long long)(ghost ? d.nx + 1 : d.nx) * d.pitch + 1;
+            stride = 1;
+            n = d.ny;
+            break;
+        case BND_BOTTOM:
+            off = (long long)d.pitch + (ghost ? 0 : 1);
+            stride = d.pitch;
+            n = d.nx;
+            break;
+        default:  // BND_TOP
+            off = (long long)d.pitch + (ghost ? d.ny + 1 : d.ny);
+            stride = d.pitch;
+            n = d.nx;
+            break;
+    }
+}
+
+// ---- queue pull ------------------------------------------------------------
+// One fetch per workgroup: thread 0 takes kWavesPerGroup consecutive items
+// (one per wave) and the base reaches every wave through LDS between two
+// barriers.  The result is made wave-uniform (an SGPR), so the loop exit and
+// everything derived from the item are uniform branches and the cross-lane
+// operations of the sweep always run with the full wave.  (Broadcasting a
+// lane-0 fetch with a shuffle inside the loop is not safe: the compiler
+// treats the shuffled value as divergent and may run the shuffle for lanes
+// 1..63 while lane 0 is masked off, which reads 0 and never ends the loop.)
+__device__ __forceinline__ int next_items(int* head, int* sBase) {
+    if (threadIdx.x == 0) *sBase = atomicAdd(head, kWavesPerGroup);
+    __syncthreads();
+    const int base = __builtin_amdgcn_readfirstlane(*sBase);
+    __syncthreads();  // every wave has read the base before the next fetch overwrites it
+    return base;
+}
+
+// ---- sweep: net-update sums of one strip tile ------------------------------
+// Same edge order and arithmetic as the single-block strip sweep: the wave
+// owns rows 62*yt+1 .. 62*yt+62 (lanes 1..62, lanes 0 and 63 are halo) and
+// marches over columns 1+xs*XS .. min(xs*XS+XS, nx).
+template <int S, int XS>
+__device__ __forceinline__ float strip_sums(const StateView& v, const SumsView& sums, int yt, int xs, int lane) {
+    const int y = kStripRows * yt + lane;
+    const bool rowValid = y <= v.ny + 1;
+    const int yc = rowValid ? y : v.ny + 1;
+    const bool owned = (lane >= 1) && (lane <= kStripRows) && (y <= v.ny);
+    const bool rowInterior = (y >= 1) && (y <= v.ny);
+    const bool hEdgeValid = (lane <= kStripRows) && (y <= v.ny);
+    const int x0 = 1 + xs * XS;
+    const int x1 = min(x0 + XS - 1, v.nx);
+    size_t o = idx(v, x0 - 1, yc);
+    CellQ cq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
+    o += v.pitch;
+    CellQ nq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
+    const dev::EdgeOut e = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+    float carryH = e.hR, carryHu = e.huR;
+    float smaxv = rowInterior ? e.speed : 0.f;
+    for (int x = x0; x <= x1; ++x) {
+        cq = nq;
+        o += v.pitch;
+        nq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
+        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+        if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
+        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(cq.yside(), cq.yside_from_lane_above());
+        if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
+        const float aH = __shfl_up(eh.hR, 1, 64);
+        const float aHv = __shfl_up(eh.huR, 1, 64);
+        if (owned) {
+            const size_t c = idx(v, x, y);
+            sums.sx[c] = carryH + ev.hL;
+            sums.su[c] = carryHu + ev.huL;
+            sums.sy[c] = aH + eh.hL;
+            sums.sv[c] = aHv + eh.huL;
+        }
+        carryH = ev.hR;
+        carryHu = ev.huR;
+    }
+    return wave_max(smaxv);
+}
+
+template <int S, int XS>
+__global__ __launch_bounds__(256) void batch_sweep_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+                                                          BatchGlobals* G, const int* items) {
+    if (G->skip) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int count = G->count;
+    __shared__ int sBase;
+    for (;;) {
+        const int item = next_items(&G->head, &sBase) + wave;
+        if (item - wave >= count) break;  // uniform over the workgroup
+        if (item >= count) continue;
+        const int code = __builtin_amdgcn_readfirstlane(items[item]);
+        const int blk = code >> kTileBits;
+        const int tile = code & ((1 << kTileBits) - 1);
+        const BatchBlockDesc& d = D[blk];
+        const StateView v = batch_view(A, d);
+        const float m = strip_sums<S, XS>(v, batch_sums(A, d), tile % d.nyt, tile / d.nyt, lane);
+        if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
+    }
+}
+
+// ---- apply: update the queued tiles' cells --------------------------------
+template <int XS>
+__global__ __launch_bounds__(256) void batch_apply_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                          const BatchBlockState* st, BatchGlobals* G,
+                                                          const int* items) {
+    if (G->skip) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int count = G->count;
+    const int mode = G->mode;
+    const float gdt = G->gdt;
+    __shared__ int sBase;
+    for (;;) {
+        const int item = next_items(&G->headApply, &sBase) + wave;
+        if (item - wave >= count) break;  // uniform over the workgroup
+        if (item >= count) continue;
+        const int code = __builtin_amdgcn_readfirstlane(items[item]);
+        const int blk = code >> kTileBits;
+        const int tile = code & ((1 << kTileBits) - 1);
+        const BatchBlockDesc& d = D[blk];
+        const StateView v = batch_view(A, d);
+        const SumsView sums = batch_sums(A, d);
+        const float dt = (mode == BATCH_GLOBAL) ? gdt : st[blk].maxTimestep;
+        const float dtdx = dt / v.dx;
+        const float dtdy = dt / v.dy;
+        BoundaryInfo bi;
+        for (int k = 0; k < 4; ++k) bi.type[k] = d.bnd[k];
+        const int y = kStripRows * (tile % d.nyt) + lane;
+        const bool owned = (lane >= 1) && (lane <= kStripRows) && (y <= v.ny);
+        const int x0 = 1 + (tile / d.nyt) * XS;
+        const int x1 = min(x0 + XS - 1, v.nx);
+        if (owned) {
+            for (int x = x0; x <= x1; ++x) {
+                const size_t c = idx(v, x, y);
+                const Cell nc = update_cell(v.h[c], v.hu[c], v.hv[c], dtdx, dtdy, sums.sx[c], sums.sy[c], sums.su[c],
+                                            sums.sv[c]);
+                v.h[c] = nc.h;
+                v.hu[c] = nc.hu;
+                v.hv[c] = nc.hv;
+                write_boundary_ghosts(v, bi, x, y, nc.h, nc.hu, nc.hv, v.h, v.hu, v.hv);
+            }
+        }
+    }
+}
+
+}  // namespace swe

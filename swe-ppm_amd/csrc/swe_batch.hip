@@ -1,0 +1,863 @@
+// swe_batch.hip -- batched engine of libswe_hip.so: control kernels of the
+// driver iteration (ghost exchange, LTS checks, queue fill, dt rounding), the
+// persistent apply, hipGraph capture / replay, the autotuner and the C ABI
+// of include/swe_hip_batch.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/swe_hip_batch.h"
+#include "kernels_batch.hpp"
+#include "swe_internal.h"
+
+using namespace swe;
+
+#define HIP_TRY(expr)                                     \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return hip_fail(_e, #expr); \
+    } while (0)
+
+namespace {
+
+constexpr int kCtlThreads = 256;
+
+// ---- initial ghost cells: applyBoundaryConditions (SWE_Block.hh:757-911)
+// for every block and the in-rank bathymetry copy
+// (SWE_DimensionalSplittingMPIOverdecomp.cpp:208-235).  One workgroup per block.
+__global__ __launch_bounds__(kCtlThreads) void batch_init_kernel(BatchArena A, const BatchBlockDesc* D) {
+    const BatchBlockDesc d = D[blockIdx.x];
+    float* h = A.h + d.off;
+    float* hu = A.hu + d.off;
+    float* hv = A.hv + d.off;
+    float* b = A.b + d.off;
+    const int nx = d.nx, ny = d.ny, p = d.pitch;
+    for (int side = 0; side < 4; ++side) {
+        const int t = d.bnd[side];
+        if (t == OUTFLOW || t == WALL) {
+            long long go, so;
+            int gs, ss, n;
+            side_line(d, side, 1, go, gs, n);
+            side_line(d, side, 0, so, ss, n);
+            const bool vertical = side == BND_LEFT || side == BND_RIGHT;
+            const float su = (t == WALL && vertical) ? -1.f : 1.f;
+            const float sv = (t == WALL && !vertical) ? -1.f : 1.f;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const long long g = go + (long long)i * gs, s = so + (long long)i * ss;
+                h[g] = h[s];
+                hu[g] = (su < 0.f) ? -hu[s] : hu[s];
+                hv[g] = (sv < 0.f) ? -hv[s] : hv[s];
+            }
+        } else if (t == CONNECT_WITHIN_RANK) {
+            const BatchBlockDesc dn = D[d.nbr[side]];
+            long long go, so;
+            int gs, ss, n, nn;
+            side_line(d, side, 1, go, gs, n);
+            side_line(dn, side ^ 1, 0, so, ss, nn);
+            const float* bn = A.b + dn.off;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) b[go + (long long)i * gs] = bn[so + (long long)i * ss];
+        }
+    }
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        const int cx = (k < 2) ? 0 : nx + 1, cy = (k & 1) ? ny + 1 : 0;
+        const int sx = (k < 2) ? 1 : nx, sy = (k & 1) ? ny : 1;
+        const long long c = (long long)cx * p + cy, s = (long long)sx * p + sy;
+        h[c] = h[s];
+        hu[c] = hu[s];
+        hv[c] = hv[s];
+    }
+}
+
+// ---- ghost: setGhostLayer of every block, CONNECT_WITHIN_RANK part
+// (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304; the OUTFLOW / WALL part
+// is written by the previous apply).  One workgroup per block.
+__global__ __launch_bounds__(kCtlThreads) void batch_ghost_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                                  BatchBlockState* st, BatchGlobals* G) {
+    const int blk = blockIdx.x;
+    const int stop = G->stop;
+    if (blk == 0 && threadIdx.x == 0) {
+        G->count = 0;
+        G->head = 0;
+        G->headApply = 0;
+        G->skip = stop;
+    }
+    if (stop) return;
+    const int mode = G->mode;
+    const bool lts = mode == BATCH_LTS;
+    const float T = G->T;
+    const BatchBlockDesc d = D[blk];
+    BatchBlockState& s = st[blk];
+    const int order[4] = {BND_RIGHT, BND_LEFT, BND_TOP, BND_BOTTOM};
+    for (int k = 0; k < 4; ++k) {
+        const int side = order[k];
+        if (d.bnd[side] != CONNECT_WITHIN_RANK) continue;
+        if (lts && !lts_receivable(s, side)) continue;  // uniform: s is only read here
+        const int nb = d.nbr[side];
+        const BatchBlockDesc dn = D[nb];
+        if (lts && threadIdx.x == 0) s.borderTs[side] = lts_total(T, st[nb]);
+        long long so, go;
+        int ss, gs, n, nn;
+        side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
+        side_line(d, side, 1, go, gs, n);        // my ghost line
+        const float* sh = A.h + dn.off + so;
+        const float* shu = A.hu + dn.off + so;
+        const float* shv = A.hv + dn.off + so;
+        if (lts) {
+            float* L = A.lines + d.line[side];
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                L[i] = sh[(long long)i * ss];
+                L[n + i] = shu[(long long)i * ss];
+                L[2 * n + i] = shv[(long long)i * ss];
+            }
+        } else {
+            float* gh = A.h + d.off + go;
+            float* ghu = A.hu + d.off + go;
+            float* ghv = A.hv + d.off + go;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                gh[(long long)i * gs] = sh[(long long)i * ss];
+                ghu[(long long)i * gs] = shu[(long long)i * ss];
+                ghv[(long long)i * gs] = shv[(long long)i * ss];
+            }
+        }
+    }
+}
+
+// ---- check: checkAllGhostlayers (SWE_Block.hh:375-436) and queue fill.
+// One workgroup per block.
+__global__ __launch_bounds__(kCtlThreads) void batch_check_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                                  BatchBlockState* st, BatchGlobals* G, int* items) {
+    if (G->skip) return;
+    const int blk = blockIdx.x;
+    const BatchBlockDesc d = D[blk];
+    BatchBlockState& s = st[blk];
+    __shared__ int action[4];  // 0 none, 1 copy, 2 interpolate
+    __shared__ float ratio[4];
+    __shared__ int activeS, base;
+    if (G->mode == BATCH_LTS) {
+        const float T = G->T, duration = G->duration;
+        if (threadIdx.x == 0) {
+            const float tot = lts_total(T, s);
+            for (int side = 0; side < 4; ++side) {
+                action[side] = 0;
+                if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) continue;
+                const float bt = s.borderTs[side];
+                if ((double)fabsf(bt - tot) <= DBL_EPSILON) {
+                    action[side] = 1;
+                    s.gl[side] = GL_SYNC;
+                } else if (bt > tot) {
+                    action[side] = 2;
+                    ratio[side] = tot / bt;  // interpolation weight t_me / t_neighbour
+                    s.gl[side] = GL_INTER;
+                } else {
+                    s.gl[side] = GL_UNVALID;
+                }
+                if (bt >= duration) s.gl[side] = GL_DONE;
+            }
+        }
+        __syncthreads();
+        for (int side = 0; side < 4; ++side) {
+            const int act = action[side];
+            if (act == 0) continue;
+            long long go;
+            int gs, n;
+            side_line(d, side, 1, go, gs, n);
+            const float* L = A.lines + d.line[side];
+            float* f[3] = {A.h + d.off + go, A.hu + d.off + go, A.hv + d.off + go};
+            const float w = ratio[side];
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                for (int c = 0; c < 3; ++c) {
+                    float& g = f[c][(long long)i * gs];
+                    const float nv = L[c * n + i];
+                    g = (act == 1) ? nv : (g + (nv - g) * w);
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            const float tot = lts_total(T, s);
+            if ((double)fabsf(duration - tot) <= DBL_EPSILON || tot >= duration) s.notified = 1;
+            if (lts_in_sync(s))
+                for (int side = 0; side < 4; ++side)
+                    if (s.gl[side] != GL_DONE) s.gl[side] = (s.gl[side] == GL_INTER) ? GL_INTER : GL_NEXT;
+            if (lts_in_sync(s) && s.stepSizeCounter < s.stepSize) s.stepSizeCounter++;
+            s.active = lts_in_sync(s) ? 1 : 0;
+            activeS = s.active;
+        }
+    } else if (threadIdx.x == 0) {
+        s.active = 1;
+        activeS = 1;
+    }
+    __syncthreads();
+    if (!activeS) return;
+    if (threadIdx.x == 0) base = atomicAdd(&G->count, d.ntiles);
+    __syncthreads();
+    for (int i = threadIdx.x; i < d.ntiles; i += blockDim.x) items[base + i] = (blk << kTileBits) | i;
+}
+
+// ---- round: per-block dt (SWE_DimensionalSplittingMpi.cpp:448-463), the
+// global minimum (swe_mpi_overdecomp.cpp:297-307), the completion test
+// (hasMaxLocalTimestep, SWE_Block.hh:355-358, swe_mpi_overdecomp.cpp:327-342)
+// and resetStepSizeCounter (SWE_Block.hh:530-535).  One workgroup.
+__global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlockDesc* D, BatchBlockState* st,
+                                                                  BatchGlobals* G, float* dtLog, int* iterLog,
+                                                                  int* hostStat) {
+    if (G->skip) return;
+    const int mode = G->mode;
+    const float T = G->T;
+    const int nb = G->nblocks;
+    __shared__ unsigned int sminBits;
+    __shared__ int allSynced, macroEnd;
+    if (threadIdx.x == 0) {
+        sminBits = __float_as_uint(FLT_MAX);
+        allSynced = 1;
+    }
+    __syncthreads();
+    for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) {
+        BatchBlockState& s = st[blk];
+        if (s.active) {
+            const BatchBlockDesc& d = D[blk];
+            float dt = timestep_from_max(__uint_as_float(s.smax), d.dx, d.dy);
+            if (mode == BATCH_LTS) {
+                if (s.stepSizeCounter <= 0) s.stepSize = lts_step_size(T, dt);
+                dt = T / s.stepSize;
+            }
+            if (mode == BATCH_ENSEMBLE) s.t += dt;
+            s.updates += 1;
+            s.maxTimestep = dt;
+            if (mode == BATCH_GLOBAL) atomicMin(&sminBits, __float_as_uint(dt));  // dt > 0: bits order as floats
+        }
+        s.smax = 0u;
+        if (mode == BATCH_LTS) {
+            const bool has = (s.stepSizeCounter >= s.stepSize && lts_in_sync(s)) || s.notified;
+            if (!has) atomicAnd(&allSynced, 0);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        G->iterations += 1;
+        G->itersUnit += 1;
+        if (mode == BATCH_GLOBAL) G->gdt = __uint_as_float(sminBits);
+        const bool unitDone = (mode != BATCH_LTS) || allSynced;
+        if (unitDone) {
+            const int u = G->done;
+            if (u < G->logCap) {
+                dtLog[u] = (mode == BATCH_GLOBAL) ? __uint_as_float(sminBits) : 0.f;
+                iterLog[u] = G->itersUnit;
+            }
+            G->done = u + 1;
+            G->itersUnit = 0;
+            if (u + 1 >= G->target) G->stop = 1;
+        }
+        macroEnd = (mode == BATCH_LTS) && allSynced;
+        hostStat[0] = G->stop;
+        hostStat[1] = G->done;
+        hostStat[2] = G->itersUnit;
+    }
+    __syncthreads();
+    if (macroEnd) {
+        for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) {
+            BatchBlockState& s = st[blk];
+            s.stepSizeCounter = 0;
+            s.stepSize = 0;
+            s.timestepCounter += 1;
+        }
+    }
+}
+
+// start of a run: `target` more units
+__global__ void batch_begin_kernel(BatchGlobals* G, int target, int logCap, int* hostStat) {
+    G->target = target;
+    G->done = 0;
+    G->stop = target <= 0 ? 1 : 0;
+    G->itersUnit = 0;
+    G->logCap = logCap;
+    hostStat[0] = G->stop;
+    hostStat[1] = 0;
+    hostStat[2] = 0;
+}
+
+template <int XS>
+int launch_apply(int groups, BatchArena A, const BatchBlockDesc* D, const BatchBlockState* st, BatchGlobals* G,
+                 const int* items, hipStream_t stream) {
+    hipLaunchKernelGGL((batch_apply_kernel<XS>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, A, D, st, G,
+                       items);
+    return 0;
+}
+
+int launch_sweep(int solver, int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+                 BatchGlobals* G, const int* items, hipStream_t stream) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            return launch_batch_sweep_solver<SOLVER_HLLE>(xs, groups, A, D, st, G, items, stream);
+        case SOLVER_FWAVE:
+            return launch_batch_sweep_solver<SOLVER_FWAVE>(xs, groups, A, D, st, G, items, stream);
+        case SOLVER_AUGRIE:
+            return launch_batch_sweep_solver<SOLVER_AUGRIE>(xs, groups, A, D, st, G, items, stream);
+        default:
+            return launch_batch_sweep_solver<SOLVER_HYBRID>(xs, groups, A, D, st, G, items, stream);
+    }
+}
+
+}  // namespace
+
+struct swe_hip_batch {
+    int nblocks = 0, solver = SOLVER_AUGRIE, mode = BATCH_ENSEMBLE;
+    float T = 1.f, duration = 0.f;
+    std::vector<BatchBlockDesc> desc;
+    long long total = 0;      // floats per field over all blocks
+    long long lineTotal = 0;  // floats of the LTS receive lines
+    int itemCap = 0;
+    BatchArena A{};
+    BatchBlockDesc* dDesc = nullptr;
+    BatchBlockState* dState = nullptr;
+    BatchGlobals* dG = nullptr;
+    int* dItems = nullptr;
+    float* dDtLog = nullptr;
+    int* dIterLog = nullptr;
+    int logCap = 0;
+    int* hostStat = nullptr;  // pinned, written by the round kernel
+    int* dHostStat = nullptr;
+    int xs = 8, groups = 0, chunk = 16, useGraph = 1;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool uploaded = false;
+};
+
+namespace {
+
+void free_graph(swe_hip_batch* k) {
+    if (k->exec) (void)hipGraphExecDestroy(k->exec);
+    k->exec = nullptr;
+}
+
+int tiles_for(swe_hip_batch* k, int xs, bool upload) {
+    long long count = 0;
+    for (auto& d : k->desc) {
+        d.nyt = (d.ny + kStripRows - 1) / kStripRows;
+        d.ntiles = d.nyt * ((d.nx + xs - 1) / xs);
+        count += d.ntiles;
+    }
+    if (count > k->itemCap) {
+        set_error("batch: %lld work items exceed the queue (%d)", count, k->itemCap);
+        return SWE_HIP_EINVAL;
+    }
+    if (upload) {
+        HIP_TRY(hipMemcpyAsync(k->dDesc, k->desc.data(), sizeof(BatchBlockDesc) * k->desc.size(),
+                               hipMemcpyHostToDevice, k->stream));
+        HIP_TRY(hipStreamSynchronize(k->stream));
+    }
+    return 0;
+}
+
+int default_groups(swe_hip_batch* k) {
+    long long items = 0;
+    for (auto& d : k->desc) items += d.ntiles;
+    const long long want = (items + kWavesPerGroup - 1) / kWavesPerGroup;
+    return (int)std::max<long long>(1, std::min<long long>(want, 8LL * k->cus));
+}
+
+// the launches of one driver iteration
+int enqueue_iteration(swe_hip_batch* k) {
+    const dim3 blk(kCtlThreads);
+    hipLaunchKernelGGL(batch_ghost_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
+    hipLaunchKernelGGL(batch_check_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG,
+                       k->dItems);
+    int rc = launch_sweep(k->solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(batch_round_kernel, dim3(1), blk, 0, k->stream, k->dDesc, k->dState, k->dG, k->dDtLog,
+                       k->dIterLog, k->dHostStat);
+    switch (k->xs) {
+        case 4:
+            launch_apply<4>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+            break;
+        case 16:
+            launch_apply<16>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+            break;
+        default:
+            launch_apply<8>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+            break;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch iteration launch");
+    return 0;
+}
+
+int ensure_graph(swe_hip_batch* k) {
+    if (k->exec) return 0;
+    HIP_TRY(hipStreamBeginCapture(k->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int i = 0; i < k->chunk && rc == 0; ++i) rc = enqueue_iteration(k);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(k->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "batch graph capture");
+    const hipError_t e2 = hipGraphInstantiate(&k->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+        k->exec = nullptr;
+        return hip_fail(e2, "batch graph instantiate");
+    }
+    return 0;
+}
+
+// one chunk of driver iterations
+int enqueue_chunk(swe_hip_batch* k) {
+    if (k->useGraph) {
+        int rc = ensure_graph(k);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(k->exec, k->stream));
+        return 0;
+    }
+    for (int i = 0; i < k->chunk; ++i) {
+        int rc = enqueue_iteration(k);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int ensure_log(swe_hip_batch* k, int units) {
+    if (units <= k->logCap) return 0;
+    const int cap = std::max(units, 1024);
+    float* dl = nullptr;
+    int* il = nullptr;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    HIP_TRY(hipMalloc(&dl, sizeof(float) * cap));
+    const hipError_t e = hipMalloc(&il, sizeof(int) * cap);
+    if (e != hipSuccess) {
+        (void)hipFree(dl);
+        return hip_fail(e, "batch log");
+    }
+    if (k->dDtLog) (void)hipFree(k->dDtLog);
+    if (k->dIterLog) (void)hipFree(k->dIterLog);
+    k->dDtLog = dl;
+    k->dIterLog = il;
+    k->logCap = cap;
+    free_graph(k);  // captured launches hold the old pointers
+    return 0;
+}
+
+// Advance until `units` are done; returns the iterations replayed.
+int advance(swe_hip_batch* k, int units, long long maxIter, long long* itersOut) {
+    hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(1), 0, k->stream, k->dG, units, k->logCap, k->dHostStat);
+    HIP_TRY(hipGetLastError());
+    long long iters = 0;
+    if (k->mode != BATCH_LTS) {
+        // the unit count fixes the iterations; the device skips the overshoot
+        while (iters < units) {
+            int rc = enqueue_chunk(k);
+            if (rc) return rc;
+            iters += k->chunk;
+        }
+        HIP_TRY(hipStreamSynchronize(k->stream));
+    } else {
+        // two chunks in flight; the round kernel reports completion in pinned memory
+        hipEvent_t ev[2];
+        HIP_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        int rc = 0;
+        for (long long r = 0;; ++r) {
+            rc = enqueue_chunk(k);
+            if (rc) break;
+            iters += k->chunk;
+            hipError_t e = hipEventRecord(ev[r & 1], k->stream);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "batch event");
+                break;
+            }
+            if (r >= 1) {
+                e = hipEventSynchronize(ev[(r - 1) & 1]);
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "batch event sync");
+                    break;
+                }
+                if (__atomic_load_n(&k->hostStat[0], __ATOMIC_ACQUIRE)) break;
+            }
+            if (iters > maxIter) {
+                set_error("batch: LTS macro step did not complete within %lld driver iterations", maxIter);
+                rc = SWE_HIP_ETIMEOUT;
+                break;
+            }
+        }
+        const hipError_t e = hipStreamSynchronize(k->stream);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "batch sync");
+    }
+    if (itersOut) *itersOut = iters;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float* dx, const float* dy, const int* bnd,
+                         const int* nbr, int solver, int mode, float max_timestep_local, float duration,
+                         swe_hip_batch** out) {
+    if (!out || nblocks < 1 || nblocks > kMaxBatchBlocks || !nx || !ny || !dx || !dy || !bnd || !nbr || solver < 0 ||
+        solver > 3 || mode < BATCH_ENSEMBLE || mode > BATCH_LTS) {
+        set_error("swe_hip_batch_create: invalid argument (nblocks=%d solver=%d mode=%d)", nblocks, solver, mode);
+        return SWE_HIP_EINVAL;
+    }
+    if (mode == BATCH_LTS && !(max_timestep_local > 0.f)) {
+        set_error("swe_hip_batch_create: LTS needs max_timestep_local > 0");
+        return SWE_HIP_EINVAL;
+    }
+    std::vector<BatchBlockDesc> desc(nblocks);
+    long long off = 0, lines = 0, items = 0;
+    for (int k = 0; k < nblocks; ++k) {
+        BatchBlockDesc& d = desc[k];
+        std::memset(&d, 0, sizeof(d));
+        if (nx[k] < 1 || ny[k] < 1 || !(dx[k] > 0.f) || !(dy[k] > 0.f) ||
+            (long long)(nx[k] + 2) * (ny[k] + 2) > (1LL << 31)) {
+            set_error("swe_hip_batch_create: block %d: invalid size or mesh (nx=%d ny=%d dx=%g dy=%g)", k, nx[k],
+                      ny[k], dx[k], dy[k]);
+            return SWE_HIP_EINVAL;
+        }
+        d.nx = nx[k];
+        d.ny = ny[k];
+        d.pitch = ny[k] + 2;
+        d.dx = dx[k];
+        d.dy = dy[k];
+        d.off = off;
+        off += (long long)(nx[k] + 2) * (ny[k] + 2);
+        const int nyt = (ny[k] + kStripRows - 1) / kStripRows;
+        const long long tiles4 = (long long)nyt * ((nx[k] + 3) / 4);
+        if (tiles4 >= (1LL << kTileBits)) {
+            set_error("swe_hip_batch_create: block %d too large for the work queue", k);
+            return SWE_HIP_EINVAL;
+        }
+        items += tiles4;
+        for (int s = 0; s < 4; ++s) {
+            const int t = bnd[4 * k + s];
+            if (t != OUTFLOW && t != WALL && t != CONNECT_WITHIN_RANK && t != PASSIVE) {
+                set_error("swe_hip_batch_create: block %d side %d: boundary type %d not supported", k, s, t);
+                return SWE_HIP_EINVAL;
+            }
+            d.bnd[s] = t;
+            d.nbr[s] = nbr[4 * k + s];
+            d.line[s] = 0;
+        }
+    }
+    // neighbour consistency: the edge is shared whole and the link is mutual
+    for (int k = 0; k < nblocks; ++k) {
+        for (int s = 0; s < 4; ++s) {
+            if (desc[k].bnd[s] != CONNECT_WITHIN_RANK) continue;
+            const int m = desc[k].nbr[s];
+            const bool vertical = s == BND_LEFT || s == BND_RIGHT;
+            if (m < 0 || m >= nblocks || m == k || desc[m].bnd[s ^ 1] != CONNECT_WITHIN_RANK ||
+                desc[m].nbr[s ^ 1] != k || (vertical ? desc[m].ny != desc[k].ny : desc[m].nx != desc[k].nx)) {
+                set_error("swe_hip_batch_create: block %d side %d: neighbour %d does not share the edge", k, s, m);
+                return SWE_HIP_EINVAL;
+            }
+            if (mode == BATCH_ENSEMBLE) {
+                set_error("swe_hip_batch_create: ensemble blocks cannot be connected (block %d)", k);
+                return SWE_HIP_EINVAL;
+            }
+            if (mode == BATCH_LTS) {
+                desc[k].line[s] = lines;
+                lines += 3LL * (vertical ? desc[k].ny : desc[k].nx);
+            }
+        }
+    }
+    if (items >= (1LL << 31)) {
+        set_error("swe_hip_batch_create: too many work items");
+        return SWE_HIP_EINVAL;
+    }
+    swe_hip_batch* k = new (std::nothrow) swe_hip_batch();
+    if (!k) {
+        set_error("swe_hip_batch_create: out of host memory");
+        return SWE_HIP_EINVAL;
+    }
+    k->nblocks = nblocks;
+    k->solver = solver;
+    k->mode = mode;
+    k->T = max_timestep_local;
+    k->duration = duration;
+    k->desc = desc;
+    k->total = off;
+    k->lineTotal = lines;
+    k->itemCap = (int)items;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+        k->cus = prop.multiProcessorCount;
+    const size_t bytes = sizeof(float) * (size_t)off;
+    hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
+    float** fields[8] = {&k->A.h, &k->A.hu, &k->A.hv, &k->A.b, &k->A.sx, &k->A.sy, &k->A.su, &k->A.sv};
+    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipMalloc(fields[i], bytes);
+    if (e == hipSuccess) e = hipMalloc(&k->A.lines, sizeof(float) * (size_t)std::max<long long>(lines, 1));
+    if (e == hipSuccess) e = hipMalloc(&k->dDesc, sizeof(BatchBlockDesc) * nblocks);
+    if (e == hipSuccess) e = hipMalloc(&k->dState, sizeof(BatchBlockState) * nblocks);
+    if (e == hipSuccess) e = hipMalloc(&k->dG, sizeof(BatchGlobals));
+    if (e == hipSuccess) e = hipMalloc(&k->dItems, sizeof(int) * (size_t)std::max(k->itemCap, 1));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&k->hostStat, 4 * sizeof(int), hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&k->dHostStat, k->hostStat, 0);
+    if (e == hipSuccess) {
+        for (int i = 4; i < 8 && e == hipSuccess; ++i) e = hipMemsetAsync(*fields[i], 0, bytes, k->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
+    if (e != hipSuccess) {
+        hip_fail(e, "swe_hip_batch_create");
+        swe_hip_batch_destroy(k);
+        return SWE_HIP_EDEVICE;
+    }
+    int rc = tiles_for(k, k->xs, true);
+    if (rc == 0) rc = ensure_log(k, 1024);
+    if (rc) {
+        swe_hip_batch_destroy(k);
+        return rc;
+    }
+    k->groups = default_groups(k);
+    *out = k;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_destroy(swe_hip_batch* k) {
+    if (!k) return SWE_HIP_OK;
+    if (k->stream) (void)hipStreamSynchronize(k->stream);
+    free_graph(k);
+    float* fields[9] = {k->A.h, k->A.hu, k->A.hv, k->A.b, k->A.sx, k->A.sy, k->A.su, k->A.sv, k->A.lines};
+    for (float* f : fields)
+        if (f) (void)hipFree(f);
+    if (k->dDesc) (void)hipFree(k->dDesc);
+    if (k->dState) (void)hipFree(k->dState);
+    if (k->dG) (void)hipFree(k->dG);
+    if (k->dItems) (void)hipFree(k->dItems);
+    if (k->dDtLog) (void)hipFree(k->dDtLog);
+    if (k->dIterLog) (void)hipFree(k->dIterLog);
+    if (k->hostStat) (void)hipHostFree(k->hostStat);
+    if (k->stream) (void)hipStreamDestroy(k->stream);
+    delete k;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_upload(swe_hip_batch* k, const float* h, const float* hu, const float* hv, const float* b) {
+    if (!k || !h || !hu || !hv || !b) {
+        set_error("swe_hip_batch_upload: null argument");
+        return SWE_HIP_EINVAL;
+    }
+    const size_t bytes = sizeof(float) * (size_t)k->total;
+    HIP_TRY(hipMemcpyAsync(k->A.h, h, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->A.hu, hu, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->A.hv, hv, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->A.b, b, bytes, hipMemcpyHostToDevice, k->stream));
+    std::vector<BatchBlockState> st(k->nblocks);
+    for (auto& s : st) {
+        std::memset(&s, 0, sizeof(s));
+        for (int i = 0; i < 4; ++i) s.gl[i] = GL_NEXT;  // SWE_Block.hh:297-300
+    }
+    HIP_TRY(hipMemcpyAsync(k->dState, st.data(), sizeof(BatchBlockState) * st.size(), hipMemcpyHostToDevice,
+                           k->stream));
+    BatchGlobals g;
+    std::memset(&g, 0, sizeof(g));
+    g.T = k->T;
+    g.duration = k->duration;
+    g.mode = k->mode;
+    g.nblocks = k->nblocks;
+    g.stop = 1;
+    HIP_TRY(hipMemcpyAsync(k->dG, &g, sizeof(g), hipMemcpyHostToDevice, k->stream));
+    hipLaunchKernelGGL(batch_init_kernel, dim3(k->nblocks), dim3(kCtlThreads), 0, k->stream, k->A, k->dDesc);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    k->uploaded = true;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_download(swe_hip_batch* k, float* h, float* hu, float* hv, float* b) {
+    if (!k || !h || !hu || !hv) {
+        set_error("swe_hip_batch_download: null argument");
+        return SWE_HIP_EINVAL;
+    }
+    const size_t bytes = sizeof(float) * (size_t)k->total;
+    HIP_TRY(hipMemcpyAsync(h, k->A.h, bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipMemcpyAsync(hu, k->A.hu, bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipMemcpyAsync(hv, k->A.hv, bytes, hipMemcpyDeviceToHost, k->stream));
+    if (b) HIP_TRY(hipMemcpyAsync(b, k->A.b, bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_run(swe_hip_batch* k, int units, float* dt_log, int* iter_log, float* elapsed_ms) {
+    if (!k || units < 0) {
+        set_error("swe_hip_batch_run: invalid argument");
+        return SWE_HIP_EINVAL;
+    }
+    if (!k->uploaded) {
+        set_error("swe_hip_batch_run: no state uploaded");
+        return SWE_HIP_ESTATE;
+    }
+    if (units == 0) {
+        if (elapsed_ms) *elapsed_ms = 0.f;
+        return SWE_HIP_OK;
+    }
+    int rc = ensure_log(k, units);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (elapsed_ms) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, k->stream));
+    }
+    const long long maxIter = (long long)units * 4 * (kMaxDivisor + 1) + 2LL * k->chunk;
+    rc = advance(k, units, maxIter, nullptr);
+    if (elapsed_ms && rc == 0) {
+        HIP_TRY(hipEventRecord(e1, k->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(elapsed_ms, e0, e1));
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (dt_log) HIP_TRY(hipMemcpy(dt_log, k->dDtLog, sizeof(float) * units, hipMemcpyDeviceToHost));
+    if (iter_log) HIP_TRY(hipMemcpy(iter_log, k->dIterLog, sizeof(int) * units, hipMemcpyDeviceToHost));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_stats(swe_hip_batch* k, int64_t* updates, float* t) {
+    if (!k) {
+        set_error("swe_hip_batch_stats: null handle");
+        return SWE_HIP_EINVAL;
+    }
+    std::vector<BatchBlockState> st(k->nblocks);
+    HIP_TRY(hipMemcpyAsync(st.data(), k->dState, sizeof(BatchBlockState) * st.size(), hipMemcpyDeviceToHost,
+                           k->stream));
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    for (int i = 0; i < k->nblocks; ++i) {
+        if (updates) updates[i] = st[i].updates;
+        if (t) t[i] = st[i].t;
+    }
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_set_tuning(swe_hip_batch* k, int xs, int groups, int chunk, int graph) {
+    if (!k || (xs != -1 && xs != 4 && xs != 8 && xs != 16) || (groups != -1 && (groups < 1 || groups > 65535)) ||
+        (chunk != -1 && (chunk < 1 || chunk > 256)) || (graph != -1 && graph != 0 && graph != 1)) {
+        set_error("swe_hip_batch_set_tuning: invalid argument (xs=%d groups=%d chunk=%d graph=%d)", xs, groups, chunk,
+                  graph);
+        return SWE_HIP_EINVAL;
+    }
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    if (xs != -1 && xs != k->xs) {
+        int rc = tiles_for(k, xs, true);
+        if (rc) return rc;
+        k->xs = xs;
+    }
+    if (groups != -1) k->groups = groups;
+    if (chunk != -1) k->chunk = chunk;
+    if (graph != -1) k->useGraph = graph;
+    free_graph(k);
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_get_tuning(swe_hip_batch* k, int* xs, int* groups, int* chunk, int* graph) {
+    if (!k) {
+        set_error("swe_hip_batch_get_tuning: null handle");
+        return SWE_HIP_EINVAL;
+    }
+    if (xs) *xs = k->xs;
+    if (groups) *groups = k->groups;
+    if (chunk) *chunk = k->chunk;
+    if (graph) *graph = k->useGraph;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int* ncand) {
+    if (!k || iterations < 1) {
+        set_error("swe_hip_batch_autotune: invalid argument");
+        return SWE_HIP_EINVAL;
+    }
+    if (!k->uploaded) {
+        set_error("swe_hip_batch_autotune: no state uploaded");
+        return SWE_HIP_ESTATE;
+    }
+    // snapshot of everything an iteration changes
+    const size_t bytes = sizeof(float) * (size_t)k->total;
+    const size_t lbytes = sizeof(float) * (size_t)std::max<long long>(k->lineTotal, 1);
+    float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sl = nullptr;
+    BatchBlockState* sst = nullptr;
+    BatchGlobals* sg = nullptr;
+    hipError_t e = hipMalloc(&sh, bytes);
+    if (e == hipSuccess) e = hipMalloc(&shu, bytes);
+    if (e == hipSuccess) e = hipMalloc(&shv, bytes);
+    if (e == hipSuccess) e = hipMalloc(&sl, lbytes);
+    if (e == hipSuccess) e = hipMalloc(&sst, sizeof(BatchBlockState) * k->nblocks);
+    if (e == hipSuccess) e = hipMalloc(&sg, sizeof(BatchGlobals));
+    auto copy_all = [&](bool save) -> hipError_t {
+        hipError_t r = hipSuccess;
+        auto cp = [&](void* a, void* b, size_t n) {
+            if (r == hipSuccess) r = hipMemcpyAsync(save ? a : b, save ? b : a, n, hipMemcpyDeviceToDevice, k->stream);
+        };
+        cp(sh, k->A.h, bytes);
+        cp(shu, k->A.hu, bytes);
+        cp(shv, k->A.hv, bytes);
+        cp(sl, k->A.lines, lbytes);
+        cp(sst, k->dState, sizeof(BatchBlockState) * k->nblocks);
+        cp(sg, k->dG, sizeof(BatchGlobals));
+        if (r == hipSuccess) r = hipStreamSynchronize(k->stream);
+        return r;
+    };
+    if (e == hipSuccess) e = copy_all(true);
+    const int xsC[3] = {4, 8, 16};
+    const int gC[3] = {2, 4, 8};
+    const int oldXs = k->xs, oldGroups = k->groups, oldChunk = k->chunk;
+    int bestXs = oldXs, bestGroups = oldGroups, n = 0, rc = 0;
+    float best = FLT_MAX;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const int units = 1 << 30;  // never reached: the timed iterations run through macro-step ends
+    for (int a = 0; a < 3 && e == hipSuccess && rc == 0; ++a) {
+        for (int g = 0; g < 3 && e == hipSuccess && rc == 0; ++g) {
+            rc = swe_hip_batch_set_tuning(k, xsC[a], gC[g] * k->cus, std::min(iterations, 256), -1);
+            if (rc) break;
+            // warm-up chunk (also captures the graph), then the timed chunks
+            hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(1), 0, k->stream, k->dG, units, k->logCap,
+                               k->dHostStat);
+            rc = enqueue_chunk(k);
+            if (rc) break;
+            e = hipEventRecord(e0, k->stream);
+            int done = 0;
+            while (rc == 0 && e == hipSuccess && done < iterations) {
+                rc = enqueue_chunk(k);
+                done += k->chunk;
+            }
+            if (rc || e != hipSuccess) break;
+            e = hipEventRecord(e1, k->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) break;
+            const float per = ms / done;
+            if (cand_ms) cand_ms[n] = per;
+            ++n;
+            if (per < best) {
+                best = per;
+                bestXs = xsC[a];
+                bestGroups = gC[g] * k->cus;
+            }
+            e = copy_all(false);
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e == hipSuccess && rc == 0) e = copy_all(false);
+    for (void* p : {(void*)sh, (void*)shu, (void*)shv, (void*)sl, (void*)sst, (void*)sg})
+        if (p) (void)hipFree(p);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "swe_hip_batch_autotune");
+    if (ncand) *ncand = n;
+    return swe_hip_batch_set_tuning(k, bestXs, bestGroups, oldChunk, -1);
+}
+
+}  // extern "C"

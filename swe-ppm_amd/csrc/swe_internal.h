@@ -6,6 +6,7 @@
 
 #include <cfloat>
 
+#include "kernels_batch.hpp"
 #include "kernels_fused.hpp"
 #include "kernels_u.hpp"
 
@@ -60,5 +61,10 @@ template <int S, int V>
 int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_t stream);
 template <int S>
 int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream);
+
+// Batched engine: strip sweep over the queued tiles (one translation unit per solver).
+template <int S>
+int launch_batch_sweep_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+                              BatchGlobals* G, const int* items, hipStream_t stream);
 
 }  // namespace swe

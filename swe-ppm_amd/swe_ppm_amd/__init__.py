@@ -39,6 +39,7 @@ def _load() -> C.CDLL:
             f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(str(LIB_PATH))
     fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
     vp = C.c_void_p
     sig = {
         "swe_hip_last_error": (C.c_char_p, []),
@@ -62,6 +63,16 @@ def _load() -> C.CDLL:
         "swe_hip_set_math": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
         "swe_hip_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "swe_hip_batch_create": (C.c_int, [C.c_int, ip, ip, fp, fp, ip, ip, C.c_int, C.c_int, C.c_float, C.c_float,
+                                           C.POINTER(vp)]),
+        "swe_hip_batch_destroy": (C.c_int, [vp]),
+        "swe_hip_batch_upload": (C.c_int, [vp, fp, fp, fp, fp]),
+        "swe_hip_batch_download": (C.c_int, [vp, fp, fp, fp, fp]),
+        "swe_hip_batch_run": (C.c_int, [vp, C.c_int, fp, ip, fp]),
+        "swe_hip_batch_stats": (C.c_int, [vp, C.POINTER(C.c_int64), fp]),
+        "swe_hip_batch_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "swe_hip_batch_get_tuning": (C.c_int, [vp, ip, ip, ip, ip]),
+        "swe_hip_batch_autotune": (C.c_int, [vp, C.c_int, fp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -175,6 +186,114 @@ class SweBlock:
         s, x = C.c_int(), C.c_int()
         _check(lib.swe_hip_get_tuning(self._h, C.byref(s), C.byref(x)), "get_tuning")
         return s.value, x.value
+
+
+BATCH_ENSEMBLE, BATCH_GLOBAL, BATCH_LTS = 0, 1, 2
+
+
+class SweBatch:
+    """Many flavour-U blocks advanced together on the GPU (include/swe_hip_batch.h).
+
+    blocks: list of dicts with nx, ny, dx, dy (or the dx/dy arguments), bnd[4]
+    and nbr[4] (neighbour index or -1); mode BATCH_ENSEMBLE (independent,
+    per-block dt), BATCH_GLOBAL (connected, global dt) or BATCH_LTS
+    (connected, local time stepping with macro step max_local)."""
+
+    def __init__(self, blocks, solver: int = AUGRIE, mode: int = BATCH_ENSEMBLE, dx=None, dy=None,
+                 max_local: float = 1.0, duration: float = 0.0):
+        self.n = len(blocks)
+        self.nx = np.array([bl["nx"] for bl in blocks], np.int32)
+        self.ny = np.array([bl["ny"] for bl in blocks], np.int32)
+        self.dx = np.array([bl.get("dx", dx) for bl in blocks], np.float32)
+        self.dy = np.array([bl.get("dy", dy) for bl in blocks], np.float32)
+        self.bnd = np.ascontiguousarray(np.concatenate([bl["bnd"] for bl in blocks]), np.int32)
+        self.nbr = np.ascontiguousarray(
+            np.concatenate([bl.get("nbr", [-1, -1, -1, -1]) for bl in blocks]), np.int32)
+        self.sizes = (self.nx.astype(np.int64) + 2) * (self.ny.astype(np.int64) + 2)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)[:-1]])
+        self.cells = int(np.sum(self.nx.astype(np.int64) * self.ny))
+        self.mode = mode
+        h = C.c_void_p()
+        _check(lib.swe_hip_batch_create(self.n, _iptr(self.nx), _iptr(self.ny), _fptr(self.dx), _fptr(self.dy),
+                                        _iptr(self.bnd), _iptr(self.nbr), solver, mode, max_local, duration,
+                                        C.byref(h)), "batch_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib.swe_hip_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def pack(self, arrays):
+        """Concatenate per-block Float2D arrays into the packed layout."""
+        for k, a in enumerate(arrays):
+            if a.shape != (self.nx[k] + 2, self.ny[k] + 2):
+                raise ValueError(f"block {k}: shape {a.shape}")
+        return np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).ravel() for a in arrays]))
+
+    def unpack(self, flat):
+        return [flat[o:o + s].reshape(self.nx[k] + 2, self.ny[k] + 2)
+                for k, (o, s) in enumerate(zip(self.offsets, self.sizes))]
+
+    def upload(self, h, hu, hv, b) -> None:
+        """Per-block lists of (nx+2, ny+2) arrays, or packed flat arrays."""
+        arrs = [a if isinstance(a, np.ndarray) and a.ndim == 1 else self.pack(a) for a in (h, hu, hv, b)]
+        arrs = [np.ascontiguousarray(a, np.float32) for a in arrs]
+        self._keep = arrs
+        _check(lib.swe_hip_batch_upload(self._h, *[_fptr(a) for a in arrs]), "batch_upload")
+
+    def download(self, with_b: bool = False, packed: bool = False):
+        tot = int(np.sum(self.sizes))
+        out = [np.empty(tot, np.float32) for _ in range(4 if with_b else 3)]
+        ptrs = [_fptr(a) for a in out] + ([None] if not with_b else [])
+        _check(lib.swe_hip_batch_download(self._h, *ptrs), "batch_download")
+        return tuple(out) if packed else tuple(self.unpack(a) for a in out)
+
+    def run(self, units: int):
+        """Advance `units` steps (ensemble / global) or macro steps (LTS).
+        Returns (dt_log, iter_log, elapsed_ms)."""
+        dts = np.zeros(max(units, 1), np.float32)
+        its = np.zeros(max(units, 1), np.int32)
+        ms = C.c_float()
+        _check(lib.swe_hip_batch_run(self._h, int(units), _fptr(dts), _iptr(its), C.byref(ms)), "batch_run")
+        return dts[:units], its[:units], ms.value
+
+    def stats(self):
+        upd = np.zeros(self.n, np.int64)
+        t = np.zeros(self.n, np.float32)
+        _check(lib.swe_hip_batch_stats(self._h, upd.ctypes.data_as(C.POINTER(C.c_int64)), _fptr(t)), "batch_stats")
+        return upd, t
+
+    def set_tuning(self, xs: int = -1, groups: int = -1, chunk: int = -1, graph: int = -1) -> None:
+        _check(lib.swe_hip_batch_set_tuning(self._h, xs, groups, chunk, graph), "batch_set_tuning")
+
+    def get_tuning(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(lib.swe_hip_batch_get_tuning(self._h, *[C.byref(x) for x in v]), "batch_get_tuning")
+        return tuple(x.value for x in v)
+
+    def autotune(self, iterations: int = 32):
+        ms = np.zeros(16, np.float32)
+        n = C.c_int()
+        _check(lib.swe_hip_batch_autotune(self._h, int(iterations), _fptr(ms), C.byref(n)), "batch_autotune")
+        return ms[:n.value]
+
+
+def _iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def solve_edges(solver: int, hL, hR, huL, huR, bL, bR, math: int = 0):

@@ -197,15 +197,17 @@ def _split_sizes(rng, total: int, parts: int, lo: int, hi: int, quantum: int = 1
 def lts_tiling(seed: int = 28, n: int = 2048, cols: int = 4, rows: int = 7, length: float = 1000.0,
                depth_lo: float = 2.0, depth_hi: float = 40.0):
     """28 blocks (cols x rows) of uneven size tiling n x n: column widths in
-    [128, 1024] and row heights in [128, 512], multiples of 128, each block at
-    its own still-water depth ~ U(depth_lo, depth_hi) over flat b = 0.
+    [n/16, n/2] and row heights in [n/16, n/4], multiples of n/16 (at n = 2048:
+    128..1024 x 128..512), each block at its own still-water depth
+    ~ U(depth_lo, depth_hi) over flat b = 0.
     Neighbours share whole edges, as the reference's in-rank ghost copy
     requires (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304).
     Block k = c*rows + r; its neighbours follow getNeighbours
     (swe_mpi_overdecomp.cpp:41-49)."""
     rng = np.random.default_rng(seed)
-    widths = _split_sizes(rng, n, cols, 128, 1024)
-    heights = _split_sizes(rng, n, rows, 128, 512)
+    q = n // 16
+    widths = _split_sizes(rng, n, cols, q, n // 2, quantum=q)
+    heights = _split_sizes(rng, n, rows, q, n // 4, quantum=q)
     depths = rng.uniform(depth_lo, depth_hi, size=cols * rows)
     dx = np.float32(length / n)
     blocks = []
