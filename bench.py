@@ -31,6 +31,19 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 28.0  # read h, hu, hv, b + write h, hu, hv (FP32)
 
 
+def measured_traffic(workload_file: str = "fused_u_augrie_2048_islands.md"):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a plain
+    copy), or None if the summary is absent."""
+    import re
+
+    path = ROOT / "profiles" / workload_file
+    if not path.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", path.read_text(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def cpu_threads() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -78,6 +91,70 @@ def cpu_baseline(h, hu, hv, b, dx, dy, bnd, solver, budget_s: float):
                       f"({elapsed:.1f} s, OpenMP {threads} threads, C++ oracle -O2)"}
 
 
+def batched_lts(swe, scenarios, autotune: bool, seed: int = 28, n: int = 2048, T: float = 1.0,
+                duration: float = 80.0):
+    """28 uneven blocks tiling n x n, per-block depth U(2,40) m, AugRie, local
+    time stepping with macro step T up to `duration` (one unit = one macro step)."""
+    blocks, dx, dy = scenarios.lts_tiling(seed=seed, n=n)
+    nmacro = int(round(duration / T))
+    fields = [[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")]
+    with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=swe.BATCH_LTS, dx=dx, dy=dy, max_local=T,
+                      duration=duration) as bt:
+        bt.upload(*fields)
+        cand = bt.autotune(iterations=32) if autotune else None
+        bt.upload(*fields)
+        bt.run(1)  # warm-up macro step
+        bt.upload(*fields)
+        _, its, ms = bt.run(nmacro)
+        upd, _ = bt.stats()
+        xs, groups, chunk, graph = bt.get_tuning()
+        h = bt.download(packed=True)[0]
+    if not np.all(np.isfinite(h)):
+        raise RuntimeError("non-finite LTS state")
+    cells = (bt.nx.astype(np.int64) * bt.ny).astype(np.int64)
+    updates = int(np.sum(cells * upd))
+    return {
+        "workload": f"{len(blocks)} blocks tiling {n}x{n} (sizes {int(bt.nx.min())}..{int(bt.nx.max())} x "
+                    f"{int(bt.ny.min())}..{int(bt.ny.max())}), still water U(2,40) m per block (seed {seed}), AugRie "
+                    f"FP32, LTS macro step {T} s, {duration} s simulated ({nmacro} macro steps)",
+        "value": round(updates / (ms * 1e-3) / 1e6, 3), "unit": "MCell-updates/s",
+        "elapsed_ms": round(ms, 3), "cell_updates": updates, "driver_iterations": int(np.sum(its)),
+        "sim_s_per_wall_s": round(duration / (ms * 1e-3), 3),
+        "hbm_frac_algorithmic": round(BYTES_PER_CELL * updates / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "tuning": {"xs": xs, "groups": groups, "chunk": chunk, "graph": graph,
+                   "autotune_ms_per_iteration": None if cand is None else [round(float(c), 4) for c in cand]},
+    }
+
+
+def batched_ensemble(swe, scenarios, autotune: bool, count: int = 512, seed: int = 512, steps: int = 300):
+    """512 independent dam-break instances of uneven size (64..384)^2, each with its own dt."""
+    inst = scenarios.ensemble_instances(count=count, seed=seed)
+    fields = [[i[f] for i in inst] for f in ("h", "hu", "hv", "b")]
+    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+        bt.upload(*fields)
+        cand = bt.autotune(iterations=16) if autotune else None
+        bt.upload(*fields)
+        bt.run(5)  # warm-up
+        bt.upload(*fields)
+        _, _, ms = bt.run(steps)
+        upd, t = bt.stats()
+        xs, groups, chunk, graph = bt.get_tuning()
+        h = bt.download(packed=True)[0]
+    if not np.all(np.isfinite(h)):
+        raise RuntimeError("non-finite ensemble state")
+    updates = int(bt.cells) * steps
+    return {
+        "workload": f"{count} independent dam-break instances, NX,NY ~ U{{64..384}} ({bt.cells} cells, seed {seed}), "
+                    f"AugRie FP32, per-instance CFL dt, {steps} steps each",
+        "value": round(updates / (ms * 1e-3) / 1e6, 3), "unit": "MCell-updates/s",
+        "elapsed_ms": round(ms, 3), "cell_updates": updates,
+        "instance_time_range_s": [round(float(t.min()), 4), round(float(t.max()), 4)],
+        "hbm_frac_algorithmic": round(BYTES_PER_CELL * updates / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "tuning": {"xs": xs, "groups": groups, "chunk": chunk, "graph": graph,
+                   "autotune_ms_per_iteration": None if cand is None else [round(float(c), 4) for c in cand]},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +167,12 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--math", type=int, default=0, help="0 IEEE (bit-compatible), 1 fast")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched LTS / ensemble lines")
+    ap.add_argument("--batched-only", choices=["lts", "ensemble"], default=None,
+                    help="only run one batched workload (for profiling) and print its line")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--lts-duration", type=float, default=80.0, help="simulated seconds of the LTS workload")
+    ap.add_argument("--ens-steps", type=int, default=300, help="steps of the ensemble workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -107,11 +190,19 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+              "(the step is measured on one GPU; sharding across GPUs is not implemented)", file=sys.stderr)
         return 2
 
     import swe_ppm_amd as swe
     from swe_ppm_amd import scenarios
+
+    if args.batched_only:
+        if args.batched_only == "lts":
+            out = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
+        else:
+            out = batched_ensemble(swe, scenarios, not args.no_autotune, steps=args.ens_steps)
+        print(json.dumps(out))
+        return 0
 
     n = args.n
     h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=n, seed=args.seed)
@@ -152,6 +243,7 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, args.cpu_budget)
 
+    traffic = measured_traffic() if (strategy == 2 and n == 2048 and args.math == 0) else None
     names = {0: ("sweep_u_kernel<AUGRIE,MODE_MAX>", "sweep_u_kernel<AUGRIE,MODE_UPDATE>"),
              1: ("sweep_u_kernel<AUGRIE,MODE_STORE>", "apply_u_kernel"),
              2: ("fused_u_kernel<AUGRIE> (first launch: sweep_u_kernel<AUGRIE,MODE_STORE>)",
@@ -185,7 +277,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of fused_u_kernel on this workload "
+                              "(profiles/fused_u_augrie_2048_islands.md)" if traffic else None,
             "kernel": f"{names[0]}: {kl[1]} launches, avg {pass1_ms * 1e3:.2f} us; {names[1]}: {kl[2]} launches, "
                       f"avg {pass2_ms * 1e3:.2f} us; boundary {kms[0] * 1e3:.1f} us total; "
                       f"{step_ms_k * 1e3:.2f} us of kernel time per step",
@@ -193,6 +287,12 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_batched:
+        lts = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
+        ens = batched_ensemble(swe, scenarios, not args.no_autotune, steps=args.ens_steps)
+        for b in (lts, ens):
+            b["vs_plain"] = round(b["value"] / mcells, 4)
+        line["batched"] = {"lts_28_blocks": lts, "ensemble_512": ens}
     print(json.dumps(line))
     return 0
 

@@ -30,9 +30,10 @@
 //   maxNewton = 10, criticalTol = 1e-6, hybridTol = 1.0 (Hybrid dispatch:
 //   f-wave when both depths >= hybridTol, augmented solver otherwise).
 //
-// Parity status of each function is stated in DESIGN.md and checked by the
-// oracle tests: conservation of the wave decomposition, lake at rest, mirror
-// symmetry, exact Riemann middle states (FP64 bisection), closed forms.
+// Each function is pinned by the oracle tests (tests/test_oracle_solvers.py,
+// tests/test_oracle_scheme.py): conservation of the wave decomposition, lake
+// at rest, mirror symmetry, exact Riemann middle states (FP64 bisection),
+// closed forms, dam-break convergence to the Stoker and Ritter solutions.
 #pragma once
 
 #include <algorithm>
@@ -402,8 +403,9 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
 
 // ---------------------------------------------------------------------------
 // HLLE solver in f-wave form (Einfeldt 1988): two waves at the Einfeldt
-// speeds, the HLL middle state splitting the flux jump.  Same dry handling
-// as the f-wave solver.
+// speeds, the HLL middle state splitting the flux jump, with the HLL
+// diffusion in surface-gradient form (it acts on the jump of h + b, so a lake
+// at rest stays at rest).  Same dry handling as the f-wave solver.
 // ---------------------------------------------------------------------------
 template <typename T>
 void hlle(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,

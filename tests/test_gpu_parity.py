@@ -3,7 +3,7 @@
 Bar: the kernels keep the oracle's single-precision operation order with
 no contraction and correctly rounded division/square root, so per-edge
 results, time steps and states after N steps are required to be identical
-(relative L-infinity 0); the north-star tolerance 1e-5 is reported beside.
+(relative L-infinity 0); the 1e-5 relative L-infinity bound is checked beside.
 """
 import numpy as np
 import pytest
