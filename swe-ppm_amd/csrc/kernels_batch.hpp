@@ -93,6 +93,7 @@ struct BatchGlobals {
     float gdt;        // global dt of this iteration
     int logCap;
     long long iterations;  // iterations that did work
+    const unsigned* mlp;   // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
 };
 
 __device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d) {
@@ -187,7 +188,8 @@ __device__ __forceinline__ int next_items(int* head, int* sBase) {
 // owns rows 62*yt+1 .. 62*yt+62 (lanes 1..62, lanes 0 and 63 are halo) and
 // marches over columns 1+xs*XS .. min(xs*XS+XS, nx).
 template <int S, int XS>
-__device__ __forceinline__ float strip_sums(const StateView& v, const SumsView& sums, int yt, int xs, int lane) {
+__device__ __forceinline__ float strip_sums(const dev::SolverCtx<S>& solver, const StateView& v, const SumsView& sums,
+                                            int yt, int xs, int lane) {
     const int y = kStripRows * yt + lane;
     const bool rowValid = y <= v.ny + 1;
     const int yc = rowValid ? y : v.ny + 1;
@@ -200,16 +202,16 @@ __device__ __forceinline__ float strip_sums(const StateView& v, const SumsView& 
     CellQ cq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
     o += v.pitch;
     CellQ nq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
-    const dev::EdgeOut e = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+    const dev::EdgeOut e = solver.solve(cq.xside(), nq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
     for (int x = x0; x <= x1; ++x) {
         cq = nq;
         o += v.pitch;
         nq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
-        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+        const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
-        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(cq.yside(), cq.yside_from_lane_above());
+        const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
         if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);
@@ -233,6 +235,7 @@ __global__ __launch_bounds__(256) void batch_sweep_kernel(BatchArena A, const Ba
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int count = G->count;
+    const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? G->mlp : nullptr);
     __shared__ int sBase;
     for (;;) {
         const int item = next_items(&G->head, &sBase) + wave;
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(256) void batch_sweep_kernel(BatchArena A, const Ba
         const int tile = code & ((1 << kTileBits) - 1);
         const BatchBlockDesc& d = D[blk];
         const StateView v = batch_view(A, d);
-        const float m = strip_sums<S, XS>(v, batch_sums(A, d), tile % d.nyt, tile / d.nyt, lane);
+        const float m = strip_sums<S, XS>(solver, v, batch_sums(A, d), tile % d.nyt, tile / d.nyt, lane);
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     }
 }

@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, B
         if (dtLog) dtLog[logIdx] = dt;
     }
     if (wave >= nyt * nxs) return;  // uniform per wave
+    const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
 
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, B
 
     CellQ pq = cell_q(prev.h, prev.hu, prev.hv, cb);
     CellQ cq = cell_q(cur.h, cur.hu, cur.hv, nb);
-    dev::EdgeOut e = dev::EdgeSolver<S>::solve(pq.xside(), cq.xside());
+    dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
 
@@ -142,9 +143,9 @@ __global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, B
         if (carryRow) store(x, cur);
         if (x + 1 == nx + 1 && carryRow) store(nx + 1, nxt);
 
-        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+        const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
-        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(cq.yside(), cq.yside_from_lane_above());
+        const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
         if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);

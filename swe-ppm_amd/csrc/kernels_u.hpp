@@ -30,6 +30,7 @@
 
 #include <cfloat>
 
+#include "mlp.hpp"
 #include "solvers_dev.hpp"
 #include "swe_common.h"
 
@@ -69,6 +70,7 @@ struct DeviceCtl {
     float t;
     int steps;
     int pad[2];
+    const unsigned* mlp;  // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
 };
 
 // Which CFL-maximum slot a launch reads or accumulates into, and which slot
@@ -245,6 +247,7 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
     const int nyt = (v.ny + kStripRows - 1) / kStripRows;
     const int nxs = (v.nx + XS - 1) / XS;
     if (wave >= nyt * nxs) return;  // uniform per wave
+    const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
     const int yt = wave % nyt;
     const int xs = wave / nyt;
     const int y = kStripRows * yt + lane;  // rows y0-1 .. y0+62 with y0 = 62*yt + 1
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
     CellQ nq = cell_q(nh, nhu, nhv, nb);
 
     // halo vertical edge (x0-1 | x0)
-    dev::EdgeOut e = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+    dev::EdgeOut e = solver.solve(cq.xside(), nq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
 
@@ -308,11 +311,11 @@ __global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums
         nq = cell_q(nh, nhu, nhv, nb);
 
         // vertical edge (x | x+1)
-        const dev::EdgeOut ev = dev::EdgeSolver<S>::solve(cq.xside(), nq.xside());
+        const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
 
         // horizontal edge (y | y+1): upper cell from lane+1
-        const dev::EdgeOut eh = dev::EdgeSolver<S>::solve(cq.yside(), cq.yside_from_lane_above());
+        const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
         if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
         // up-going part of edge (y-1 | y) from lane-1
         const float aH = __shfl_up(eh.hR, 1, 64);

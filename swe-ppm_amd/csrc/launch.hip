@@ -111,6 +111,10 @@ static int sweep_u_v(int solver, int mode, int xseg, StateView v, SumsView sums,
         case SOLVER_HYBRID:
             return launch_sweep_u_solver<SOLVER_HYBRID, V>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed,
                                                            fuseBc, dtLog, logIdx, stream);
+        case SOLVER_AUGRIE_MLP:
+            // the learned middle state is built with IEEE arithmetic only
+            return launch_sweep_u_solver<SOLVER_AUGRIE_MLP, 0>(mode, xseg, v, sums, hn, hun, hvn, bi, ctl, sl,
+                                                               dtFixed, fuseBc, dtLog, logIdx, stream);
         default:
             set_error("unknown solver %d", solver);
             return -1;
@@ -160,6 +164,9 @@ static int fused_u_v(int solver, int xseg, StateView v, FusedIO io, BoundaryInfo
         case SOLVER_AUGRIE:
             return launch_fused_u_solver<SOLVER_AUGRIE, V>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
                                                            logIdx, stream);
+        case SOLVER_AUGRIE_MLP:
+            return launch_fused_u_solver<SOLVER_AUGRIE_MLP, 0>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
+                                                               logIdx, stream);
         default:
             return launch_fused_u_solver<SOLVER_HYBRID, V>(xseg, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog,
                                                            logIdx, stream);

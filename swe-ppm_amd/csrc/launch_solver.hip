@@ -120,11 +120,13 @@ int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_
 template int launch_sweep_u_solver<SWE_SOLVER_ID, SWE_VARIANT>(int, int, StateView, SumsView, float*, float*,
                                                                float*, BoundaryInfo, DeviceCtl*, Slots, float, int,
                                                                float*, int, hipStream_t);
+#if SWE_SOLVER_ID != 4
 template int launch_solve_edges_solver<SWE_SOLVER_ID, SWE_VARIANT>(int64_t, const float*, float*, hipStream_t);
+#endif
 template int launch_fused_u_solver<SWE_SOLVER_ID, SWE_VARIANT>(int, StateView, FusedIO, BoundaryInfo, DeviceCtl*, int,
                                                                int, int, float*, int, hipStream_t);
 
-#if SWE_VARIANT == 0
+#if SWE_VARIANT == 0 && SWE_SOLVER_ID != 4
 template <int S>
 int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream) {
     if (dir == 0) {

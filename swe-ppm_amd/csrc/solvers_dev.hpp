@@ -221,65 +221,84 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
 // middle_state is the default, the learned surrogate plugs in here.
 // ---------------------------------------------------------------------------
 struct ExactMiddle {
-    __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR) const {
+    __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR, float,
+                                                 float) const {
         return middle_state(hL, hR, uL, uR, cL, cR);
     }
 };
 
-template <typename MiddleFn>
-__device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin, const MiddleFn& middle) {
+// Wet/dry handling of the augmented solver (George 2008): a dry side next to
+// a wet one is either a wall (the wet side's middle state cannot reach its
+// bed: reflect) or inundated (the bed step is cut to the water surface).
+struct AugPrep {
+    float hL, huL, uL, cL, bL;
+    float hR, huR, uR, cR, bR;
+    bool wallL, wallR, dryBoth;
+};
+
+__device__ __forceinline__ AugPrep augrie_prepare(const Side& Lin, const Side& Rin) {
     const bool dryLin = Lin.h < kDryTol;
     const bool dryRin = Rin.h < kDryTol;
-    const bool dryBoth = dryLin && dryRin;
-    float hL = dryLin ? 0.f : Lin.h;
-    float huL = dryLin ? 0.f : Lin.hu;
-    float uL = dryLin ? 0.f : Lin.u;
-    float cL = dryLin ? 0.f : Lin.c;
-    float bL = Lin.b;
-    float hR = dryRin ? 0.f : Rin.h;
-    float huR = dryRin ? 0.f : Rin.hu;
-    float uR = dryRin ? 0.f : Rin.u;
-    float cR = dryRin ? 0.f : Rin.c;
-    float bR = Rin.b;
-
-    // wet/dry interface: wall or inundation
-    bool wallL = false, wallR = false;
-    if ((dryLin || dryRin) && !dryBoth) {
+    AugPrep p;
+    p.dryBoth = dryLin && dryRin;
+    p.hL = dryLin ? 0.f : Lin.h;
+    p.huL = dryLin ? 0.f : Lin.hu;
+    p.uL = dryLin ? 0.f : Lin.u;
+    p.cL = dryLin ? 0.f : Lin.c;
+    p.bL = Lin.b;
+    p.hR = dryRin ? 0.f : Rin.h;
+    p.huR = dryRin ? 0.f : Rin.hu;
+    p.uR = dryRin ? 0.f : Rin.u;
+    p.cR = dryRin ? 0.f : Rin.c;
+    p.bR = Rin.b;
+    p.wallL = false;
+    p.wallR = false;
+    if ((dryLin || dryRin) && !p.dryBoth) {
         if (dryRin) {
-            const Middle t = middle_state(hL, hL, uL, -uL, cL, cL);
-            const float hTest = smax(hL, t.hm);
-            if (hTest + bL < bR) {
-                wallR = true;
-                hR = hL;
-                huR = -huL;
-                uR = -uL;
-                cR = cL;
-                bR = bL;
-            } else if (hL + bL < bR) {
-                bR = hL + bL;
+            const Middle t = middle_state(p.hL, p.hL, p.uL, -p.uL, p.cL, p.cL);
+            const float hTest = smax(p.hL, t.hm);
+            if (hTest + p.bL < p.bR) {
+                p.wallR = true;
+                p.hR = p.hL;
+                p.huR = -p.huL;
+                p.uR = -p.uL;
+                p.cR = p.cL;
+                p.bR = p.bL;
+            } else if (p.hL + p.bL < p.bR) {
+                p.bR = p.hL + p.bL;
             }
         } else {
-            const Middle t = middle_state(hR, hR, -uR, uR, cR, cR);
-            const float hTest = smax(hR, t.hm);
-            if (hTest + bR < bL) {
-                wallL = true;
-                hL = hR;
-                huL = -huR;
-                uL = -uR;
-                cL = cR;
-                bL = bR;
-            } else if (hR + bR < bL) {
-                bL = hR + bR;
+            const Middle t = middle_state(p.hR, p.hR, -p.uR, p.uR, p.cR, p.cR);
+            const float hTest = smax(p.hR, t.hm);
+            if (hTest + p.bR < p.bL) {
+                p.wallL = true;
+                p.hL = p.hR;
+                p.huL = -p.huR;
+                p.uL = -p.uR;
+                p.cL = p.cR;
+                p.bL = p.bR;
+            } else if (p.hR + p.bR < p.bL) {
+                p.bL = p.hR + p.bR;
             }
         }
     }
+    return p;
+}
+
+template <typename MiddleFn>
+__device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin, const MiddleFn& middle) {
+    const AugPrep p = augrie_prepare(Lin, Rin);
+    const bool dryBoth = p.dryBoth;
+    const bool wallL = p.wallL, wallR = p.wallR;
+    const float hL = p.hL, huL = p.huL, uL = p.uL, cL = p.cL, bL = p.bL;
+    const float hR = p.hR, huR = p.huR, uR = p.uR, cR = p.cR, bR = p.bR;
 
     const float uHat = (cL * uL + cR * uR) / (cL + cR);
     const float cHat = sqrtf(kGravity * 0.5f * (hR + hL));
     const float sE1 = smin(uL - cL, uHat - cHat);
     const float sE2 = smax(uR + cR, uHat + cHat);
 
-    const Middle mid = middle(hL, hR, uL, uR, cL, cR);
+    const Middle mid = middle(hL, hR, uL, uR, cL, cR, bL, bR);
 
     const float l1 = smin(sE1, mid.s2m);
     const float l3 = smax(sE2, mid.s1m);
@@ -416,6 +435,15 @@ struct EdgeSolver<SOLVER_HYBRID> {
         if (L.h >= kHybridTol && R.h >= kHybridTol) return fwave(L, R);
         return augrie(L, R);
     }
+};
+
+// Solver with per-kernel state, built once at kernel start from an optional
+// parameter block (the learned middle state keeps its network weights in
+// registers; the exact solvers have no state).
+template <int S>
+struct SolverCtx {
+    __device__ __forceinline__ explicit SolverCtx(const unsigned*) {}
+    __device__ __forceinline__ EdgeOut solve(const Side& L, const Side& R) const { return EdgeSolver<S>::solve(L, R); }
 };
 
 // Convenience for raw edge states (one-off evaluations).

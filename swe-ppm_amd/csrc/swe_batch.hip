@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/swe_hip_batch.h"
+#include "../../include/swe_hip_mlp.h"
 #include "kernels_batch.hpp"
 #include "swe_internal.h"
 
@@ -297,6 +298,8 @@ int launch_sweep(int solver, int xs, int groups, BatchArena A, const BatchBlockD
             return launch_batch_sweep_solver<SOLVER_FWAVE>(xs, groups, A, D, st, G, items, stream);
         case SOLVER_AUGRIE:
             return launch_batch_sweep_solver<SOLVER_AUGRIE>(xs, groups, A, D, st, G, items, stream);
+        case SOLVER_AUGRIE_MLP:
+            return launch_batch_sweep_solver<SOLVER_AUGRIE_MLP>(xs, groups, A, D, st, G, items, stream);
         default:
             return launch_batch_sweep_solver<SOLVER_HYBRID>(xs, groups, A, D, st, G, items, stream);
     }
@@ -326,6 +329,7 @@ struct swe_hip_batch {
     hipStream_t stream = nullptr;
     hipGraphExec_t exec = nullptr;
     bool uploaded = false;
+    const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
 };
 
 namespace {
@@ -367,7 +371,8 @@ int enqueue_iteration(swe_hip_batch* k) {
     hipLaunchKernelGGL(batch_ghost_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
     hipLaunchKernelGGL(batch_check_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG,
                        k->dItems);
-    int rc = launch_sweep(k->solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+    const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
+    int rc = launch_sweep(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(batch_round_kernel, dim3(1), blk, 0, k->stream, k->dDesc, k->dState, k->dG, k->dDtLog,
                        k->dIterLog, k->dHostStat);
@@ -666,6 +671,7 @@ int swe_hip_batch_upload(swe_hip_batch* k, const float* h, const float* hu, cons
     g.mode = k->mode;
     g.nblocks = k->nblocks;
     g.stop = 1;
+    g.mlp = k->mlp;
     HIP_TRY(hipMemcpyAsync(k->dG, &g, sizeof(g), hipMemcpyHostToDevice, k->stream));
     hipLaunchKernelGGL(batch_init_kernel, dim3(k->nblocks), dim3(kCtlThreads), 0, k->stream, k->A, k->dDesc);
     HIP_TRY(hipGetLastError());
@@ -757,6 +763,23 @@ int swe_hip_batch_set_tuning(swe_hip_batch* k, int xs, int groups, int chunk, in
     if (chunk != -1) k->chunk = chunk;
     if (graph != -1) k->useGraph = graph;
     free_graph(k);
+    return SWE_HIP_OK;
+}
+
+int swe_hip_batch_set_surrogate(swe_hip_batch* k, swe_hip_mlp* m) {
+    if (!k) {
+        set_error("swe_hip_batch_set_surrogate: null handle");
+        return SWE_HIP_EINVAL;
+    }
+    if (m && k->solver != SOLVER_AUGRIE) {
+        set_error("swe_hip_batch_set_surrogate: the learned middle state serves AugRie batches only");
+        return SWE_HIP_EINVAL;
+    }
+    const unsigned* frags = m ? swe::mlp_frags(m) : nullptr;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    HIP_TRY(hipMemcpy(&k->dG->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
+    k->mlp = frags;
+    free_graph(k);  // the captured sweep launches the other instantiation
     return SWE_HIP_OK;
 }
 

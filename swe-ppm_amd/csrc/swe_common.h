@@ -10,6 +10,9 @@ namespace swe {
 // (src/blocks/SWE_DimensionalSplittingMpi.hh:42-50: 0 HLLE, 1 f-wave,
 // 2 augmented Riemann), with 3 = Hybrid (src/blocks/SWE_DimensionalSplitting.hh:39).
 enum Solver : int { SOLVER_HLLE = 0, SOLVER_FWAVE = 1, SOLVER_AUGRIE = 2, SOLVER_HYBRID = 3 };
+// AugRie whose Newton middle state starts from a learned guess (internal id:
+// selected when a block has a surrogate network attached).
+constexpr int SOLVER_AUGRIE_MLP = 4;
 
 // Step flavours: U = every parallel block of the reference (all edges from
 // one state, unsplit update with dry clamp), S = swe_simple's split sweep.

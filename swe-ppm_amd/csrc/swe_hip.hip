@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/swe_hip.h"
+#include "../../include/swe_hip_mlp.h"
 #include "kernels_s.hpp"
 #include "kernels_u.hpp"
 #include "swe_internal.h"
@@ -60,11 +61,17 @@ struct swe_hip_block {
     int dtlogCap = 0;
     hipStream_t stream = nullptr;
     float fluxDt = -1.f;  // dt of the last flavour-S flux pass (its Q* depends on it)
+    const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
 };
 
 namespace {
 
 size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pitch; }
+
+// AugRie with a network attached runs the learned-middle-state instantiation
+int eff_solver(const swe_hip_block* k) {
+    return (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
+}
 
 StateView view(swe_hip_block* k, int buf) {
     StateView v;
@@ -158,7 +165,7 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             if (int r = launch_ctl_reset(k->ctl, slot, k->stream)) return r;
         SumsView set[2] = {sums_view(k), sums2_view(k)};
         if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
-        if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), set[0], nullptr, nullptr,
+        if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_STORE, k->xseg, view(k, k->cur), set[0], nullptr, nullptr,
                                    nullptr, k->bi, k->ctl, Slots{0, 1}, -1.f, 0, nullptr, 0, k->stream))
             return r;
         for (int s = 1; s < nsteps; ++s) {
@@ -166,7 +173,7 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             FusedIO io{k->h[k->cur], k->hu[k->cur], k->hv[k->cur], k->b, k->h[o], k->hu[o], k->hv[o],
                        set[(s - 1) & 1], set[s & 1]};
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
-            if (int r = launch_fused_u(k->solver, k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl, (s - 1) % 3,
+            if (int r = launch_fused_u(eff_solver(k), k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl, (s - 1) % 3,
                                        s % 3, (s + 1) % 3, k->dtlog, s - 1, k->stream))
                 return r;
             k->cur = o;
@@ -182,16 +189,16 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
         if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
         if (k->strategy == 0) {
             const int o = k->cur ^ 1;
-            if (int r = launch_sweep_u(k->solver, k->math, MODE_MAX, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
+            if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_MAX, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
                                        nullptr, k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, 0, nullptr, 0, k->stream))
                 return r;
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
-            if (int r = launch_sweep_u(k->solver, k->math, MODE_UPDATE, k->xseg, view(k, k->cur), sums_view(k), k->h[o],
+            if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_UPDATE, k->xseg, view(k, k->cur), sums_view(k), k->h[o],
                                        k->hu[o], k->hv[o], k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, fuse, k->dtlog, s, k->stream))
                 return r;
             k->cur = o;
         } else {
-            if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr,
+            if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr,
                                        nullptr, nullptr, k->bi, k->ctl, Slots{par, par ^ 1}, -1.f, 0, nullptr, 0, k->stream))
                 return r;
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
@@ -210,6 +217,18 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
 }
 
 }  // namespace
+
+namespace swe {
+int block_view(swe_hip_block* k, StateView* v, hipStream_t* stream) {
+    if (!k || k->flavour != FLAVOUR_U) {
+        set_error("block_view: a flavour-U block is needed");
+        return SWE_HIP_EINVAL;
+    }
+    *v = view(k, k->cur);
+    *stream = k->stream;
+    return 0;
+}
+}  // namespace swe
 
 extern "C" {
 
@@ -350,7 +369,7 @@ int swe_hip_fluxes(swe_hip_block* k, float* max_timestep) {
     unsigned int bits = 0;
     if (k->flavour == FLAVOUR_U) {
         if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
-        if (int r = launch_sweep_u(k->solver, k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
+        if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
                                    nullptr, k->bi, k->ctl, Slots{0, 1}, -1.f, 0, nullptr, 0, k->stream))
             return r;
         HIP_TRY(hipMemcpyAsync(&bits, &k->ctl->smax[0], sizeof(bits), hipMemcpyDeviceToHost, k->stream));
@@ -672,6 +691,10 @@ int swe_hip_set_math(swe_hip_block* k, int math) {
         set_error("swe_hip_set_math: the fast variant exists for flavour U only");
         return SWE_HIP_EINVAL;
     }
+    if (math == 1 && k->mlp) {
+        set_error("swe_hip_set_math: the learned middle state needs the IEEE variant");
+        return SWE_HIP_EINVAL;
+    }
     k->math = math;
     return SWE_HIP_OK;
 }
@@ -680,6 +703,23 @@ int swe_hip_get_tuning(swe_hip_block* k, int* strategy, int* xseg) {
     if (int r = check(k)) return r;
     if (strategy) *strategy = k->strategy;
     if (xseg) *xseg = k->xseg;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
+    if (int r = check(k)) return r;
+    if (m && (k->solver != SOLVER_AUGRIE || k->flavour != FLAVOUR_U)) {
+        set_error("swe_hip_set_surrogate: the learned middle state serves flavour-U AugRie blocks only");
+        return SWE_HIP_EINVAL;
+    }
+    if (m && k->math != 0) {
+        set_error("swe_hip_set_surrogate: the learned middle state needs the IEEE variant");
+        return SWE_HIP_EINVAL;
+    }
+    const unsigned* frags = m ? swe::mlp_frags(m) : nullptr;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    HIP_TRY(hipMemcpy(&k->ctl->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
+    k->mlp = frags;
     return SWE_HIP_OK;
 }
 

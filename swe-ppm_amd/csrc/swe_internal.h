@@ -10,9 +10,18 @@
 #include "kernels_fused.hpp"
 #include "kernels_u.hpp"
 
+#include "../../include/swe_hip.h"
+#include "../../include/swe_hip_batch.h"
+#include "../../include/swe_hip_mlp.h"
+
 namespace swe {
 
 void set_error(const char* fmt, ...);
+
+// Cross-unit accessors: the current state of a flavour-U block and the
+// device weight fragments of a network.
+int block_view(swe_hip_block* k, StateView* v, hipStream_t* stream);
+const unsigned* mlp_frags(swe_hip_mlp* m);
 int hip_fail(hipError_t e, const char* what);
 
 // Host copy of the flavour-U dt rule (SWE_DimensionalSplittingMpi.cpp:448-456);

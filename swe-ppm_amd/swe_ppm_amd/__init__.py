@@ -73,6 +73,23 @@ def _load() -> C.CDLL:
         "swe_hip_batch_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "swe_hip_batch_get_tuning": (C.c_int, [vp, ip, ip, ip, ip]),
         "swe_hip_batch_autotune": (C.c_int, [vp, C.c_int, fp, ip]),
+        "swe_hip_mlp_create": (C.c_int, [C.c_int64, C.c_uint64, C.POINTER(vp)]),
+        "swe_hip_mlp_destroy": (C.c_int, [vp]),
+        "swe_hip_mlp_get_params": (C.c_int, [vp, fp]),
+        "swe_hip_mlp_set_params": (C.c_int, [vp, fp]),
+        "swe_hip_mlp_log_block": (C.c_int, [vp, vp, C.c_float, C.c_uint64, C.POINTER(C.c_int64)]),
+        "swe_hip_mlp_add_samples": (C.c_int, [vp, C.c_int64, fp]),
+        "swe_hip_mlp_read_samples": (C.c_int, [vp, C.c_int64, C.c_int64, fp]),
+        "swe_hip_mlp_sample_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "swe_hip_mlp_clear_samples": (C.c_int, [vp]),
+        "swe_hip_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, fp]),
+        "swe_hip_mlp_gradient": (C.c_int, [vp, C.c_int64, fp, fp, fp]),
+        "swe_hip_mlp_infer": (C.c_int, [vp, C.c_int64, fp, fp]),
+        "swe_hip_mlp_features": (C.c_int, [C.c_int64] + [fp] * 7),
+        "swe_hip_mlp_middle": (C.c_int, [vp, C.c_int64] + [fp] * 9 + [ip]),
+        "swe_hip_mlp_solve_edges": (C.c_int, [vp, C.c_int64] + [fp] * 11),
+        "swe_hip_set_surrogate": (C.c_int, [vp, vp]),
+        "swe_hip_batch_set_surrogate": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -182,6 +199,12 @@ class SweBlock:
         """0 = IEEE (bit-compatible with the CPU oracle), 1 = fast."""
         _check(lib.swe_hip_set_math(self._h, int(math)), "set_math")
 
+    def set_surrogate(self, mlp) -> None:
+        """Attach a MiddleStateMLP (None detaches): AugRie then starts its
+        Newton iteration from the network's guess."""
+        _check(lib.swe_hip_set_surrogate(self._h, mlp._h if mlp is not None else None), "set_surrogate")
+        self._mlp = mlp
+
     def get_tuning(self):
         s, x = C.c_int(), C.c_int()
         _check(lib.swe_hip_get_tuning(self._h, C.byref(s), C.byref(x)), "get_tuning")
@@ -284,11 +307,131 @@ class SweBatch:
         _check(lib.swe_hip_batch_get_tuning(self._h, *[C.byref(x) for x in v]), "batch_get_tuning")
         return tuple(x.value for x in v)
 
+    def set_surrogate(self, mlp) -> None:
+        _check(lib.swe_hip_batch_set_surrogate(self._h, mlp._h if mlp is not None else None), "batch_set_surrogate")
+        self._mlp = mlp
+
     def autotune(self, iterations: int = 32):
         ms = np.zeros(16, np.float32)
         n = C.c_int()
         _check(lib.swe_hip_batch_autotune(self._h, int(iterations), _fptr(ms), C.byref(n)), "batch_autotune")
         return ms[:n.value]
+
+
+MLP_PARAMS = 4738
+MLP_SAMPLE_FLOATS = 8
+
+
+class MiddleStateMLP:
+    """Learned middle state of the augmented Riemann solver (include/swe_hip_mlp.h):
+    a 6-64-64-2 network with a device sample store, trained by the library's
+    own kernels, run by the solver on the matrix cores."""
+
+    def __init__(self, capacity: int = 0, seed: int = 0):
+        h = C.c_void_p()
+        _check(lib.swe_hip_mlp_create(int(capacity), int(seed), C.byref(h)), "mlp_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib.swe_hip_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(MLP_PARAMS, np.float32)
+        _check(lib.swe_hip_mlp_get_params(self._h, _fptr(p)), "mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.shape != (MLP_PARAMS,):
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        _check(lib.swe_hip_mlp_set_params(self._h, _fptr(p)), "mlp_set_params")
+
+    def log_block(self, blk: "SweBlock", keep: float = 1.0, seed: int = 0) -> int:
+        n = C.c_int64()
+        _check(lib.swe_hip_mlp_log_block(self._h, blk._h, float(keep), int(seed), C.byref(n)), "mlp_log_block")
+        return n.value
+
+    def add_samples(self, samples) -> None:
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, MLP_SAMPLE_FLOATS)
+        _check(lib.swe_hip_mlp_add_samples(self._h, s.shape[0], _fptr(s)), "mlp_add_samples")
+
+    @property
+    def count(self) -> int:
+        n = C.c_int64()
+        _check(lib.swe_hip_mlp_sample_count(self._h, C.byref(n)), "mlp_sample_count")
+        return n.value
+
+    def read_samples(self, first: int = 0, n: int = -1) -> np.ndarray:
+        if n < 0:
+            n = self.count - first
+        out = np.empty((n, MLP_SAMPLE_FLOATS), np.float32)
+        _check(lib.swe_hip_mlp_read_samples(self._h, int(first), int(n), _fptr(out)), "mlp_read_samples")
+        return out
+
+    def clear_samples(self) -> None:
+        _check(lib.swe_hip_mlp_clear_samples(self._h), "mlp_clear_samples")
+
+    def train(self, steps: int, batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> np.ndarray:
+        loss = np.zeros(max(steps, 1), np.float32)
+        _check(lib.swe_hip_mlp_train(self._h, int(steps), int(batch), float(lr), int(seed), _fptr(loss)), "mlp_train")
+        return loss[:steps]
+
+    def gradient(self, samples):
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, MLP_SAMPLE_FLOATS)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib.swe_hip_mlp_gradient(self._h, s.shape[0], _fptr(s), _fptr(g), C.byref(loss)), "mlp_gradient")
+        return g, loss.value
+
+    def infer(self, features) -> np.ndarray:
+        x = np.ascontiguousarray(features, np.float32).reshape(-1, 6)
+        out = np.empty((x.shape[0], 2), np.float32)
+        _check(lib.swe_hip_mlp_infer(self._h, x.shape[0], _fptr(x), _fptr(out)), "mlp_infer")
+        return out
+
+    def middle(self, hL, hR, uL, uR, bL=None, bR=None):
+        ins = [np.ascontiguousarray(np.atleast_1d(a), np.float32) for a in (hL, hR, uL, uR)]
+        n = ins[0].size
+        bL = np.zeros(n, np.float32) if bL is None else np.ascontiguousarray(bL, np.float32)
+        bR = np.zeros(n, np.float32) if bR is None else np.ascontiguousarray(bR, np.float32)
+        outs = [np.empty(n, np.float32) for _ in range(3)]
+        acc = np.empty(n, np.int32)
+        _check(lib.swe_hip_mlp_middle(self._h, n, *[_fptr(a) for a in ins], _fptr(bL), _fptr(bR),
+                                      *[_fptr(a) for a in outs], _iptr(acc)), "mlp_middle")
+        return (*outs, acc)
+
+    def solve_edges(self, hL, hR, huL, huR, bL, bR):
+        ins = [np.ascontiguousarray(np.atleast_1d(a), dtype=np.float32) for a in (hL, hR, huL, huR, bL, bR)]
+        n = ins[0].size
+        outs = [np.empty(n, np.float32) for _ in range(5)]
+        _check(lib.swe_hip_mlp_solve_edges(self._h, n, *[_fptr(a) for a in ins], *[_fptr(a) for a in outs]),
+               "mlp_solve_edges")
+        return tuple(outs)
+
+
+def mlp_features(hL, hR, uL, uR, bL, bR) -> np.ndarray:
+    """The device feature function (6 floats per edge)."""
+    ins = [np.ascontiguousarray(np.atleast_1d(a), np.float32) for a in (hL, hR, uL, uR, bL, bR)]
+    n = ins[0].size
+    out = np.empty((n, 6), np.float32)
+    _check(lib.swe_hip_mlp_features(n, *[_fptr(a) for a in ins], _fptr(out)), "mlp_features")
+    return out
 
 
 def _iptr(a: np.ndarray):

@@ -228,6 +228,20 @@ def lts_tiling(seed: int = 28, n: int = 2048, cols: int = 4, rows: int = 7, leng
     return blocks, float(dx), float(dx)
 
 
+def lts_tiling_as_block(seed: int = 28, n: int = 2048):
+    """The state of lts_tiling as one n x n block (each tile at its own still
+    water depth), outflow boundaries."""
+    blocks, dx, dy = lts_tiling(seed=seed, n=n)
+    h = np.zeros((n + 2, n + 2), np.float32)
+    for bl in blocks:
+        x0 = int(round(bl["ox"] / dx))
+        y0 = int(round(bl["oy"] / dy))
+        h[1 + x0:1 + x0 + bl["nx"], 1 + y0:1 + y0 + bl["ny"]] = bl["h"][1:-1, 1:-1]
+    b = np.zeros_like(h)
+    bnd = [OUTFLOW] * 4
+    return (*finish(h, b, bnd), dx, dy, bnd)
+
+
 def trench_hump(n: int = 4096, seed: int = 4096, length: float = 100000.0):
     """Trench: still-water depth 10 m far away, 4000 m on a seeded axis, with
     a Gaussian cross-section; a 2 m Gaussian sea-surface hump at a seeded

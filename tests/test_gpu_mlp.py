@@ -1,0 +1,174 @@
+"""GPU tests of the learned middle state (include/swe_hip_mlp.h).
+
+Training kernels: gradients against the float64 oracle (oracle/mlp_oracle.py,
+itself pinned by finite differences), relative error bound 1e-4 in norm
+(FP32 sums in another order).  Inference on the matrix cores: against the
+oracle's BF16 emulation, bound 1e-3 of the output scale.  Logged samples:
+the exact middle state of the scaled Riemann problem (FP64 oracle), 2e-5
+relative.  Surrogate solver: within the exact solver's own FP32-vs-FP64
+error.  And what the trained network gains: the share of Newton edges whose
+corrected guess passes the residual check, trained against untrained.
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_params(seed, scale):
+    return np.random.default_rng(seed).normal(0, scale, M.NPARAMS).astype(np.float32)
+
+
+def test_training_gradient_matches_oracle(swe):
+    rng = np.random.default_rng(3)
+    p = _random_params(4, 0.25)
+    samples = np.concatenate([rng.normal(0, 1, (3000, 6)), rng.normal(0, 1, (3000, 2))], axis=1).astype(np.float32)
+    with swe.MiddleStateMLP(capacity=0) as mlp:
+        mlp.params = p
+        g, loss = mlp.gradient(samples)
+    lref, gref = M.loss_and_grad(p, samples)
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert np.linalg.norm(g - gref) <= 1e-4 * np.linalg.norm(gref)
+    for name, a, b in (("W1", M.OFF_W1, M.OFF_B1), ("W2", M.OFF_W2, M.OFF_B2), ("W3", M.OFF_W3, M.OFF_B3)):
+        assert np.linalg.norm(g[a:b] - gref[a:b]) <= 1e-4 * np.linalg.norm(gref[a:b]), name
+
+
+def test_matrix_core_inference_matches_bf16_emulation(swe):
+    rng = np.random.default_rng(8)
+    p = _random_params(9, 0.2)
+    x = rng.normal(0, 1, (1000, 6)).astype(np.float32)  # 15 full waves and a ragged one
+    with swe.MiddleStateMLP(capacity=0) as mlp:
+        mlp.params = p
+        y = mlp.infer(x)
+    ye = M.infer_bf16(p, x)
+    yf, _ = M.forward(p, x)
+    scale = np.max(np.abs(ye))
+    assert np.max(np.abs(y - ye)) <= 1e-3 * scale
+    assert np.max(np.abs(y - yf)) <= 3e-2 * scale  # and BF16 stays close to the FP32 network
+
+
+def test_feature_function_bitwise(swe):
+    rng = np.random.default_rng(2)
+    n = 5000
+    hL, hR = rng.uniform(0.02, 50, n), rng.uniform(0.02, 50, n)
+    uL, uR = rng.normal(0, 3, n), rng.normal(0, 3, n)
+    bL, bR = rng.normal(0, 5, n), rng.normal(0, 5, n)
+    got = swe.mlp_features(hL, hR, uL, uR, bL, bR)
+    want, _, _ = M.features(hL, hR, uL, uR, bL, bR)
+    assert np.array_equal(got, want)
+
+
+def test_logged_samples_are_exact_middle_states(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=96, seed=5)
+    with swe.SweBlock(96, 96, dx, dy, solver=swe.AUGRIE) as blk, swe.MiddleStateMLP(capacity=100_000) as mlp:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        blk.run(15, want_dts=False)
+        n_all = mlp.log_block(blk, keep=1.0, seed=1)
+        n_half = mlp.log_block(blk, keep=0.5, seed=2)
+        s = mlp.read_samples()
+    edges = 2 * 96 * 97
+    assert 0.5 * edges < n_all < edges  # most edges of the wet dam break need the iteration
+    assert abs(n_half - 0.5 * n_all) < 4 * np.sqrt(n_all)
+    # the middle state of the scaled problem (hs = 1, cs = sqrt(g)) equals h*/hs;
+    # targets are stored as corrections to the linearised middle state
+    x, t = s[:, :6].astype(np.float64), (s[:, 6:] + M.linear_part(s[:, :6])).astype(np.float64)
+    sg = np.sqrt(9.81)
+    hm, s1, s2, kind, _ = oracle.middle_state(x[:, 0], x[:, 1], x[:, 2] * sg, x[:, 3] * sg)
+    assert np.all(kind >= 2)  # only Newton edges are logged
+    assert np.max(np.abs(t[:, 0] - hm) / hm) <= 2e-5
+    assert np.max(np.abs(t[:, 1] - 0.5 * (s1 + s2) / sg)) <= 2e-5 * (1 + np.max(np.abs(t[:, 1])))
+
+
+def _trained_network(swe, seed=0):
+    """Log a few small runs, train briefly."""
+    from swe_ppm_amd import scenarios
+
+    mlp = swe.MiddleStateMLP(capacity=2_000_000, seed=seed)
+    for st, solver in ((scenarios.dam_break_islands(n=256, seed=2048), swe.AUGRIE),
+                       (scenarios.lts_tiling_as_block(seed=28, n=256), swe.AUGRIE),
+                       (scenarios.dam_break_c1(128), swe.FWAVE)):
+        h, hu, hv, b, dx, dy, bnd = st
+        with swe.SweBlock(h.shape[0] - 2, h.shape[1] - 2, dx, dy, solver=solver) as blk:
+            blk.set_boundaries(bnd)
+            blk.upload(h, hu, hv, b)
+            for k in range(12):
+                mlp.log_block(blk, keep=0.5, seed=100 * seed + k)
+                blk.run(8, want_dts=False)
+    loss = mlp.train(1500, batch=4096, lr=3e-3, seed=seed)
+    return mlp, loss
+
+
+def _held_out_edges(oracle):
+    """Newton edges of another scenario (islands seed 9, 192^2, after 40 steps)."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=192, seed=9)
+    h, hu, hv, _ = oracle.run_u(h, hu, hv, b, dx, dy, bnd, 2, 40, nthreads=8)
+    hL, hR = h[:-1, 1:-1].ravel(), h[1:, 1:-1].ravel()
+    uL = np.where(hL > 0, hu[:-1, 1:-1].ravel() / np.maximum(hL, 1e-30), 0)
+    uR = np.where(hR > 0, hu[1:, 1:-1].ravel() / np.maximum(hR, 1e-30), 0)
+    bL, bR = b[:-1, 1:-1].ravel(), b[1:, 1:-1].ravel()
+    hm, s1, s2, kind, iters = oracle.middle_state(hL.astype(np.float64), hR, uL, uR)
+    sel = kind >= 2
+    return [a[sel].astype(np.float32) for a in (hL, hR, uL, uR, bL, bR)], iters[sel]
+
+
+def test_training_reduces_loss_and_trained_network_passes_the_residual_check(swe, oracle):
+    edges, iters = _held_out_edges(oracle)
+    trained, loss = _trained_network(swe)
+    with trained, swe.MiddleStateMLP(capacity=0, seed=99) as untrained:
+        assert np.mean(loss[-50:]) < 0.05 * loss[0]
+        hm_t, s1_t, s2_t, acc_t = trained.middle(*edges)
+        hm_u, _, _, acc_u = untrained.middle(*edges)
+    exact = oracle.middle_state(*[a.astype(np.float64) for a in edges[:4]])
+    rate_t, rate_u = float(np.mean(acc_t)), float(np.mean(acc_u))
+    # the gain: with the trained network most Newton edges need one evaluation
+    # instead of the exact iteration (mean iterations of the exact path stated beside)
+    print(f"accepted: trained {rate_t:.4f}, untrained {rate_u:.4f}; exact Newton iterations "
+          f"mean {np.mean(iters):.2f}")
+    assert rate_t >= 0.9
+    assert rate_t >= rate_u + 0.2
+    # accepted middle depths are as close to the FP64 root as the exact FP32 iteration's
+    rel = np.abs(hm_t - exact[0]) / exact[0]
+    assert np.max(rel) <= 2e-5
+
+
+def test_surrogate_block_matches_exact_solver_to_its_fp32_error(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    trained, _ = _trained_network(swe, seed=1)
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=128, seed=3)
+    n = 60
+    out = {}
+    with trained:
+        for name, net in (("exact", None), ("surrogate", trained)):
+            with swe.SweBlock(128, 128, dx, dy, solver=swe.AUGRIE) as blk:
+                blk.set_boundaries(bnd)
+                blk.upload(h, hu, hv, b)
+                blk.set_surrogate(net)
+                blk.run(n, want_dts=False)
+                out[name] = blk.download()
+        # per-edge solver with the learned middle state against the exact one
+        rng = np.random.default_rng(4)
+        k = 20000
+        hL, hR = rng.uniform(0.5, 20, k).astype(np.float32), rng.uniform(0.5, 20, k).astype(np.float32)
+        huL, huR = (hL * rng.normal(0, 2, k)).astype(np.float32), (hR * rng.normal(0, 2, k)).astype(np.float32)
+        bL, bR = rng.normal(0, 0.2, k).astype(np.float32), rng.normal(0, 0.2, k).astype(np.float32)
+        ge = swe.solve_edges(swe.AUGRIE, hL, hR, huL, huR, bL, bR)
+        gs = trained.solve_edges(hL, hR, huL, huR, bL, bR)
+    for a, c in zip(ge, gs):
+        assert np.max(np.abs(a - c)) <= 1e-4 * (1 + np.max(np.abs(a)))
+    # both FP32 runs against the FP64 oracle run of the same scheme
+    ref = oracle.run_u(h, hu, hv, b, dx, dy, bnd, 2, n, nthreads=8, dtype=np.float64)
+    err = {}
+    for name, (gh, ghu, ghv) in out.items():
+        err[name] = max(np.max(np.abs(gh[1:-1, 1:-1] - ref[0][1:-1, 1:-1])) / np.max(np.abs(ref[0])),
+                        np.max(np.abs(ghu[1:-1, 1:-1] - ref[1][1:-1, 1:-1])) / np.max(np.abs(ref[1])))
+    d = np.max(np.abs(out["exact"][0] - out["surrogate"][0])) / np.max(np.abs(out["exact"][0]))
+    print(f"rel Linf vs FP64: exact {err['exact']:.3e}, surrogate {err['surrogate']:.3e}; between {d:.3e}")
+    assert err["surrogate"] <= 1.5 * err["exact"] + 1e-5
