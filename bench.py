@@ -155,6 +155,49 @@ def batched_ensemble(swe, scenarios, autotune: bool, count: int = 512, seed: int
     }
 
 
+def surrogate_compare(swe, state, weights, steps):
+    """Plain AugRie against AugRie with the learned middle state on one
+    workload, `steps` global steps each: MCell-updates/s of both."""
+    from swe_ppm_amd import surrogate
+
+    h, hu, hv, b, dx, dy, bnd = state
+    n = h.shape[0] - 2
+    res = {}
+    mlp = surrogate.load_default(path=weights)
+    for name, net in (("plain", None), ("surrogate", mlp)):
+        with swe.SweBlock(n, n, dx, dy, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            blk.set_surrogate(net)
+            blk.upload(h, hu, hv, b)
+            blk.run(10, want_dts=False)  # warm-up
+            blk.upload(h, hu, hv, b)
+            t_sim, total_ms, _, _ = timed_run(swe, blk, steps, per_kernel=False)
+            gh = blk.download()[0]
+        if not np.all(np.isfinite(gh)):
+            raise RuntimeError(f"non-finite trench state ({name})")
+        res[name] = {"value": round(n * n * steps / (total_ms * 1e-3) / 1e6, 3), "ms_per_step": round(total_ms / steps, 6),
+                     "simulated_s": round(t_sim, 4)}
+    mlp.close()
+    return {"unit": "MCell-updates/s", "plain": res["plain"], "surrogate": res["surrogate"],
+            "surrogate_vs_plain": round(res["surrogate"]["value"] / res["plain"]["value"], 4),
+            "weights": str(Path(weights).name)}
+
+
+def surrogate_trench(swe, scenarios, weights, n: int = 4096, steps: int = 1000, seed: int = 4096):
+    """The n x n trench (depth 10..4000 m, 2 m hump), plain and learned."""
+    out = {"workload": f"{n}x{n} trench (depth 10 m far field, 4000 m on the axis, Gaussian profile) with a 2 m "
+                       f"Gaussian sea-surface hump (seed {seed}), AugRie FP32, outflow, {steps} steps"}
+    out.update(surrogate_compare(swe, scenarios.trench_hump(n=n, seed=seed), weights, steps))
+    return out
+
+
+def surrogate_islands(swe, scenarios, weights, n: int = 2048, steps: int = 500, seed: int = 2048):
+    """The plain line's dam break over islands, plain and learned."""
+    out = {"workload": f"{n}x{n} dam break over 16 bumps with 3 dry islands (seed {seed}), AugRie FP32, {steps} steps"}
+    out.update(surrogate_compare(swe, scenarios.dam_break_islands(n=n, seed=seed), weights, steps))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,11 +211,16 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--math", type=int, default=0, help="0 IEEE (bit-compatible), 1 fast")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched LTS / ensemble lines")
-    ap.add_argument("--batched-only", choices=["lts", "ensemble"], default=None,
+    ap.add_argument("--batched-only", choices=["lts", "ensemble", "surrogate"], default=None,
                     help="only run one batched workload (for profiling) and print its line")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--lts-duration", type=float, default=80.0, help="simulated seconds of the LTS workload")
     ap.add_argument("--ens-steps", type=int, default=300, help="steps of the ensemble workload")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-middle-state line")
+    ap.add_argument("--surrogate-weights", type=Path, default=None,
+                    help="trained network (.npy); default: the packaged weights")
+    ap.add_argument("--trench-n", type=int, default=4096)
+    ap.add_argument("--trench-steps", type=int, default=1000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -196,6 +244,15 @@ def main() -> int:
     import swe_ppm_amd as swe
     from swe_ppm_amd import scenarios
 
+    weights = args.surrogate_weights
+    if weights is None:
+        from swe_ppm_amd import surrogate as _sur
+
+        weights = _sur.DEFAULT_WEIGHTS
+    if args.batched_only == "surrogate":
+        print(json.dumps({"trench": surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps),
+                          "islands": surrogate_islands(swe, scenarios, weights, steps=args.steps)}))
+        return 0
     if args.batched_only:
         if args.batched_only == "lts":
             out = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
@@ -293,6 +350,13 @@ def main() -> int:
         for b in (lts, ens):
             b["vs_plain"] = round(b["value"] / mcells, 4)
         line["batched"] = {"lts_28_blocks": lts, "ensemble_512": ens}
+    if not args.no_surrogate:
+        if Path(weights).exists():
+            line["surrogate"] = {
+                "trench": surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps),
+                "islands": surrogate_islands(swe, scenarios, weights, n=n, steps=args.steps, seed=args.seed)}
+        else:
+            line["surrogate"] = {"skipped": f"no trained network at {weights}"}
     print(json.dumps(line))
     return 0
 

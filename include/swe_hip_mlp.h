@@ -5,9 +5,10 @@
  * its middle state by Newton iteration inside the un-vendored solver
  * submodule (called from SWE_DimensionalSplittingMpi.cpp:403-410, 432-439).
  * This interface trains a small network on middle states that the library's
- * own exact solver logs, and lets a block's AugRie start its Newton iteration
- * from the network's guess (one FP32 evaluation as residual check, exact
- * iteration as fallback).
+ * own exact solver logs, and lets a block's AugRie replace its Newton
+ * iteration by a guess (linearised middle state, else the network's) with one
+ * FP32 Newton evaluation as residual check and the exact iteration as
+ * fallback.
  *
  * Network: 6 -> 64 -> 64 -> 2, ReLU.  Inputs of an edge with both sides wet
  * (after the solver's wet/dry handling), with hs = (hL + hR)/2 and
@@ -53,11 +54,13 @@ int swe_hip_mlp_set_params(swe_hip_mlp* m, const float* params);
 
 /* Log samples from the current state of a flavour-U block: every edge of the
  * block whose AugRie middle state needs the Newton iteration (both sides wet,
- * not two rarefactions) is kept with probability `keep` (seeded hash of the
- * edge index) and its inputs and exact FP32 middle state are appended to the
- * store (up to its capacity).  *added (may be NULL) receives the number
- * appended. */
-int swe_hip_mlp_log_block(swe_hip_mlp* m, swe_hip_block* blk, float keep, uint64_t seed, int64_t* added);
+ * not two rarefactions) -- with hard_only != 0 only those whose linearised
+ * guess fails the residual check -- is kept with probability `keep` (seeded
+ * hash of the edge index) and its inputs and exact FP32 middle state are
+ * appended to the store (up to its capacity).  *added (may be NULL) receives
+ * the number appended. */
+int swe_hip_mlp_log_block(swe_hip_mlp* m, swe_hip_block* blk, float keep, uint64_t seed, int hard_only,
+                          int64_t* added);
 
 /* Append host samples / read samples back / count / empty the store. */
 int swe_hip_mlp_add_samples(swe_hip_mlp* m, int64_t n, const float* samples);
@@ -85,9 +88,11 @@ int swe_hip_mlp_features(int64_t n, const float* hL, const float* hR, const floa
                          const float* bL, const float* bR, float* features);
 
 /* Middle states of n edges (hL, hR, uL, uR, bL, bR host arrays) as the
- * surrogate solver computes them: network guess, residual check, exact
- * fallback.  accepted[k] = 1 where the corrected guess was kept, 0 where the
- * exact iteration ran (or the edge needed none). */
+ * surrogate solver computes them: linearised guess, network guess for the
+ * waves (64 consecutive edges) where it fails somewhere, residual check,
+ * exact fallback.  accepted[k] = 1 where the corrected linearised guess was
+ * kept, 2 where the corrected network guess was kept, 0 where the exact
+ * middle state was computed (no iteration needed, or fallback). */
 int swe_hip_mlp_middle(swe_hip_mlp* m, int64_t n, const float* hL, const float* hR, const float* uL, const float* uR,
                        const float* bL, const float* bR, float* hm, float* s1m, float* s2m, int32_t* accepted);
 

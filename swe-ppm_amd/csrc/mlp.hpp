@@ -2,11 +2,12 @@
 //
 // A 6 -> 64 -> 64 -> 2 ReLU network predicts the middle depth h* and velocity
 // u* of an edge's Riemann problem from scaled edge states.  In the solver it
-// replaces the Newton iteration's starting point: the guess gets one FP32
-// Newton evaluation (the residual check); if the resulting step is small the
-// corrected depth is taken, otherwise the exact iteration runs from its usual
-// start (hMin or hMax).  Dry, near-dry and two-rarefaction edges never use the
-// network.
+// replaces the Newton iteration: a guess gets one FP32 Newton evaluation (the
+// residual check); if the resulting step is small the corrected depth is
+// taken, otherwise the exact iteration runs from its usual start (hMin or
+// hMax).  The linearised (acoustic) middle state is tried first; the network
+// runs for the waves in which it fails somewhere.  Dry, near-dry and
+// two-rarefaction edges never use either guess.
 //
 // Inference runs on the matrix cores: the 64 lanes of a wave hold 64 edges and
 // each layer is a chain of v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32
@@ -84,30 +85,18 @@ __device__ __forceinline__ void linear_part(const float* x, float& l0, float& l1
     l1 = 0.5f * (x[2] + x[3]) - 0.5f * (x[1] - x[0]);
 }
 
-struct MlpFrags {
-    u32x4 l1[2];
-    u32x4 l2[2][5];
-    u32x4 l3[4];
-    float b3[2];
-
-    __device__ __forceinline__ static u32x4 ld4(const unsigned* f, int d, int lane) {
-        u32x4 r;
-        r.x = f[(d + 0) * 64 + lane];
-        r.y = f[(d + 1) * 64 + lane];
-        r.z = f[(d + 2) * 64 + lane];
-        r.w = f[(d + 3) * 64 + lane];
-        return r;
-    }
-    __device__ __forceinline__ void load(const unsigned* f) {
-        const int lane = threadIdx.x & 63;
-        for (int t = 0; t < 2; ++t) l1[t] = ld4(f, kFragL1 + 4 * t, lane);
-        for (int t = 0; t < 2; ++t)
-            for (int c = 0; c < 5; ++c) l2[t][c] = ld4(f, kFragL2 + 4 * (5 * t + c), lane);
-        for (int c = 0; c < 4; ++c) l3[c] = ld4(f, kFragL3 + 4 * c, lane);
-        b3[0] = __uint_as_float(f[kFragB3 * 64 + lane]);
-        b3[1] = __uint_as_float(f[(kFragB3 + 1) * 64 + lane]);
-    }
-};
+// Four fragment dwords of this lane (global memory; the network is only
+// evaluated for waves in which the linearised guess failed, so its weights
+// are fetched at use instead of occupying registers for the whole kernel).
+__device__ __forceinline__ u32x4 frag4(const unsigned* __restrict__ f, int d) {
+    const int lane = threadIdx.x & 63;
+    u32x4 r;
+    r.x = f[(d + 0) * 64 + lane];
+    r.y = f[(d + 1) * 64 + lane];
+    r.z = f[(d + 2) * 64 + lane];
+    r.w = f[(d + 3) * 64 + lane];
+    return r;
+}
 
 __device__ __forceinline__ bf16x8 as_bf16(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ __forceinline__ u32x4 as_u32(bf16x8 v) { return __builtin_bit_cast(u32x4, v); }
@@ -130,15 +119,25 @@ __device__ __forceinline__ bf16x8 relu_chunk(const f32x16& d, int base) {
     return r;
 }
 
+__device__ __forceinline__ f32x16 mfma(u32x4 a, bf16x8 b, const f32x16& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(a), b, c, 0, 0, 0);
+}
+
 // Network outputs (y0, y1) for the edge of every lane.  Must be called with
-// the whole wave active (uniform control flow).
-__device__ __forceinline__ void infer(const MlpFrags& F, const float* x, float& y0, float& y1) {
+// the whole wave active (uniform control flow).  Per edge tile the layers
+// are chained output tile by output tile, so that at most three accumulators
+// (layer 1, layer 2 tile, layer 3) are live.  Not inlined: the solver calls
+// it only for waves where the linearised guess failed, and a call keeps the
+// network's registers out of the sweep's allocation (inlined, the sweep
+// kernels needed four times the registers of the exact solver).
+__device__ __attribute__((noinline)) float2 infer_wave(const unsigned* __restrict__ F, float x0, float x1, float x2,
+                                                       float x3, float x4, float x5) {
     const int lane = threadIdx.x & 63;
     const bool upper = lane >= 32;
     // inputs: 6 features, the constant 1 (carries b1), 0; split into a BF16
     // high part and a BF16 residual so that the 16-deep K of layer 1 sees
     // about 16 significant bits of every input
-    const float v[8] = {x[0], x[1], x[2], x[3], x[4], x[5], 1.f, 0.f};
+    const float v[8] = {x0, x1, x2, x3, x4, x5, 1.f, 0.f};
     bf16x8 hi, lo;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -157,82 +156,114 @@ __device__ __forceinline__ void infer(const MlpFrags& F, const float* x, float& 
     float out[2][2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-        f32x16 d1[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-            d1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(F.l1[t]), as_bf16(B[e]), zero, 0, 0, 0);
-        f32x16 d2[2];
-#pragma unroll
-        for (int t2 = 0; t2 < 2; ++t2) {
-            f32x16 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(F.l2[t2][4]), biasB, zero, 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(F.l2[t2][c]), relu_chunk(d1[c >> 1], 8 * (c & 1)),
-                                                              acc, 0, 0, 0);
-            d2[t2] = acc;
-        }
+        const f32x16 d10 = mfma(frag4(F, kFragL1 + 0), as_bf16(B[e]), zero);
+        const f32x16 d11 = mfma(frag4(F, kFragL1 + 4), as_bf16(B[e]), zero);
         f32x16 d3 = zero;
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            d3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(F.l3[c]), relu_chunk(d2[c >> 1], 8 * (c & 1)), d3, 0, 0,
-                                                         0);
+        for (int t2 = 0; t2 < 2; ++t2) {
+            // keep each stage's fragment loads next to their MFMAs (hoisting
+            // all of them would hold 64 more registers for the whole chain)
+            __builtin_amdgcn_sched_barrier(0);
+            const int l2 = kFragL2 + 20 * t2;
+            f32x16 acc = mfma(frag4(F, l2 + 16), biasB, zero);  // b2 via the constant chunk
+            acc = mfma(frag4(F, l2 + 0), relu_chunk(d10, 0), acc);
+            acc = mfma(frag4(F, l2 + 4), relu_chunk(d10, 8), acc);
+            acc = mfma(frag4(F, l2 + 8), relu_chunk(d11, 0), acc);
+            acc = mfma(frag4(F, l2 + 12), relu_chunk(d11, 8), acc);
+            __builtin_amdgcn_sched_barrier(0);
+            // layer-3 chunks 2 t2 and 2 t2 + 1 come from this output tile
+            d3 = mfma(frag4(F, kFragL3 + 8 * t2 + 0), relu_chunk(acc, 0), d3);
+            d3 = mfma(frag4(F, kFragL3 + 8 * t2 + 4), relu_chunk(acc, 8), d3);
+        }
+        __builtin_amdgcn_sched_barrier(0);
         // rows 0 and 1 (the two outputs) sit in registers 0 and 1 of lanes 0..31
         out[e][0] = d3[0];
         out[e][1] = d3[1];
     }
     const float q0 = __shfl_xor(out[1][0], 32, 64);
     const float q1 = __shfl_xor(out[1][1], 32, 64);
-    y0 = (upper ? q0 : out[0][0]) + F.b3[0];
-    y1 = (upper ? q1 : out[0][1]) + F.b3[1];
+    float2 y;
+    y.x = (upper ? q0 : out[0][0]) + __uint_as_float(F[kFragB3 * 64 + lane]);
+    y.y = (upper ? q1 : out[0][1]) + __uint_as_float(F[(kFragB3 + 1) * 64 + lane]);
+    return y;
 }
 
-// Middle state from a guess of h*: the same case analysis as the exact
-// middle_state; for the Newton cases one evaluation at the guess (moved into
-// the bracket the exact iteration starts from), the corrected depth is kept
-// if the step is at most kAcceptStep of the depth, else the exact iteration
-// runs.  *accepted tells which.
-__device__ __forceinline__ dev::Middle middle_from_guess(float hL, float hR, float uL, float uR, float cL, float cR,
-                                                        float hGuess, int* accepted) {
+__device__ __forceinline__ void infer(const unsigned* __restrict__ F, const float* x, float& y0, float& y1) {
+    const float2 y = infer_wave(F, x[0], x[1], x[2], x[3], x[4], x[5]);
+    y0 = y.x;
+    y1 = y.y;
+}
+
+// Case analysis of the exact middle_state (dry, two rarefactions, Newton).
+struct Bracket {
+    float hMin, hMax, du, cMax, invMin, invMax;
+    bool newton, shocks;
+};
+
+__device__ __forceinline__ Bracket classify(float hL, float hR, float uL, float uR, float cL, float cR) {
     using namespace dev;
-    const float hMin = smin(hL, hR);
-    const float hMax = smax(hL, hR);
-    const float du = uR - uL;
-    *accepted = 0;
-    if (hMin < kDryTol) return middle_state(hL, hR, uL, uR, cL, cR);
+    Bracket k;
+    k.hMin = smin(hL, hR);
+    k.hMax = smax(hL, hR);
+    k.du = uR - uL;
     const float cMin = (hR < hL) ? cR : cL;
-    const float cMax = (hL < hR) ? cR : cL;
-    const float invMin = 1.f / hMin;
-    const float invMax = 1.f / hMax;
+    k.cMax = (hL < hR) ? cR : cL;
+    k.invMin = 1.f / k.hMin;
+    k.invMax = 1.f / k.hMax;
+    const float fMin = k.du + 2.f * (cMin - k.cMax);
+    const float fMax = k.du + (k.hMax - k.hMin) * sqrtf(0.5f * kGravity * (k.invMax + k.invMin));
+    k.newton = !(k.hMin < kDryTol) && !(fMin > 0.f);
+    k.shocks = fMax <= 0.f;
+    return k;
+}
+
+struct Eval {
+    float h, gMin, s2;
+    bool ok;
+};
+
+// One Newton evaluation of the exact iteration at a guess moved into the
+// bracket the iteration starts from (two shocks: h >= hMax; shock and
+// rarefaction: hMin <= h <= hMax).  The corrected depth is accepted when the
+// step is at most kAcceptStep of the depth: Newton's error after such a step
+// is of the order of the step squared.
+__device__ __forceinline__ Eval eval_guess(const Bracket& k, float hGuess) {
     const float halfG = 0.5f * kGravity;
     const float quarterG = 0.25f * kGravity;
-    const float fMin = du + 2.f * (cMin - cMax);
-    if (fMin > 0.f) return middle_state(hL, hR, uL, uR, cL, cR);
-    const float fMax = du + (hMax - hMin) * sqrtf(halfG * (invMax + invMin));
-    const bool shocks = fMax <= 0.f;
-    float h = smax(hGuess, shocks ? hMax : hMin);
-    if (!shocks) h = smin(h, hMax);
+    const bool shocks = k.shocks;
+    float h = dev::smax(hGuess, shocks ? k.hMax : k.hMin);
+    if (!shocks) h = dev::smin(h, k.hMax);
     const float invH = 1.f / h;
-    const float gMin = sqrtf(halfG * (invH + invMin));
-    const float s2 = sqrtf(shocks ? halfG * (invH + invMax) : kGravity * h);
-    const float t2 = shocks ? (h - hMax) * s2 : 2.f * (s2 - cMax);
-    const float phi = (du + (h - hMin) * gMin) + t2;
+    const float gMin = sqrtf(halfG * (invH + k.invMin));
+    const float s2 = sqrtf(shocks ? halfG * (invH + k.invMax) : kGravity * h);
+    const float t2 = shocks ? (h - k.hMax) * s2 : 2.f * (s2 - k.cMax);
+    const float phi = (k.du + (h - k.hMin) * gMin) + t2;
     const float q = shocks ? s2 : 1.f;
-    const float num = (h - hMin) * q + (shocks ? (h - hMax) * gMin : 0.f);
+    const float num = (h - k.hMin) * q + (shocks ? (h - k.hMax) * gMin : 0.f);
     const float den = gMin * q;
     const float first = gMin + (shocks ? s2 : s2 * invH);
     const float dphi = first - quarterG * invH * invH * (num / den);
     const float step = phi / dphi;
-    const float hNew = h - step;
-    if (!(fabsf(step) <= kAcceptStep * h) || !(hNew > 0.f)) return middle_state(hL, hR, uL, uR, cL, cR);
-    *accepted = 1;
-    h = hNew;
-    Middle m;
+    Eval e;
+    e.h = h - step;
+    e.gMin = gMin;
+    e.s2 = s2;
+    e.ok = (fabsf(step) <= kAcceptStep * h) && (e.h > 0.f);
+    return e;
+}
+
+// Speeds of the middle state from an accepted evaluation, as after the exact
+// iteration (curves evaluated at the point the last step was taken from).
+__device__ __forceinline__ dev::Middle finish(const Bracket& k, const Eval& e, float hL, float hR, float uL, float uR,
+                                              float cL, float cR) {
+    dev::Middle m;
+    const float h = e.h;
     m.hm = h;
     const float cm = sqrtf(kGravity * h);
-    if (shocks) {
+    if (k.shocks) {
         const bool leftIsMin = !(hR < hL);
-        const float gL = leftIsMin ? gMin : s2;
-        const float gR = leftIsMin ? s2 : gMin;
+        const float gL = leftIsMin ? e.gMin : e.s2;
+        const float gR = leftIsMin ? e.s2 : e.gMin;
         const float u1m = uL - (h - hL) * gL;
         const float u2m = uR + (h - hR) * gR;
         m.s1m = u1m - cm;
@@ -247,27 +278,46 @@ __device__ __forceinline__ dev::Middle middle_from_guess(float hL, float hR, flo
     return m;
 }
 
-// The network's guess for the edge of every lane (whole wave active); lanes
-// whose edge has a dry side get a dummy wet state.
-__device__ __forceinline__ float guess_depth(const MlpFrags& F, float hL, float hR, float uL, float uR, float bL,
-                                             float bR) {
-    const bool wet = dev::smin(hL, hR) >= kDryTol;
+// Learned middle state for the edge of every lane (whole wave active).
+// Newton edges first try the linearised guess; if any of them in the wave
+// fails the residual check, the network runs for the wave and the failed
+// edges try its guess; what still fails runs the exact iteration.
+// *how: 0 exact middle_state, 1 linearised guess accepted, 2 network guess.
+__device__ __forceinline__ dev::Middle learned_middle(const unsigned* __restrict__ F, float hL, float hR, float uL,
+                                                      float uR, float cL, float cR, float bL, float bR, int* how) {
+    // nothing moves across the learned part: two edge solves interleaved by
+    // the scheduler would each hold their network state at once
+    __builtin_amdgcn_sched_barrier(0);
+    const Bracket k = classify(hL, hR, uL, uR, cL, cR);
     float x[kIn], hs, cs;
-    features(wet ? hL : 1.f, wet ? hR : 1.f, wet ? uL : 0.f, wet ? uR : 0.f, bL, bR, x, hs, cs);
-    float y0, y1;
-    infer(F, x, y0, y1);
+    features(k.newton ? hL : 1.f, k.newton ? hR : 1.f, k.newton ? uL : 0.f, k.newton ? uR : 0.f, bL, bR, x, hs, cs);
     float l0, l1;
     linear_part(x, l0, l1);
-    return (l0 + y0) * hs;
+    Eval ev = eval_guess(k, l0 * hs);
+    int used = (k.newton && ev.ok) ? 1 : 0;
+    if (__any(k.newton && !ev.ok)) {
+        float y0, y1;
+        infer(F, x, y0, y1);
+        if (k.newton && !ev.ok) {
+            const Eval e2 = eval_guess(k, (l0 + y0) * hs);
+            if (e2.ok) {
+                ev = e2;
+                used = 2;
+            }
+        }
+    }
+    *how = used;
+    __builtin_amdgcn_sched_barrier(0);
+    if (used) return finish(k, ev, hL, hR, uL, uR, cL, cR);
+    return dev::middle_state(hL, hR, uL, uR, cL, cR);
 }
 
 struct LearnedMiddle {
-    const MlpFrags* F;
+    const unsigned* F;
     __device__ __forceinline__ dev::Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR,
                                                       float bL, float bR) const {
-        const float g = guess_depth(*F, hL, hR, uL, uR, bL, bR);
-        int accepted;
-        return middle_from_guess(hL, hR, uL, uR, cL, cR, g, &accepted);
+        int how;
+        return learned_middle(F, hL, hR, uL, uR, cL, cR, bL, bR, &how);
     }
 };
 
@@ -276,10 +326,10 @@ struct LearnedMiddle {
 namespace dev {
 template <>
 struct SolverCtx<SOLVER_AUGRIE_MLP> {
-    mlp::MlpFrags f;
-    __device__ __forceinline__ explicit SolverCtx(const unsigned* frags) { f.load(frags); }
+    const unsigned* F;
+    __device__ __forceinline__ explicit SolverCtx(const unsigned* frags) : F(frags) {}
     __device__ __forceinline__ EdgeOut solve(const Side& L, const Side& R) const {
-        return augrie_with(L, R, mlp::LearnedMiddle{&f});
+        return augrie_with(L, R, mlp::LearnedMiddle{F});
     }
 };
 }  // namespace dev

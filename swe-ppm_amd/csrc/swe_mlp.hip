@@ -43,7 +43,7 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
 }
 
 // ---- logging: every Newton edge of a block state, kept with probability `keep`
-__global__ __launch_bounds__(256) void mlp_log_kernel(StateView v, float keep, unsigned long long seed,
+__global__ __launch_bounds__(256) void mlp_log_kernel(StateView v, float keep, unsigned long long seed, int hardOnly,
                                                       float* samples, long long cap, unsigned long long* count) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long nV = (long long)(v.nx + 1) * v.ny;
@@ -71,13 +71,18 @@ __global__ __launch_bounds__(256) void mlp_log_kernel(StateView v, float keep, u
     const float cMin = (p.hR < p.hL) ? p.cR : p.cL;
     const float cMax = (p.hL < p.hR) ? p.cR : p.cL;
     if ((p.uR - p.uL) + 2.f * (cMin - cMax) > 0.f) return;  // two rarefactions: closed form, no iteration
-    const dev::Middle m = dev::middle_state(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR);
     float x[kIn], hs, cs;
     features(p.hL, p.hR, p.uL, p.uR, p.bL, p.bR, x, hs, cs);
-    const unsigned long long at = atomicAdd(count, 1ULL);
-    if ((long long)at >= cap) return;
     float l0, l1;
     linear_part(x, l0, l1);
+    if (hardOnly) {
+        // only edges whose linearised guess fails the residual check
+        const Bracket k = classify(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR);
+        if (eval_guess(k, l0 * hs).ok) return;
+    }
+    const dev::Middle m = dev::middle_state(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR);
+    const unsigned long long at = atomicAdd(count, 1ULL);
+    if ((long long)at >= cap) return;
     float* o = samples + (size_t)at * kSampleFloats;
     for (int i = 0; i < kIn; ++i) o[i] = x[i];
     o[kIn] = m.hm / hs - l0;
@@ -241,14 +246,12 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const unsigned* __restri
                                                         float* __restrict__ out) {
     const long long base = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
     if (base >= n) return;  // uniform per wave
-    MlpFrags F;
-    F.load(frags);
     const long long s = base + (threadIdx.x & 63);
     const long long sc = s < n ? s : n - 1;
     float x[kIn];
     for (int k = 0; k < kIn; ++k) x[k] = feats[sc * kIn + k];
     float y0, y1;
-    infer(F, x, y0, y1);
+    infer(frags, x, y0, y1);
     if (s < n) {
         out[2 * s] = y0;
         out[2 * s + 1] = y1;
@@ -269,16 +272,13 @@ __global__ __launch_bounds__(256) void mlp_middle_kernel(const unsigned* __restr
                                                          const float* in, float* out, int* accepted) {
     const long long base = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
     if (base >= n) return;
-    MlpFrags F;
-    F.load(frags);
     const long long s = base + (threadIdx.x & 63);
     const long long sc = s < n ? s : n - 1;
     const float hL = in[sc], hR = in[n + sc], uL = in[2 * n + sc], uR = in[3 * n + sc];
     const float bL = in[4 * n + sc], bR = in[5 * n + sc];
     const float cL = sqrtf(kGravity * hL), cR = sqrtf(kGravity * hR);
-    const float g = guess_depth(F, hL, hR, uL, uR, bL, bR);
     int acc;
-    const dev::Middle m = middle_from_guess(hL, hR, uL, uR, cL, cR, g, &acc);
+    const dev::Middle m = learned_middle(frags, hL, hR, uL, uR, cL, cR, bL, bR, &acc);
     if (s < n) {
         out[s] = m.hm;
         out[n + s] = m.s1m;
@@ -534,7 +534,8 @@ int swe_hip_mlp_set_params(swe_hip_mlp* m, const float* params) {
     return repack(m);
 }
 
-int swe_hip_mlp_log_block(swe_hip_mlp* m, swe_hip_block* blk, float keep, uint64_t seed, int64_t* added) {
+int swe_hip_mlp_log_block(swe_hip_mlp* m, swe_hip_block* blk, float keep, uint64_t seed, int hard_only,
+                          int64_t* added) {
     if (!m || !blk || !(keep >= 0.f) || keep > 1.f) {
         set_error("swe_hip_mlp_log_block: invalid argument");
         return SWE_HIP_EINVAL;
@@ -547,7 +548,7 @@ int swe_hip_mlp_log_block(swe_hip_mlp* m, swe_hip_block* blk, float keep, uint64
     HIP_TRY(hipStreamSynchronize(bs));  // the block's queued steps have written the state
     const long long edges = (long long)(v.nx + 1) * v.ny + (long long)v.nx * (v.ny + 1);
     hipLaunchKernelGGL(mlp_log_kernel, dim3((unsigned)((edges + 255) / 256)), dim3(256), 0, m->stream, v, keep,
-                       (unsigned long long)seed, m->samples, m->cap, m->dCount);
+                       (unsigned long long)seed, hard_only ? 1 : 0, m->samples, m->cap, m->dCount);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(m->stream));
     const long long after = count_of(m);

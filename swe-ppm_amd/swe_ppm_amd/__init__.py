@@ -77,7 +77,7 @@ def _load() -> C.CDLL:
         "swe_hip_mlp_destroy": (C.c_int, [vp]),
         "swe_hip_mlp_get_params": (C.c_int, [vp, fp]),
         "swe_hip_mlp_set_params": (C.c_int, [vp, fp]),
-        "swe_hip_mlp_log_block": (C.c_int, [vp, vp, C.c_float, C.c_uint64, C.POINTER(C.c_int64)]),
+        "swe_hip_mlp_log_block": (C.c_int, [vp, vp, C.c_float, C.c_uint64, C.c_int, C.POINTER(C.c_int64)]),
         "swe_hip_mlp_add_samples": (C.c_int, [vp, C.c_int64, fp]),
         "swe_hip_mlp_read_samples": (C.c_int, [vp, C.c_int64, C.c_int64, fp]),
         "swe_hip_mlp_sample_count": (C.c_int, [vp, C.POINTER(C.c_int64)]),
@@ -362,9 +362,12 @@ class MiddleStateMLP:
             raise ValueError(f"expected {MLP_PARAMS} parameters")
         _check(lib.swe_hip_mlp_set_params(self._h, _fptr(p)), "mlp_set_params")
 
-    def log_block(self, blk: "SweBlock", keep: float = 1.0, seed: int = 0) -> int:
+    def log_block(self, blk: "SweBlock", keep: float = 1.0, seed: int = 0, hard_only: bool = False) -> int:
+        """Append the block's Newton edges (hard_only: only those the
+        linearised guess does not settle); returns the number appended."""
         n = C.c_int64()
-        _check(lib.swe_hip_mlp_log_block(self._h, blk._h, float(keep), int(seed), C.byref(n)), "mlp_log_block")
+        _check(lib.swe_hip_mlp_log_block(self._h, blk._h, float(keep), int(seed), int(bool(hard_only)), C.byref(n)),
+               "mlp_log_block")
         return n.value
 
     def add_samples(self, samples) -> None:

@@ -6,9 +6,11 @@ logs while it steps the plain workloads:
   * the 2048^2 dam break over 16 bumps with 3 dry islands (AugRie, 500 steps),
   * the 28-tile still-water tiling of the LTS workload as one 2048^2 block
     (AugRie, 300 steps: the depth steps between tiles radiate waves),
-every `every`-th step, each Newton edge kept with a fixed probability so that
-about `target` samples are collected.  Training runs the library's FP32
-forward / backward / Adam kernels on the GPU.
+every `every`-th step.  The solver settles most Newton edges with the
+linearised middle state; the network is for the rest, so only edges whose
+linearised guess fails the residual check are logged (hard_only), each kept
+with a fixed probability so that about `target` samples are collected.
+Training runs the library's FP32 forward / backward / Adam kernels on the GPU.
 
   python -m swe_ppm_amd.surrogate --out swe-ppm_amd/swe_ppm_amd/data/middle_state_mlp.npy
 """
@@ -26,7 +28,7 @@ from . import AUGRIE, FWAVE, MiddleStateMLP, SweBlock, scenarios
 DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "data" / "middle_state_mlp.npy"
 
 
-def _log_run(mlp, state, solver, steps, every, keep, seed):
+def _log_run(mlp, state, solver, steps, every, keep, seed, hard_only=True):
     h, hu, hv, b, dx, dy, bnd = state
     nx, ny = h.shape[0] - 2, h.shape[1] - 2
     added = 0
@@ -34,26 +36,21 @@ def _log_run(mlp, state, solver, steps, every, keep, seed):
         blk.set_boundaries(bnd)
         blk.upload(h, hu, hv, b)
         for k in range(0, steps, every):
-            added += mlp.log_block(blk, keep=keep, seed=seed + k)
+            added += mlp.log_block(blk, keep=keep, seed=seed + k, hard_only=hard_only)
             blk.run(every, want_dts=False)
     return added
 
 
-def log_workloads(mlp, target: int = 50_000_000, seed: int = 1, scale: int = 1):
-    """Log about `target` samples from the three workloads (sizes divided by
-    `scale` for quick runs).  Returns per-workload sample counts."""
+def log_workloads(mlp, keep: float = 1.0, seed: int = 1, scale: int = 1, every: int = 2):
+    """Log the hard Newton edges of the three workloads (sizes divided by
+    `scale` for quick runs) every `every` steps.  Returns per-workload
+    sample counts."""
     n2 = 2048 // scale
-    # edges needing the iteration per logged step, measured shares of all edges
-    plan = [("dam_break_256", scenarios.dam_break_c1(256 // scale if scale > 1 else 256), FWAVE, 200, 10),
-            ("islands_2048", scenarios.dam_break_islands(n=n2, seed=2048), AUGRIE, 500, 10),
-            ("lts_tiles_2048", scenarios.lts_tiling_as_block(seed=28, n=n2), AUGRIE, 300, 10)]
-    shares = [0.1, 0.6, 0.3]
+    plan = [("dam_break_256", scenarios.dam_break_c1(256 // scale if scale > 1 else 256), FWAVE, 200),
+            ("islands_2048", scenarios.dam_break_islands(n=n2, seed=2048), AUGRIE, 500),
+            ("lts_tiles_2048", scenarios.lts_tiling_as_block(seed=28, n=n2), AUGRIE, 300)]
     counts = {}
-    for (name, state, solver, steps, every), share in zip(plan, shares):
-        nx, ny = state[0].shape[0] - 2, state[0].shape[1] - 2
-        edges_per_log = 2.0 * nx * ny
-        logs = steps // every
-        keep = min(1.0, share * target / (0.9 * edges_per_log * logs))
+    for name, state, solver, steps in plan:
         counts[name] = _log_run(mlp, state, solver, steps, every, keep, seed)
     return counts
 
@@ -73,7 +70,8 @@ def load_default(mlp=None, path: Path = DEFAULT_WEIGHTS):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
-    ap.add_argument("--target", type=int, default=50_000_000)
+    ap.add_argument("--capacity", type=int, default=60_000_000)
+    ap.add_argument("--every", type=int, default=2)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -81,8 +79,8 @@ def main() -> int:
     ap.add_argument("--report", type=Path, default=None)
     args = ap.parse_args()
     t0 = time.perf_counter()
-    mlp = MiddleStateMLP(capacity=int(args.target * 1.2), seed=11)
-    counts = log_workloads(mlp, target=args.target, scale=args.scale)
+    mlp = MiddleStateMLP(capacity=args.capacity, seed=11)
+    counts = log_workloads(mlp, scale=args.scale, every=args.every)
     t1 = time.perf_counter()
     print(f"logged {mlp.count} samples {counts} in {t1 - t0:.1f} s", flush=True)
     loss = train(mlp, steps=args.steps, batch=args.batch, lr=args.lr)

@@ -96,9 +96,9 @@ def _trained_network(swe, seed=0):
         with swe.SweBlock(h.shape[0] - 2, h.shape[1] - 2, dx, dy, solver=solver) as blk:
             blk.set_boundaries(bnd)
             blk.upload(h, hu, hv, b)
-            for k in range(12):
-                mlp.log_block(blk, keep=0.5, seed=100 * seed + k)
-                blk.run(8, want_dts=False)
+            for k in range(24):
+                mlp.log_block(blk, keep=1.0, seed=100 * seed + k, hard_only=True)
+                blk.run(4, want_dts=False)
     loss = mlp.train(1500, batch=4096, lr=3e-3, seed=seed)
     return mlp, loss
 
@@ -126,13 +126,21 @@ def test_training_reduces_loss_and_trained_network_passes_the_residual_check(swe
         hm_t, s1_t, s2_t, acc_t = trained.middle(*edges)
         hm_u, _, _, acc_u = untrained.middle(*edges)
     exact = oracle.middle_state(*[a.astype(np.float64) for a in edges[:4]])
-    rate_t, rate_u = float(np.mean(acc_t)), float(np.mean(acc_u))
-    # the gain: with the trained network most Newton edges need one evaluation
-    # instead of the exact iteration (mean iterations of the exact path stated beside)
-    print(f"accepted: trained {rate_t:.4f}, untrained {rate_u:.4f}; exact Newton iterations "
-          f"mean {np.mean(iters):.2f}")
+    # acc: 1 linearised guess kept, 2 network guess kept, 0 exact iteration
+    lin = acc_t == 1
+    assert np.array_equal(lin, acc_u == 1)  # the linearised guess does not depend on the network
+    hard = ~lin
+    rescue_t = float(np.mean(acc_t[hard] == 2))
+    rescue_u = float(np.mean(acc_u[hard] == 2))
+    rate_t = float(np.mean(acc_t > 0))
+    # the gain: of the Newton edges the linearised guess does not settle, the
+    # trained network settles most with one evaluation; the exact path needs
+    # its full iteration for them (mean iterations stated beside)
+    print(f"linearised guess kept {np.mean(lin):.4f}; of the rest the network keeps: trained {rescue_t:.4f}, "
+          f"untrained {rescue_u:.4f}; overall without iteration {rate_t:.4f}; exact Newton iterations mean "
+          f"{np.mean(iters):.2f} (hard edges {np.mean(iters[hard]):.2f})")
     assert rate_t >= 0.9
-    assert rate_t >= rate_u + 0.2
+    assert rescue_t >= rescue_u + 0.3
     # accepted middle depths are as close to the FP64 root as the exact FP32 iteration's
     rel = np.abs(hm_t - exact[0]) / exact[0]
     assert np.max(rel) <= 2e-5
