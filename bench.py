@@ -63,6 +63,16 @@ def timed_run(swe, blk, nsteps, per_kernel):
     return tt.value, total.value, list(kms), list(kl)
 
 
+def timed_run_from(swe, blk, nsteps, t0):
+    """swe_hip_run_timed continuing from simulated time t0."""
+    tt = C.c_float(t0)
+    total = C.c_float(0.0)
+    rc = swe.lib.swe_hip_run_timed(blk._h, nsteps, C.byref(tt), C.byref(total), None, None)
+    if rc != 0:
+        raise RuntimeError(swe.lib.swe_hip_last_error().decode())
+    return tt.value, total.value, None, None
+
+
 def cpu_baseline(h, hu, hv, b, dx, dy, bnd, solver, budget_s: float):
     """Oracle steps on all host threads until the time budget is used."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -107,12 +117,13 @@ def batched_lts(swe, scenarios, autotune: bool, seed: int = 28, n: int = 2048, T
         bt.upload(*fields)
         _, its, ms = bt.run(nmacro)
         upd, _ = bt.stats()
-        xs, groups, chunk, graph = bt.get_tuning()
+        xs, groups, chunk, graph, fused = bt.get_tuning()
         h = bt.download(packed=True)[0]
     if not np.all(np.isfinite(h)):
         raise RuntimeError("non-finite LTS state")
     cells = (bt.nx.astype(np.int64) * bt.ny).astype(np.int64)
     updates = int(np.sum(cells * upd))
+    plain = plain_lts_domain(swe, scenarios, duration, seed=seed, n=n)
     return {
         "workload": f"{len(blocks)} blocks tiling {n}x{n} (sizes {int(bt.nx.min())}..{int(bt.nx.max())} x "
                     f"{int(bt.ny.min())}..{int(bt.ny.max())}), still water U(2,40) m per block (seed {seed}), AugRie "
@@ -120,10 +131,46 @@ def batched_lts(swe, scenarios, autotune: bool, seed: int = 28, n: int = 2048, T
         "value": round(updates / (ms * 1e-3) / 1e6, 3), "unit": "MCell-updates/s",
         "elapsed_ms": round(ms, 3), "cell_updates": updates, "driver_iterations": int(np.sum(its)),
         "sim_s_per_wall_s": round(duration / (ms * 1e-3), 3),
+        "plain_global_dt_same_domain": plain,
+        "sim_speed_vs_plain": round((duration / (ms * 1e-3)) / plain["sim_s_per_wall_s"], 4),
         "hbm_frac_algorithmic": round(BYTES_PER_CELL * updates / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-        "tuning": {"xs": xs, "groups": groups, "chunk": chunk, "graph": graph,
+        "tuning": {"xs": xs, "groups": groups, "chunk": chunk, "graph": graph, "fused": fused,
                    "autotune_ms_per_iteration": None if cand is None else [round(float(c), 4) for c in cand]},
     }
+
+
+def plain_lts_domain(swe, scenarios, duration: float, seed: int = 28, n: int = 2048):
+    """The LTS workload's domain as one plain block with the global CFL dt:
+    simulated seconds per wall second, for comparison with local time stepping."""
+    h, hu, hv, b, dx, dy, bnd = scenarios.lts_tiling_as_block(seed=seed, n=n)
+    with swe.SweBlock(n, n, dx, dy, solver=swe.AUGRIE) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        blk.run(10, want_dts=False)
+        blk.upload(h, hu, hv, b)
+        # advance in chunks of 200 steps until the simulated time is reached
+        t, total_ms, steps = 0.0, 0.0, 0
+        while t < duration:
+            t, ms, _, _ = timed_run_from(swe, blk, 200, t)
+            total_ms += ms
+            steps += 200
+    return {"sim_s_per_wall_s": round(t / (total_ms * 1e-3), 3), "steps": steps, "simulated_s": round(t, 3),
+            "value": round(n * n * steps / (total_ms * 1e-3) / 1e6, 3), "unit": "MCell-updates/s",
+            "cell_updates": n * n * steps}
+
+
+def plain_ensemble(swe, inst, steps: int):
+    """The ensemble's instances one after another through the plain block path."""
+    total_ms, updates = 0.0, 0
+    for i in inst:
+        with swe.SweBlock(i["nx"], i["ny"], i["dx"], i["dy"], solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(i["bnd"])
+            blk.upload(i["h"], i["hu"], i["hv"], i["b"])
+            _, ms, _, _ = timed_run(swe, blk, steps, per_kernel=False)
+            total_ms += ms
+            updates += i["nx"] * i["ny"] * steps
+    return {"value": round(updates / (total_ms * 1e-3) / 1e6, 3), "unit": "MCell-updates/s", "steps": steps,
+            "elapsed_ms": round(total_ms, 3)}
 
 
 def batched_ensemble(swe, scenarios, autotune: bool, count: int = 512, seed: int = 512, steps: int = 300):
@@ -138,19 +185,22 @@ def batched_ensemble(swe, scenarios, autotune: bool, count: int = 512, seed: int
         bt.upload(*fields)
         _, _, ms = bt.run(steps)
         upd, t = bt.stats()
-        xs, groups, chunk, graph = bt.get_tuning()
+        xs, groups, chunk, graph, fused = bt.get_tuning()
         h = bt.download(packed=True)[0]
     if not np.all(np.isfinite(h)):
         raise RuntimeError("non-finite ensemble state")
     updates = int(bt.cells) * steps
+    plain = plain_ensemble(swe, inst, steps=min(steps, 50))
     return {
         "workload": f"{count} independent dam-break instances, NX,NY ~ U{{64..384}} ({bt.cells} cells, seed {seed}), "
                     f"AugRie FP32, per-instance CFL dt, {steps} steps each",
         "value": round(updates / (ms * 1e-3) / 1e6, 3), "unit": "MCell-updates/s",
         "elapsed_ms": round(ms, 3), "cell_updates": updates,
         "instance_time_range_s": [round(float(t.min()), 4), round(float(t.max()), 4)],
+        "plain_one_instance_after_another": plain,
+        "vs_plain_same_instances": round((updates / (ms * 1e-3) / 1e6) / plain["value"], 4),
         "hbm_frac_algorithmic": round(BYTES_PER_CELL * updates / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-        "tuning": {"xs": xs, "groups": groups, "chunk": chunk, "graph": graph,
+        "tuning": {"xs": xs, "groups": groups, "chunk": chunk, "graph": graph, "fused": fused,
                    "autotune_ms_per_iteration": None if cand is None else [round(float(c), 4) for c in cand]},
     }
 

@@ -79,10 +79,11 @@ int swe_hip_batch_stats(swe_hip_batch* bt, int64_t* updates, float* t);
 /* Tuning: xs = strip columns per work item (4, 8 or 16), groups = workgroups
  * of the persistent sweep / apply grids (1..65535), chunk = driver iterations
  * per captured hipGraph (1..256), graph = 1 replays captured graphs, 0
- * launches the kernels one by one; -1 keeps a value.  Results do not depend
- * on the tuning. */
-int swe_hip_batch_set_tuning(swe_hip_batch* bt, int xs, int groups, int chunk, int graph);
-int swe_hip_batch_get_tuning(swe_hip_batch* bt, int* xs, int* groups, int* chunk, int* graph);
+ * launches the kernels one by one, fused = 1 applies each block's update on
+ * its next load (one sweep per iteration, the default), 0 runs a separate
+ * apply pass; -1 keeps a value.  Results do not depend on the tuning. */
+int swe_hip_batch_set_tuning(swe_hip_batch* bt, int xs, int groups, int chunk, int graph, int fused);
+int swe_hip_batch_get_tuning(swe_hip_batch* bt, int* xs, int* groups, int* chunk, int* graph, int* fused);
 
 /* Autotuner: times every (xs, groups) candidate on `iterations` driver
  * iterations of the current state, restores the state afterwards and keeps

@@ -59,21 +59,26 @@ struct BatchBlockDesc {
 
 struct BatchBlockState {
     unsigned int smax;  // CFL maximum of this iteration (float bits)
-    float maxTimestep;  // dt of this iteration's update
+    float maxTimestep;  // dt of the block's last update (pending or applied)
     int stepSize, stepSizeCounter, timestepCounter;
     int gl[4];
     float borderTs[4];
     int notified;
     int active;  // in sync this iteration: computes and updates
+    int cur;     // state buffer (0/1) holding the block
+    int pend;    // fused strategy: the last update (sums set sidx, dt maxTimestep) is not yet applied
+    int sidx;    // sums set of the pending update
     int pad;
     long long updates;  // updates performed
     float t;            // simulated time of an independent block
     int pad2;
 };
 
+// State double-buffered per block (the fused sweep reads one buffer and
+// writes the other), bathymetry single, two sets of net-update sums.
 struct BatchArena {
-    float *h, *hu, *hv, *b;
-    float *sx, *sy, *su, *sv;
+    float *h[2], *hu[2], *hv[2], *b;
+    float* s[2][4];  // sums sets: sx, sy, su, sv
     float* lines;
 };
 
@@ -94,13 +99,15 @@ struct BatchGlobals {
     int logCap;
     long long iterations;  // iterations that did work
     const unsigned* mlp;   // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
+    int fused;             // 1: fused sweep (update applied on the next load), 0: sweep + apply
+    int pad;
 };
 
-__device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d) {
+__device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d, int buf) {
     StateView v;
-    v.h = A.h + d.off;
-    v.hu = A.hu + d.off;
-    v.hv = A.hv + d.off;
+    v.h = A.h[buf] + d.off;
+    v.hu = A.hu[buf] + d.off;
+    v.hv = A.hv[buf] + d.off;
     v.b = A.b + d.off;
     v.nx = d.nx;
     v.ny = d.ny;
@@ -109,8 +116,8 @@ __device__ __forceinline__ StateView batch_view(const BatchArena& A, const Batch
     v.dy = d.dy;
     return v;
 }
-__device__ __forceinline__ SumsView batch_sums(const BatchArena& A, const BatchBlockDesc& d) {
-    return SumsView{A.sx + d.off, A.sy + d.off, A.su + d.off, A.sv + d.off};
+__device__ __forceinline__ SumsView batch_sums(const BatchArena& A, const BatchBlockDesc& d, int set) {
+    return SumsView{A.s[set][0] + d.off, A.s[set][1] + d.off, A.s[set][2] + d.off, A.s[set][3] + d.off};
 }
 
 // ---- LTS scalars of one block (SWE_Block.hh:311-367, 530-562) -------------
@@ -245,8 +252,8 @@ __global__ __launch_bounds__(256) void batch_sweep_kernel(BatchArena A, const Ba
         const int blk = code >> kTileBits;
         const int tile = code & ((1 << kTileBits) - 1);
         const BatchBlockDesc& d = D[blk];
-        const StateView v = batch_view(A, d);
-        const float m = strip_sums<S, XS>(solver, v, batch_sums(A, d), tile % d.nyt, tile / d.nyt, lane);
+        const StateView v = batch_view(A, d, st[blk].cur);
+        const float m = strip_sums<S, XS>(solver, v, batch_sums(A, d, 0), tile % d.nyt, tile / d.nyt, lane);
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     }
 }
@@ -271,8 +278,8 @@ __global__ __launch_bounds__(256) void batch_apply_kernel(BatchArena A, const Ba
         const int blk = code >> kTileBits;
         const int tile = code & ((1 << kTileBits) - 1);
         const BatchBlockDesc& d = D[blk];
-        const StateView v = batch_view(A, d);
-        const SumsView sums = batch_sums(A, d);
+        const StateView v = batch_view(A, d, st[blk].cur);
+        const SumsView sums = batch_sums(A, d, 0);
         const float dt = (mode == BATCH_GLOBAL) ? gdt : st[blk].maxTimestep;
         const float dtdx = dt / v.dx;
         const float dtdy = dt / v.dy;
@@ -293,6 +300,134 @@ __global__ __launch_bounds__(256) void batch_apply_kernel(BatchArena A, const Ba
                 write_boundary_ghosts(v, bi, x, y, nc.h, nc.hu, nc.hv, v.h, v.hu, v.hv);
             }
         }
+    }
+}
+
+// ---- fused strategy: previous update applied on load ----------------------
+struct BQ3 {
+    float h, hu, hv;
+};
+
+// One strip tile of a block with its pending update (sums sIn, dt) applied as
+// the cells are loaded: the updated state goes to buffer `o` (interior cells,
+// OUTFLOW/WALL ghosts derived from the updated interior, other ghosts copied,
+// corners from the diagonal interior cell), the edges are solved on it and the
+// new sums go to sOut.  The same loads, edge order and arithmetic as the
+// single-block fused kernel, so sums and maxima equal the sweep + apply pair.
+template <int S, int XS>
+__device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, const StateView& v, const StateView& o,
+                                            const SumsView& sIn, const SumsView& sOut, bool pend, float dt,
+                                            const int* bnd, int yt, int xs, int lane) {
+    const int nx = v.nx, ny = v.ny;
+    const float dtdx = dt / v.dx;
+    const float dtdy = dt / v.dy;
+    const int y = kStripRows * yt + lane;
+    const bool rowValid = y <= ny + 1;
+    const int yc = rowValid ? y : ny + 1;
+    const bool owned = (lane >= 1) && (lane <= kStripRows) && (y <= ny);
+    const bool rowInterior = (y >= 1) && (y <= ny);
+    const bool hEdgeValid = (lane <= kStripRows) && (y <= ny);
+    const bool rowBottom = (y == 0);
+    const bool rowTop = (y == ny + 1);
+    const bool carryRow = owned || (rowBottom && lane == 0) || (rowTop && lane >= 1);
+    const int x0 = 1 + xs * XS;
+    const int x1 = min(x0 + XS - 1, nx);
+    const int tB = bnd[BND_BOTTOM], tT = bnd[BND_TOP], tL = bnd[BND_LEFT], tR = bnd[BND_RIGHT];
+    auto ow = [](int t) { return t == OUTFLOW || t == WALL; };
+
+    auto load_updated = [&](int x) -> BQ3 {
+        const size_t c = idx(v, x, yc);
+        BQ3 q{v.h[c], v.hu[c], v.hv[c]};
+        if (pend && x >= 1 && x <= nx && rowInterior) {
+            const Cell u = update_cell(q.h, q.hu, q.hv, dtdx, dtdy, sIn.sx[c], sIn.sy[c], sIn.su[c], sIn.sv[c]);
+            q = BQ3{u.h, u.hu, u.hv};
+        }
+        return q;
+    };
+    auto fix_rows = [&](BQ3 q) -> BQ3 {
+        const float dh = __shfl_down(q.h, 1, 64), dhu = __shfl_down(q.hu, 1, 64), dhv = __shfl_down(q.hv, 1, 64);
+        const float uh = __shfl_up(q.h, 1, 64), uhu = __shfl_up(q.hu, 1, 64), uhv = __shfl_up(q.hv, 1, 64);
+        if (rowBottom && ow(tB)) q = BQ3{dh, dhu, (tB == WALL) ? -dhv : dhv};
+        if (rowTop && ow(tT)) q = BQ3{uh, uhu, (tT == WALL) ? -uhv : uhv};
+        return q;
+    };
+    auto fix_column = [&](BQ3 raw, BQ3 in, int type) -> BQ3 {
+        const float dh = __shfl_down(in.h, 1, 64), dhu = __shfl_down(in.hu, 1, 64), dhv = __shfl_down(in.hv, 1, 64);
+        const float uh = __shfl_up(in.h, 1, 64), uhu = __shfl_up(in.hu, 1, 64), uhv = __shfl_up(in.hv, 1, 64);
+        BQ3 q = raw;
+        if (rowInterior && ow(type)) q = BQ3{in.h, (type == WALL) ? -in.hu : in.hu, in.hv};
+        if (rowBottom) q = BQ3{dh, dhu, dhv};
+        if (rowTop) q = BQ3{uh, uhu, uhv};
+        return q;
+    };
+    auto store = [&](int x, const BQ3& q) {
+        const size_t c = idx(o, x, y);
+        o.h[c] = q.h;
+        o.hu[c] = q.hu;
+        o.hv[c] = q.hv;
+    };
+
+    BQ3 cur = fix_rows(load_updated(x0));
+    BQ3 prev = load_updated(x0 - 1);
+    if (x0 == 1) prev = fix_column(prev, cur, tL);
+    if (x0 == 1 && carryRow) store(0, prev);
+    CellQ pq = cell_q(prev.h, prev.hu, prev.hv, v.b[idx(v, x0 - 1, yc)]);
+    CellQ cq = cell_q(cur.h, cur.hu, cur.hv, v.b[idx(v, x0, yc)]);
+    const dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
+    float carryH = e.hR, carryHu = e.huR;
+    float smaxv = rowInterior ? e.speed : 0.f;
+    for (int x = x0; x <= x1; ++x) {
+        BQ3 nxt = fix_rows(load_updated(x + 1));
+        if (x + 1 == nx + 1) nxt = fix_column(nxt, cur, tR);
+        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, v.b[idx(v, x + 1, yc)]);
+        if (carryRow) store(x, cur);
+        if (x + 1 == nx + 1 && carryRow) store(nx + 1, nxt);
+        const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
+        if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
+        const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
+        if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
+        const float aH = __shfl_up(eh.hR, 1, 64);
+        const float aHv = __shfl_up(eh.huR, 1, 64);
+        if (owned) {
+            const size_t c = idx(v, x, y);
+            sOut.sx[c] = carryH + ev.hL;
+            sOut.su[c] = carryHu + ev.huL;
+            sOut.sy[c] = aH + eh.hL;
+            sOut.sv[c] = aHv + eh.huL;
+        }
+        carryH = ev.hR;
+        carryHu = ev.huR;
+        cur = nxt;
+        cq = nq;
+    }
+    return wave_max(smaxv);
+}
+
+template <int S, int XS>
+__global__ __launch_bounds__(256) void batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+                                                          BatchGlobals* G, const int* items) {
+    if (G->skip) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int count = G->count;
+    const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? G->mlp : nullptr);
+    __shared__ int sBase;
+    for (;;) {
+        const int item = next_items(&G->head, &sBase) + wave;
+        if (item - wave >= count) break;  // uniform over the workgroup
+        if (item >= count) continue;
+        const int code = __builtin_amdgcn_readfirstlane(items[item]);
+        const int blk = code >> kTileBits;
+        const int tile = code & ((1 << kTileBits) - 1);
+        const BatchBlockDesc& d = D[blk];
+        const int cur = __builtin_amdgcn_readfirstlane(st[blk].cur);
+        const int sidx = __builtin_amdgcn_readfirstlane(st[blk].sidx);
+        const bool pend = __builtin_amdgcn_readfirstlane(st[blk].pend) != 0;
+        const float dt = st[blk].maxTimestep;
+        const float m = fused_tile<S, XS>(solver, batch_view(A, d, cur), batch_view(A, d, cur ^ 1),
+                                          batch_sums(A, d, sidx), batch_sums(A, d, sidx ^ 1), pend, dt, d.bnd,
+                                          tile % d.nyt, tile / d.nyt, lane);
+        if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     }
 }
 

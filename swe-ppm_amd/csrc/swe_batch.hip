@@ -32,9 +32,9 @@ constexpr int kCtlThreads = 256;
 // (SWE_DimensionalSplittingMPIOverdecomp.cpp:208-235).  One workgroup per block.
 __global__ __launch_bounds__(kCtlThreads) void batch_init_kernel(BatchArena A, const BatchBlockDesc* D) {
     const BatchBlockDesc d = D[blockIdx.x];
-    float* h = A.h + d.off;
-    float* hu = A.hu + d.off;
-    float* hv = A.hv + d.off;
+    float* h = A.h[0] + d.off;
+    float* hu = A.hu[0] + d.off;
+    float* hv = A.hv[0] + d.off;
     float* b = A.b + d.off;
     const int nx = d.nx, ny = d.ny, p = d.pitch;
     for (int side = 0; side < 4; ++side) {
@@ -105,24 +105,39 @@ __global__ __launch_bounds__(kCtlThreads) void batch_ghost_kernel(BatchArena A, 
         int ss, gs, n, nn;
         side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
         side_line(d, side, 1, go, gs, n);        // my ghost line
-        const float* sh = A.h + dn.off + so;
-        const float* shu = A.hu + dn.off + so;
-        const float* shv = A.hv + dn.off + so;
+        // the neighbour's current state: its buffer, with a pending update applied
+        const BatchBlockState& sn = st[nb];
+        const int ncur = sn.cur;
+        const bool npend = sn.pend != 0;
+        const float ndt = sn.maxTimestep;
+        const float ndtdx = ndt / dn.dx, ndtdy = ndt / dn.dy;
+        const float* sh = A.h[ncur] + dn.off + so;
+        const float* shu = A.hu[ncur] + dn.off + so;
+        const float* shv = A.hv[ncur] + dn.off + so;
+        const SumsView ps = batch_sums(A, dn, sn.sidx);
+        auto value = [&](int i) -> Cell {
+            const long long c = so + (long long)i * ss;
+            Cell q{sh[(long long)i * ss], shu[(long long)i * ss], shv[(long long)i * ss]};
+            if (npend) q = update_cell(q.h, q.hu, q.hv, ndtdx, ndtdy, ps.sx[c], ps.sy[c], ps.su[c], ps.sv[c]);
+            return q;
+        };
         if (lts) {
             float* L = A.lines + d.line[side];
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                L[i] = sh[(long long)i * ss];
-                L[n + i] = shu[(long long)i * ss];
-                L[2 * n + i] = shv[(long long)i * ss];
+                const Cell q = value(i);
+                L[i] = q.h;
+                L[n + i] = q.hu;
+                L[2 * n + i] = q.hv;
             }
         } else {
-            float* gh = A.h + d.off + go;
-            float* ghu = A.hu + d.off + go;
-            float* ghv = A.hv + d.off + go;
+            float* gh = A.h[s.cur] + d.off + go;
+            float* ghu = A.hu[s.cur] + d.off + go;
+            float* ghv = A.hv[s.cur] + d.off + go;
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                gh[(long long)i * gs] = sh[(long long)i * ss];
-                ghu[(long long)i * gs] = shu[(long long)i * ss];
-                ghv[(long long)i * gs] = shv[(long long)i * ss];
+                const Cell q = value(i);
+                gh[(long long)i * gs] = q.h;
+                ghu[(long long)i * gs] = q.hu;
+                ghv[(long long)i * gs] = q.hv;
             }
         }
     }
@@ -168,7 +183,7 @@ __global__ __launch_bounds__(kCtlThreads) void batch_check_kernel(BatchArena A, 
             int gs, n;
             side_line(d, side, 1, go, gs, n);
             const float* L = A.lines + d.line[side];
-            float* f[3] = {A.h + d.off + go, A.hu + d.off + go, A.hv + d.off + go};
+            float* f[3] = {A.h[s.cur] + d.off + go, A.hu[s.cur] + d.off + go, A.hv[s.cur] + d.off + go};
             const float w = ratio[side];
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
                 for (int c = 0; c < 3; ++c) {
@@ -259,13 +274,110 @@ __global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlo
         hostStat[2] = G->itersUnit;
     }
     __syncthreads();
-    if (macroEnd) {
-        for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) {
-            BatchBlockState& s = st[blk];
+    const bool fused = G->fused != 0;
+    const float gdt = __uint_as_float(sminBits);
+    for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) {
+        BatchBlockState& s = st[blk];
+        if (s.active) {
+            // every block of the global loop updates with the minimum (swe_mpi_overdecomp.cpp:305)
+            if (mode == BATCH_GLOBAL) s.maxTimestep = gdt;
+            // fused strategy: the update is applied by the block's next load, into the other buffer
+            if (fused) {
+                s.cur ^= 1;
+                s.sidx ^= 1;
+                s.pend = 1;
+            }
+        }
+        if (macroEnd) {
             s.stepSizeCounter = 0;
             s.stepSize = 0;
             s.timestepCounter += 1;
         }
+    }
+}
+
+// ---- end of a fused run: apply the pending updates ---------------------------
+// Pass 1 writes the updated state of every block with a pending update into
+// its other buffer (interior updated, OUTFLOW / WALL ghosts from the updated
+// interior, other ghosts copied, corners from the updated diagonal cell).
+// Pass 2 copies every block whose state then sits in buffer 1 to buffer 0.
+// Pass 3 resets the flags.  Grid: (64, nblocks).
+__device__ __forceinline__ Cell pending_cell(const BatchArena& A, const BatchBlockDesc& d, const BatchBlockState& s,
+                                             int x, int y) {
+    const StateView v = batch_view(A, d, s.cur);
+    const size_t c = idx(v, x, y);
+    Cell q{v.h[c], v.hu[c], v.hv[c]};
+    if (x >= 1 && x <= d.nx && y >= 1 && y <= d.ny) {
+        const SumsView ps = batch_sums(A, d, s.sidx);
+        q = update_cell(q.h, q.hu, q.hv, s.maxTimestep / d.dx, s.maxTimestep / d.dy, ps.sx[c], ps.sy[c], ps.su[c],
+                        ps.sv[c]);
+    }
+    return q;
+}
+
+__global__ __launch_bounds__(256) void batch_normalize_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                              const BatchBlockState* st) {
+    const int blk = blockIdx.y;
+    const BatchBlockState s = st[blk];
+    if (!s.pend) return;
+    const BatchBlockDesc d = D[blk];
+    const int nx = d.nx, ny = d.ny, p = d.pitch;
+    const long long cells = (long long)(nx + 2) * p;
+    const StateView o = batch_view(A, d, s.cur ^ 1);
+    auto ow = [](int t) { return t == OUTFLOW || t == WALL; };
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cells;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(c / p), y = (int)(c % p);
+        const bool gx = x == 0 || x == nx + 1, gy = y == 0 || y == ny + 1;
+        Cell q;
+        if (gx && gy) {
+            q = pending_cell(A, d, s, x == 0 ? 1 : nx, y == 0 ? 1 : ny);  // corner: diagonal interior cell
+        } else if (gx) {
+            const int t = d.bnd[x == 0 ? BND_LEFT : BND_RIGHT];
+            if (ow(t)) {
+                q = pending_cell(A, d, s, x == 0 ? 1 : nx, y);
+                if (t == WALL) q.hu = -q.hu;
+            } else {
+                q = pending_cell(A, d, s, x, y);  // ghost: as stored
+            }
+        } else if (gy) {
+            const int t = d.bnd[y == 0 ? BND_BOTTOM : BND_TOP];
+            if (ow(t)) {
+                q = pending_cell(A, d, s, x, y == 0 ? 1 : ny);
+                if (t == WALL) q.hv = -q.hv;
+            } else {
+                q = pending_cell(A, d, s, x, y);
+            }
+        } else {
+            q = pending_cell(A, d, s, x, y);
+        }
+        o.h[c] = q.h;
+        o.hu[c] = q.hu;
+        o.hv[c] = q.hv;
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_gather_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                           const BatchBlockState* st) {
+    const int blk = blockIdx.y;
+    const BatchBlockState s = st[blk];
+    const int where = s.pend ? (s.cur ^ 1) : s.cur;
+    if (where == 0) return;
+    const BatchBlockDesc d = D[blk];
+    const long long cells = (long long)(d.nx + 2) * d.pitch;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cells;
+         c += (long long)gridDim.x * blockDim.x) {
+        A.h[0][d.off + c] = A.h[1][d.off + c];
+        A.hu[0][d.off + c] = A.hu[1][d.off + c];
+        A.hv[0][d.off + c] = A.hv[1][d.off + c];
+    }
+}
+
+__global__ void batch_flags_kernel(BatchBlockState* st, int nblocks) {
+    for (int blk = threadIdx.x; blk < nblocks; blk += blockDim.x) {
+        st[blk].cur = 0;
+        st[blk].pend = 0;
+        st[blk].sidx = 0;
     }
 }
 
@@ -287,6 +399,22 @@ int launch_apply(int groups, BatchArena A, const BatchBlockDesc* D, const BatchB
     hipLaunchKernelGGL((batch_apply_kernel<XS>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, A, D, st, G,
                        items);
     return 0;
+}
+
+int launch_fused(int solver, int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+                 BatchGlobals* G, const int* items, hipStream_t stream) {
+    switch (solver) {
+        case SOLVER_HLLE:
+            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, A, D, st, G, items, stream);
+        case SOLVER_FWAVE:
+            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, A, D, st, G, items, stream);
+        case SOLVER_AUGRIE:
+            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, A, D, st, G, items, stream);
+        case SOLVER_AUGRIE_MLP:
+            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, A, D, st, G, items, stream);
+        default:
+            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, A, D, st, G, items, stream);
+    }
 }
 
 int launch_sweep(int solver, int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
@@ -330,6 +458,7 @@ struct swe_hip_batch {
     hipGraphExec_t exec = nullptr;
     bool uploaded = false;
     const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
+    int fused = 1;                  // 1: fused sweep (update applied on load), 0: sweep + apply
 };
 
 namespace {
@@ -372,6 +501,15 @@ int enqueue_iteration(swe_hip_batch* k) {
     hipLaunchKernelGGL(batch_check_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG,
                        k->dItems);
     const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
+    if (k->fused) {
+        int rc = launch_fused(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(batch_round_kernel, dim3(1), blk, 0, k->stream, k->dDesc, k->dState, k->dG, k->dDtLog,
+                           k->dIterLog, k->dHostStat);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "batch iteration launch");
+        return 0;
+    }
     int rc = launch_sweep(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(batch_round_kernel, dim3(1), blk, 0, k->stream, k->dDesc, k->dState, k->dG, k->dDtLog,
@@ -501,6 +639,19 @@ int advance(swe_hip_batch* k, int units, long long maxIter, long long* itersOut)
     return 0;
 }
 
+// Apply the pending updates of a fused run and bring every block back to
+// buffer 0 (downloads, snapshots and the next upload see one layout).
+int finish_run(swe_hip_batch* k) {
+    if (!k->fused) return 0;
+    const dim3 grid(64, k->nblocks);
+    hipLaunchKernelGGL(batch_normalize_kernel, grid, dim3(256), 0, k->stream, k->A, k->dDesc, k->dState);
+    hipLaunchKernelGGL(batch_gather_kernel, grid, dim3(256), 0, k->stream, k->A, k->dDesc, k->dState);
+    hipLaunchKernelGGL(batch_flags_kernel, dim3(1), dim3(256), 0, k->stream, k->dState, k->nblocks);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -599,8 +750,10 @@ int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float*
         k->cus = prop.multiProcessorCount;
     const size_t bytes = sizeof(float) * (size_t)off;
     hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
-    float** fields[8] = {&k->A.h, &k->A.hu, &k->A.hv, &k->A.b, &k->A.sx, &k->A.sy, &k->A.su, &k->A.sv};
-    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipMalloc(fields[i], bytes);
+    float** fields[15] = {&k->A.h[0],    &k->A.hu[0],   &k->A.hv[0],   &k->A.h[1],    &k->A.hu[1],
+                          &k->A.hv[1],   &k->A.b,       &k->A.s[0][0], &k->A.s[0][1], &k->A.s[0][2],
+                          &k->A.s[0][3], &k->A.s[1][0], &k->A.s[1][1], &k->A.s[1][2], &k->A.s[1][3]};
+    for (int i = 0; i < 15 && e == hipSuccess; ++i) e = hipMalloc(fields[i], bytes);
     if (e == hipSuccess) e = hipMalloc(&k->A.lines, sizeof(float) * (size_t)std::max<long long>(lines, 1));
     if (e == hipSuccess) e = hipMalloc(&k->dDesc, sizeof(BatchBlockDesc) * nblocks);
     if (e == hipSuccess) e = hipMalloc(&k->dState, sizeof(BatchBlockState) * nblocks);
@@ -609,7 +762,8 @@ int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float*
     if (e == hipSuccess) e = hipHostMalloc((void**)&k->hostStat, 4 * sizeof(int), hipHostMallocMapped);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&k->dHostStat, k->hostStat, 0);
     if (e == hipSuccess) {
-        for (int i = 4; i < 8 && e == hipSuccess; ++i) e = hipMemsetAsync(*fields[i], 0, bytes, k->stream);
+        for (int i = 3; i < 15 && e == hipSuccess; ++i)
+            if (i != 6) e = hipMemsetAsync(*fields[i], 0, bytes, k->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
     if (e != hipSuccess) {
@@ -632,7 +786,9 @@ int swe_hip_batch_destroy(swe_hip_batch* k) {
     if (!k) return SWE_HIP_OK;
     if (k->stream) (void)hipStreamSynchronize(k->stream);
     free_graph(k);
-    float* fields[9] = {k->A.h, k->A.hu, k->A.hv, k->A.b, k->A.sx, k->A.sy, k->A.su, k->A.sv, k->A.lines};
+    float* fields[16] = {k->A.h[0],    k->A.hu[0],   k->A.hv[0],   k->A.h[1],    k->A.hu[1],   k->A.hv[1],
+                         k->A.b,       k->A.s[0][0], k->A.s[0][1], k->A.s[0][2], k->A.s[0][3], k->A.s[1][0],
+                         k->A.s[1][1], k->A.s[1][2], k->A.s[1][3], k->A.lines};
     for (float* f : fields)
         if (f) (void)hipFree(f);
     if (k->dDesc) (void)hipFree(k->dDesc);
@@ -653,9 +809,9 @@ int swe_hip_batch_upload(swe_hip_batch* k, const float* h, const float* hu, cons
         return SWE_HIP_EINVAL;
     }
     const size_t bytes = sizeof(float) * (size_t)k->total;
-    HIP_TRY(hipMemcpyAsync(k->A.h, h, bytes, hipMemcpyHostToDevice, k->stream));
-    HIP_TRY(hipMemcpyAsync(k->A.hu, hu, bytes, hipMemcpyHostToDevice, k->stream));
-    HIP_TRY(hipMemcpyAsync(k->A.hv, hv, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->A.h[0], h, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->A.hu[0], hu, bytes, hipMemcpyHostToDevice, k->stream));
+    HIP_TRY(hipMemcpyAsync(k->A.hv[0], hv, bytes, hipMemcpyHostToDevice, k->stream));
     HIP_TRY(hipMemcpyAsync(k->A.b, b, bytes, hipMemcpyHostToDevice, k->stream));
     std::vector<BatchBlockState> st(k->nblocks);
     for (auto& s : st) {
@@ -672,6 +828,7 @@ int swe_hip_batch_upload(swe_hip_batch* k, const float* h, const float* hu, cons
     g.nblocks = k->nblocks;
     g.stop = 1;
     g.mlp = k->mlp;
+    g.fused = k->fused;
     HIP_TRY(hipMemcpyAsync(k->dG, &g, sizeof(g), hipMemcpyHostToDevice, k->stream));
     hipLaunchKernelGGL(batch_init_kernel, dim3(k->nblocks), dim3(kCtlThreads), 0, k->stream, k->A, k->dDesc);
     HIP_TRY(hipGetLastError());
@@ -686,9 +843,9 @@ int swe_hip_batch_download(swe_hip_batch* k, float* h, float* hu, float* hv, flo
         return SWE_HIP_EINVAL;
     }
     const size_t bytes = sizeof(float) * (size_t)k->total;
-    HIP_TRY(hipMemcpyAsync(h, k->A.h, bytes, hipMemcpyDeviceToHost, k->stream));
-    HIP_TRY(hipMemcpyAsync(hu, k->A.hu, bytes, hipMemcpyDeviceToHost, k->stream));
-    HIP_TRY(hipMemcpyAsync(hv, k->A.hv, bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipMemcpyAsync(h, k->A.h[0], bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipMemcpyAsync(hu, k->A.hu[0], bytes, hipMemcpyDeviceToHost, k->stream));
+    HIP_TRY(hipMemcpyAsync(hv, k->A.hv[0], bytes, hipMemcpyDeviceToHost, k->stream));
     if (b) HIP_TRY(hipMemcpyAsync(b, k->A.b, bytes, hipMemcpyDeviceToHost, k->stream));
     HIP_TRY(hipStreamSynchronize(k->stream));
     return SWE_HIP_OK;
@@ -717,6 +874,7 @@ int swe_hip_batch_run(swe_hip_batch* k, int units, float* dt_log, int* iter_log,
     }
     const long long maxIter = (long long)units * 4 * (kMaxDivisor + 1) + 2LL * k->chunk;
     rc = advance(k, units, maxIter, nullptr);
+    if (rc == 0) rc = finish_run(k);
     if (elapsed_ms && rc == 0) {
         HIP_TRY(hipEventRecord(e1, k->stream));
         HIP_TRY(hipEventSynchronize(e1));
@@ -746,14 +904,20 @@ int swe_hip_batch_stats(swe_hip_batch* k, int64_t* updates, float* t) {
     return SWE_HIP_OK;
 }
 
-int swe_hip_batch_set_tuning(swe_hip_batch* k, int xs, int groups, int chunk, int graph) {
+int swe_hip_batch_set_tuning(swe_hip_batch* k, int xs, int groups, int chunk, int graph, int fused) {
     if (!k || (xs != -1 && xs != 4 && xs != 8 && xs != 16) || (groups != -1 && (groups < 1 || groups > 65535)) ||
-        (chunk != -1 && (chunk < 1 || chunk > 256)) || (graph != -1 && graph != 0 && graph != 1)) {
-        set_error("swe_hip_batch_set_tuning: invalid argument (xs=%d groups=%d chunk=%d graph=%d)", xs, groups, chunk,
-                  graph);
+        (chunk != -1 && (chunk < 1 || chunk > 256)) || (graph != -1 && graph != 0 && graph != 1) ||
+        (fused != -1 && fused != 0 && fused != 1)) {
+        set_error("swe_hip_batch_set_tuning: invalid argument (xs=%d groups=%d chunk=%d graph=%d fused=%d)", xs, groups,
+                  chunk, graph, fused);
         return SWE_HIP_EINVAL;
     }
     HIP_TRY(hipStreamSynchronize(k->stream));
+    if (fused != -1 && fused != k->fused) {
+        // the state between runs has no pending update, so the strategy may change here
+        HIP_TRY(hipMemcpy(&k->dG->fused, &fused, sizeof(int), hipMemcpyHostToDevice));
+        k->fused = fused;
+    }
     if (xs != -1 && xs != k->xs) {
         int rc = tiles_for(k, xs, true);
         if (rc) return rc;
@@ -783,7 +947,7 @@ int swe_hip_batch_set_surrogate(swe_hip_batch* k, swe_hip_mlp* m) {
     return SWE_HIP_OK;
 }
 
-int swe_hip_batch_get_tuning(swe_hip_batch* k, int* xs, int* groups, int* chunk, int* graph) {
+int swe_hip_batch_get_tuning(swe_hip_batch* k, int* xs, int* groups, int* chunk, int* graph, int* fused) {
     if (!k) {
         set_error("swe_hip_batch_get_tuning: null handle");
         return SWE_HIP_EINVAL;
@@ -792,6 +956,7 @@ int swe_hip_batch_get_tuning(swe_hip_batch* k, int* xs, int* groups, int* chunk,
     if (groups) *groups = k->groups;
     if (chunk) *chunk = k->chunk;
     if (graph) *graph = k->useGraph;
+    if (fused) *fused = k->fused;
     return SWE_HIP_OK;
 }
 
@@ -821,9 +986,9 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
         auto cp = [&](void* a, void* b, size_t n) {
             if (r == hipSuccess) r = hipMemcpyAsync(save ? a : b, save ? b : a, n, hipMemcpyDeviceToDevice, k->stream);
         };
-        cp(sh, k->A.h, bytes);
-        cp(shu, k->A.hu, bytes);
-        cp(shv, k->A.hv, bytes);
+        cp(sh, k->A.h[0], bytes);
+        cp(shu, k->A.hu[0], bytes);
+        cp(shv, k->A.hv[0], bytes);
         cp(sl, k->A.lines, lbytes);
         cp(sst, k->dState, sizeof(BatchBlockState) * k->nblocks);
         cp(sg, k->dG, sizeof(BatchGlobals));
@@ -842,7 +1007,7 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
     const int units = 1 << 30;  // never reached: the timed iterations run through macro-step ends
     for (int a = 0; a < 3 && e == hipSuccess && rc == 0; ++a) {
         for (int g = 0; g < 3 && e == hipSuccess && rc == 0; ++g) {
-            rc = swe_hip_batch_set_tuning(k, xsC[a], gC[g] * k->cus, std::min(iterations, 256), -1);
+            rc = swe_hip_batch_set_tuning(k, xsC[a], gC[g] * k->cus, std::min(iterations, 256), -1, -1);
             if (rc) break;
             // warm-up chunk (also captures the graph), then the timed chunks
             hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(1), 0, k->stream, k->dG, units, k->logCap,
@@ -880,7 +1045,7 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "swe_hip_batch_autotune");
     if (ncand) *ncand = n;
-    return swe_hip_batch_set_tuning(k, bestXs, bestGroups, oldChunk, -1);
+    return swe_hip_batch_set_tuning(k, bestXs, bestGroups, oldChunk, -1, -1);
 }
 
 }  // extern "C"

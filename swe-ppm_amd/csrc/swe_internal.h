@@ -75,5 +75,8 @@ int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, i
 template <int S>
 int launch_batch_sweep_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
                               BatchGlobals* G, const int* items, hipStream_t stream);
+template <int S>
+int launch_batch_fused_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+                              BatchGlobals* G, const int* items, hipStream_t stream);
 
 }  // namespace swe

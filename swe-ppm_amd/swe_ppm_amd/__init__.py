@@ -70,8 +70,8 @@ def _load() -> C.CDLL:
         "swe_hip_batch_download": (C.c_int, [vp, fp, fp, fp, fp]),
         "swe_hip_batch_run": (C.c_int, [vp, C.c_int, fp, ip, fp]),
         "swe_hip_batch_stats": (C.c_int, [vp, C.POINTER(C.c_int64), fp]),
-        "swe_hip_batch_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
-        "swe_hip_batch_get_tuning": (C.c_int, [vp, ip, ip, ip, ip]),
+        "swe_hip_batch_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "swe_hip_batch_get_tuning": (C.c_int, [vp, ip, ip, ip, ip, ip]),
         "swe_hip_batch_autotune": (C.c_int, [vp, C.c_int, fp, ip]),
         "swe_hip_mlp_create": (C.c_int, [C.c_int64, C.c_uint64, C.POINTER(vp)]),
         "swe_hip_mlp_destroy": (C.c_int, [vp]),
@@ -299,11 +299,12 @@ class SweBatch:
         _check(lib.swe_hip_batch_stats(self._h, upd.ctypes.data_as(C.POINTER(C.c_int64)), _fptr(t)), "batch_stats")
         return upd, t
 
-    def set_tuning(self, xs: int = -1, groups: int = -1, chunk: int = -1, graph: int = -1) -> None:
-        _check(lib.swe_hip_batch_set_tuning(self._h, xs, groups, chunk, graph), "batch_set_tuning")
+    def set_tuning(self, xs: int = -1, groups: int = -1, chunk: int = -1, graph: int = -1, fused: int = -1) -> None:
+        _check(lib.swe_hip_batch_set_tuning(self._h, xs, groups, chunk, graph, fused), "batch_set_tuning")
 
     def get_tuning(self):
-        v = [C.c_int() for _ in range(4)]
+        """(xs, groups, chunk, graph, fused)"""
+        v = [C.c_int() for _ in range(5)]
         _check(lib.swe_hip_batch_get_tuning(self._h, *[C.byref(x) for x in v]), "batch_get_tuning")
         return tuple(x.value for x in v)
 

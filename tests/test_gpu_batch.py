@@ -30,14 +30,15 @@ def _assert_blocks_equal(oracle, got, want, blocks, what):
             assert np.array_equal(a, c), f"{what}: block {k} {name} ghost cells differ"
 
 
-@pytest.mark.parametrize("xs,chunk,graph", [(8, 16, 1), (4, 7, 1), (16, 5, 0)])
-def test_ensemble_matches_independent_oracle_runs(swe, oracle, xs, chunk, graph):
+@pytest.mark.parametrize("xs,chunk,graph,fused", [(8, 16, 1, 1), (4, 7, 1, 1), (16, 5, 0, 1), (8, 16, 1, 0),
+                                                  (16, 5, 0, 0)])
+def test_ensemble_matches_independent_oracle_runs(swe, oracle, xs, chunk, graph, fused):
     from swe_ppm_amd import scenarios
 
     inst = scenarios.ensemble_instances(count=24, seed=7, nmin=8, nmax=140)
     nsteps = 37
     with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
-        bt.set_tuning(xs=xs, chunk=chunk, graph=graph)
+        bt.set_tuning(xs=xs, chunk=chunk, graph=graph, fused=fused)
         bt.upload([i["h"] for i in inst], [i["hu"] for i in inst], [i["hv"] for i in inst], [i["b"] for i in inst])
         dts, its, ms = bt.run(nsteps)
         got = bt.download()
@@ -71,14 +72,14 @@ def test_ensemble_runs_continue_across_calls(swe, oracle):
     _assert_blocks_equal(oracle, got, want, inst, "ensemble 1+4+9 steps")
 
 
-@pytest.mark.parametrize("solver", [2, 3])
-def test_global_dt_blocks_match_oracle(swe, oracle, solver):
+@pytest.mark.parametrize("solver,fused", [(2, 1), (3, 1), (2, 0)])
+def test_global_dt_blocks_match_oracle(swe, oracle, solver, fused):
     from swe_ppm_amd import scenarios
 
     blocks, dx, dy = scenarios.lts_tiling(n=256)
     nsteps = 25
     with swe.SweBatch(blocks, solver=solver, mode=swe.BATCH_GLOBAL, dx=dx, dy=dy) as bt:
-        bt.set_tuning(chunk=6)
+        bt.set_tuning(chunk=6, fused=fused)
         bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
         dts, its, _ = bt.run(nsteps)
         got = bt.download()
@@ -89,8 +90,9 @@ def test_global_dt_blocks_match_oracle(swe, oracle, solver):
     _assert_blocks_equal(oracle, got, states, blocks, "global dt")
 
 
-@pytest.mark.parametrize("T,duration,nmacro", [(1.0, 3.0, 3), (1.0, 2.5, 3), (0.5, 1.2, 3)])
-def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro):
+@pytest.mark.parametrize("T,duration,nmacro,fused", [(1.0, 3.0, 3, 1), (1.0, 2.5, 3, 1), (0.5, 1.2, 3, 1),
+                                                     (1.0, 3.0, 3, 0), (0.5, 1.2, 3, 0)])
+def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro, fused):
     """Local time stepping over 28 uneven blocks: copies, interpolated ghost
     layers, power-of-two sub-steps, blocks reaching the end time mid-step."""
     from swe_ppm_amd import scenarios
@@ -98,7 +100,7 @@ def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro):
     blocks, dx, dy = scenarios.lts_tiling(n=256)
     with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=swe.BATCH_LTS, dx=dx, dy=dy, max_local=T,
                       duration=duration) as bt:
-        bt.set_tuning(chunk=5)
+        bt.set_tuning(chunk=5, fused=fused)
         bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
         _, its, _ = bt.run(nmacro)
         got = bt.download()
