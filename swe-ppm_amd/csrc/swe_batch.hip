@@ -74,138 +74,146 @@ __global__ __launch_bounds__(kCtlThreads) void batch_init_kernel(BatchArena A, c
     }
 }
 
+// The ghost-line kernels run one workgroup per (block, side, chunk of the
+// line): the lines of a block are short and strided (bottom / top), so
+// spreading them keeps each thread at one or two elements.
+constexpr int kSideChunks = 4;
+
 // ---- ghost: setGhostLayer of every block, CONNECT_WITHIN_RANK part
 // (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304; the OUTFLOW / WALL part
-// is written by the previous apply).  One workgroup per block.
+// is written with the updated cells).  Grid (blocks, 4 sides, kSideChunks).
 __global__ __launch_bounds__(kCtlThreads) void batch_ghost_kernel(BatchArena A, const BatchBlockDesc* D,
                                                                   BatchBlockState* st, BatchGlobals* G) {
-    const int blk = blockIdx.x;
+    const int blk = blockIdx.x, side = blockIdx.y, chunk = blockIdx.z;
     const int stop = G->stop;
-    if (blk == 0 && threadIdx.x == 0) {
+    if (blk == 0 && side == 0 && chunk == 0 && threadIdx.x == 0) {
         G->count = 0;
         G->head = 0;
         G->headApply = 0;
         G->skip = stop;
     }
     if (stop) return;
-    const int mode = G->mode;
-    const bool lts = mode == BATCH_LTS;
-    const float T = G->T;
-    const BatchBlockDesc d = D[blk];
+    const BatchBlockDesc& d = D[blk];
+    if (d.bnd[side] != CONNECT_WITHIN_RANK) return;
+    const bool lts = G->mode == BATCH_LTS;
     BatchBlockState& s = st[blk];
-    const int order[4] = {BND_RIGHT, BND_LEFT, BND_TOP, BND_BOTTOM};
-    for (int k = 0; k < 4; ++k) {
-        const int side = order[k];
-        if (d.bnd[side] != CONNECT_WITHIN_RANK) continue;
-        if (lts && !lts_receivable(s, side)) continue;  // uniform: s is only read here
-        const int nb = d.nbr[side];
-        const BatchBlockDesc dn = D[nb];
-        if (lts && threadIdx.x == 0) s.borderTs[side] = lts_total(T, st[nb]);
-        long long so, go;
-        int ss, gs, n, nn;
-        side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
-        side_line(d, side, 1, go, gs, n);        // my ghost line
-        // the neighbour's current state: its buffer, with a pending update applied
-        const BatchBlockState& sn = st[nb];
-        const int ncur = sn.cur;
-        const bool npend = sn.pend != 0;
-        const float ndt = sn.maxTimestep;
-        const float ndtdx = ndt / dn.dx, ndtdy = ndt / dn.dy;
-        const float* sh = A.h[ncur] + dn.off + so;
-        const float* shu = A.hu[ncur] + dn.off + so;
-        const float* shv = A.hv[ncur] + dn.off + so;
-        const SumsView ps = batch_sums(A, dn, sn.sidx);
-        auto value = [&](int i) -> Cell {
-            const long long c = so + (long long)i * ss;
-            Cell q{sh[(long long)i * ss], shu[(long long)i * ss], shv[(long long)i * ss]};
-            if (npend) q = update_cell(q.h, q.hu, q.hv, ndtdx, ndtdy, ps.sx[c], ps.sy[c], ps.su[c], ps.sv[c]);
-            return q;
-        };
-        if (lts) {
-            float* L = A.lines + d.line[side];
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const Cell q = value(i);
-                L[i] = q.h;
-                L[n + i] = q.hu;
-                L[2 * n + i] = q.hv;
-            }
-        } else {
-            float* gh = A.h[s.cur] + d.off + go;
-            float* ghu = A.hu[s.cur] + d.off + go;
-            float* ghv = A.hv[s.cur] + d.off + go;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const Cell q = value(i);
-                gh[(long long)i * gs] = q.h;
-                ghu[(long long)i * gs] = q.hu;
-                ghv[(long long)i * gs] = q.hv;
-            }
+    if (lts && !lts_receivable(s, side)) return;  // s.gl / s.notified are only read here
+    const int nb = d.nbr[side];
+    const BatchBlockDesc& dn = D[nb];
+    const BatchBlockState& sn = st[nb];
+    if (lts && chunk == 0 && threadIdx.x == 0) s.borderTs[side] = lts_total(G->T, sn);
+    long long so, go;
+    int ss, gs, n, nn;
+    side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
+    side_line(d, side, 1, go, gs, n);        // my ghost line
+    // the neighbour's current state: its buffer, with a pending update applied
+    const int ncur = sn.cur;
+    const bool npend = sn.pend != 0;
+    const float ndt = sn.maxTimestep;
+    const float ndtdx = ndt / dn.dx, ndtdy = ndt / dn.dy;
+    const float* sh = A.h[ncur] + dn.off + so;
+    const float* shu = A.hu[ncur] + dn.off + so;
+    const float* shv = A.hv[ncur] + dn.off + so;
+    const SumsView ps = batch_sums(A, dn, sn.sidx);
+    auto value = [&](int i) -> Cell {
+        const long long c = so + (long long)i * ss;
+        Cell q{sh[(long long)i * ss], shu[(long long)i * ss], shv[(long long)i * ss]};
+        if (npend) q = update_cell(q.h, q.hu, q.hv, ndtdx, ndtdy, ps.sx[c], ps.sy[c], ps.su[c], ps.sv[c]);
+        return q;
+    };
+    const int stride = blockDim.x * gridDim.z;
+    if (lts) {
+        float* L = A.lines + d.line[side];
+        for (int i = chunk * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const Cell q = value(i);
+            L[i] = q.h;
+            L[n + i] = q.hu;
+            L[2 * n + i] = q.hv;
+        }
+    } else {
+        float* gh = A.h[s.cur] + d.off + go;
+        float* ghu = A.hu[s.cur] + d.off + go;
+        float* ghv = A.hv[s.cur] + d.off + go;
+        for (int i = chunk * blockDim.x + threadIdx.x; i < n; i += stride) {
+            const Cell q = value(i);
+            gh[(long long)i * gs] = q.h;
+            ghu[(long long)i * gs] = q.hu;
+            ghv[(long long)i * gs] = q.hv;
         }
     }
 }
 
-// ---- check: checkAllGhostlayers (SWE_Block.hh:375-436) and queue fill.
+// ---- check, part 1: the ghost-line copy or interpolation of
+// checkAllGhostlayers (SWE_Block.hh:375-436), old + (new - old) * (t_me /
+// t_neighbour).  Grid (blocks, 4 sides, kSideChunks); LTS only.
+__global__ __launch_bounds__(kCtlThreads) void batch_check_sides_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                                        const BatchBlockState* st, BatchGlobals* G) {
+    if (G->skip || G->mode != BATCH_LTS) return;
+    const int blk = blockIdx.x, side = blockIdx.y, chunk = blockIdx.z;
+    const BatchBlockDesc& d = D[blk];
+    if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) return;
+    const BatchBlockState& s = st[blk];
+    const float tot = lts_total(G->T, s);
+    const float bt = s.borderTs[side];
+    int act = 0;  // 1 copy, 2 interpolate
+    float w = 0.f;
+    if ((double)fabsf(bt - tot) <= DBL_EPSILON) {
+        act = 1;
+    } else if (bt > tot) {
+        act = 2;
+        w = tot / bt;
+    }
+    if (!act) return;
+    long long go;
+    int gs, n;
+    side_line(d, side, 1, go, gs, n);
+    const float* L = A.lines + d.line[side];
+    float* f[3] = {A.h[s.cur] + d.off + go, A.hu[s.cur] + d.off + go, A.hv[s.cur] + d.off + go};
+    for (int i = chunk * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.z) {
+        for (int c = 0; c < 3; ++c) {
+            float& g = f[c][(long long)i * gs];
+            const float nv = L[c * n + i];
+            g = (act == 1) ? nv : (g + (nv - g) * w);
+        }
+    }
+}
+
+// ---- check, part 2: the state transitions of checkAllGhostlayers
+// (SWE_Block.hh:375-436: ghost-layer states, notifiedLastTimestep, the
+// sub-step counter) and the queue fill with the tiles of every block in sync.
 // One workgroup per block.
-__global__ __launch_bounds__(kCtlThreads) void batch_check_kernel(BatchArena A, const BatchBlockDesc* D,
-                                                                  BatchBlockState* st, BatchGlobals* G, int* items) {
+__global__ __launch_bounds__(kCtlThreads) void batch_check_state_kernel(const BatchBlockDesc* D, BatchBlockState* st,
+                                                                        BatchGlobals* G, int* items) {
     if (G->skip) return;
     const int blk = blockIdx.x;
-    const BatchBlockDesc d = D[blk];
+    const BatchBlockDesc& d = D[blk];
     BatchBlockState& s = st[blk];
-    __shared__ int action[4];  // 0 none, 1 copy, 2 interpolate
-    __shared__ float ratio[4];
     __shared__ int activeS, base;
-    if (G->mode == BATCH_LTS) {
-        const float T = G->T, duration = G->duration;
-        if (threadIdx.x == 0) {
-            const float tot = lts_total(T, s);
+    if (threadIdx.x == 0) {
+        if (G->mode == BATCH_LTS) {
+            const float duration = G->duration;
+            const float tot = lts_total(G->T, s);
             for (int side = 0; side < 4; ++side) {
-                action[side] = 0;
                 if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) continue;
                 const float bt = s.borderTs[side];
-                if ((double)fabsf(bt - tot) <= DBL_EPSILON) {
-                    action[side] = 1;
+                if ((double)fabsf(bt - tot) <= DBL_EPSILON)
                     s.gl[side] = GL_SYNC;
-                } else if (bt > tot) {
-                    action[side] = 2;
-                    ratio[side] = tot / bt;  // interpolation weight t_me / t_neighbour
+                else if (bt > tot)
                     s.gl[side] = GL_INTER;
-                } else {
+                else
                     s.gl[side] = GL_UNVALID;
-                }
                 if (bt >= duration) s.gl[side] = GL_DONE;
             }
-        }
-        __syncthreads();
-        for (int side = 0; side < 4; ++side) {
-            const int act = action[side];
-            if (act == 0) continue;
-            long long go;
-            int gs, n;
-            side_line(d, side, 1, go, gs, n);
-            const float* L = A.lines + d.line[side];
-            float* f[3] = {A.h[s.cur] + d.off + go, A.hu[s.cur] + d.off + go, A.hv[s.cur] + d.off + go};
-            const float w = ratio[side];
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                for (int c = 0; c < 3; ++c) {
-                    float& g = f[c][(long long)i * gs];
-                    const float nv = L[c * n + i];
-                    g = (act == 1) ? nv : (g + (nv - g) * w);
-                }
-            }
-        }
-        if (threadIdx.x == 0) {
-            const float tot = lts_total(T, s);
             if ((double)fabsf(duration - tot) <= DBL_EPSILON || tot >= duration) s.notified = 1;
             if (lts_in_sync(s))
                 for (int side = 0; side < 4; ++side)
                     if (s.gl[side] != GL_DONE) s.gl[side] = (s.gl[side] == GL_INTER) ? GL_INTER : GL_NEXT;
             if (lts_in_sync(s) && s.stepSizeCounter < s.stepSize) s.stepSizeCounter++;
             s.active = lts_in_sync(s) ? 1 : 0;
-            activeS = s.active;
+        } else {
+            s.active = 1;
         }
-    } else if (threadIdx.x == 0) {
-        s.active = 1;
-        activeS = 1;
+        activeS = s.active;
     }
     __syncthreads();
     if (!activeS) return;
@@ -497,8 +505,11 @@ int default_groups(swe_hip_batch* k) {
 // the launches of one driver iteration
 int enqueue_iteration(swe_hip_batch* k) {
     const dim3 blk(kCtlThreads);
-    hipLaunchKernelGGL(batch_ghost_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
-    hipLaunchKernelGGL(batch_check_kernel, dim3(k->nblocks), blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG,
+    const dim3 sides(k->nblocks, 4, kSideChunks);
+    hipLaunchKernelGGL(batch_ghost_kernel, sides, blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
+    if (k->mode == BATCH_LTS)
+        hipLaunchKernelGGL(batch_check_sides_kernel, sides, blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
+    hipLaunchKernelGGL(batch_check_state_kernel, dim3(k->nblocks), blk, 0, k->stream, k->dDesc, k->dState, k->dG,
                        k->dItems);
     const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
     if (k->fused) {
