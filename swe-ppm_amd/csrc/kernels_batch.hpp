@@ -9,27 +9,36 @@
 //     LTS loop, swe_mpi_overdecomp.cpp:263-343, with the per-block state
 //     machine of SWE_Block.hh:311-562).
 //
-// One driver iteration is five launches.  Every launch reads its parameters
-// from device memory, so a chunk of iterations is captured once in a hipGraph
-// and replayed:
-//   ghost : setGhostLayer of every block: neighbour time stamp and copy layer
-//           of CONNECT_WITHIN_RANK sides
-//           (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304), straight into
-//           the ghost line (global dt) or into a receive line (LTS); resets
-//           the work queue
-//   check : checkAllGhostlayers (SWE_Block.hh:375-436): ghost copy or
-//           interpolation from the receive line; the strip tiles of every
-//           block in sync are appended to the work queue
+// One driver iteration is three launches (five with a separate apply pass).
+// Every launch reads its parameters from device memory, so a chunk of
+// iterations is captured once in a hipGraph and replayed:
+//   lines : per (block, side, piece of the side) workgroup, setGhostLayer's
+//           CONNECT_WITHIN_RANK part
+//           (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304): the
+//           neighbour's copy layer straight into the ghost line (global dt),
+//           or, for LTS, into the side's receive line when the side receives
+//           this iteration, followed by checkAllGhostlayers' copy or
+//           interpolation (SWE_Block.hh:375-436).
 //   sweep : waves pull (block, tile) items from the queue and solve the
-//           tile's edges: per-cell net-update sums + per-block CFL maximum
-//   round : per-block dt (CFL; LTS: getRoundTimestep; global: minimum over
-//           blocks), the driver's completion test (hasMaxLocalTimestep) and
-//           resetStepSizeCounter at the end of a macro step
-//   apply : waves pull the same items and update their cells, writing the
-//           OUTFLOW / WALL ghosts that the next setGhostLayer would write
+//           tile's edges: per-cell net-update sums + per-block CFL maximum;
+//           with the fused strategy a block's pending update is applied as
+//           its cells are loaded.
+//   round : one workgroup: per-block dt (CFL; LTS: getRoundTimestep; global:
+//           minimum over blocks), the driver's completion test
+//           (hasMaxLocalTimestep), resetStepSizeCounter at the end of a macro
+//           step; then, as every block's time is now known, the next
+//           iteration's LTS bookkeeping: which sides receive and with what
+//           time stamp (receiveGhostLayer, SWE_Block.hh:311-374) and the
+//           ghost-layer state transitions of checkAllGhostlayers; and the
+//           work queue of the next iteration (the tiles of the blocks in
+//           sync).
+//   apply : (store strategy only) waves pull the same items and update their
+//           cells, writing the OUTFLOW / WALL ghosts that the next
+//           setGhostLayer would write; a one-workgroup fill launch then
+//           builds the next queue.
 // The sweep and apply grids are persistent (sized to the machine, not to the
-// work): each wave loops until the queue is drained.  No wave ever waits for
-// another inside a launch; the launch boundaries order the phases.
+// work): each wave loops until the queue is drained.  No workgroup ever waits
+// for another inside a launch; the launch boundaries order the phases.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,7 +62,8 @@ struct BatchBlockDesc {
     long long off;      // offset of the block's fields in the arena (floats)
     int bnd[4];         // BoundaryType per side
     int nbr[4];         // neighbour block index or -1
-    int nyt, ntiles;    // strip tiles: nyt * ceil(nx / xs)
+    int nyt, ntiles;    // strips of kStripRows rows, work items (tile_lanes)
+    int nxs, pack;      // column segments of xs columns, tail segments per packed item
     long long line[4];  // LTS receive lines (h, hu, hv of one side) in the line arena
 };
 
@@ -68,10 +78,10 @@ struct BatchBlockState {
     int cur;     // state buffer (0/1) holding the block
     int pend;    // fused strategy: the last update (sums set sidx, dt maxTimestep) is not yet applied
     int sidx;    // sums set of the pending update
-    int pad;
+    int recv;    // LTS: sides (bit per side) whose receive line takes the neighbour's layer this iteration
     long long updates;  // updates performed
     float t;            // simulated time of an independent block
-    int pad2;
+    float total;        // LTS: lts_total at the end of the last iteration (what neighbours read)
 };
 
 // State double-buffered per block (the fused sweep reads one buffer and
@@ -100,7 +110,7 @@ struct BatchGlobals {
     long long iterations;  // iterations that did work
     const unsigned* mlp;   // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
     int fused;             // 1: fused sweep (update applied on the next load), 0: sweep + apply
-    int pad;
+    int allItems;          // work items of all blocks (the queue of an iteration without LTS)
 };
 
 __device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d, int buf) {
@@ -190,21 +200,82 @@ __device__ __forceinline__ int next_items(int* head, int* sBase) {
     return base;
 }
 
+// ---- tiles -----------------------------------------------------------------
+// A block's work items: first one item per (strip, column segment) for the
+// strips below the last one, then the last strip (the tail, R = ny - 62 *
+// (nyt - 1) rows).  When R is small the tail's inner column segments (full
+// XS columns, away from the left and right boundary columns) share waves:
+// `pack` segments per item, each on a group of R + 2 lanes (R rows with a
+// halo lane on either side) laid side by side in the wave.  Every lane
+// computes what the lane of an unpacked strip computes for its row and
+// column, so the packing does not change any result.
+struct TileLanes {
+    int yt;           // strip (uniform)
+    int xs0;          // first column segment of the item (uniform)
+    int xs;           // column segment of this lane's group
+    int r;            // row of the lane in the strip: 0 and R+1 are halo rows; 63 on idle lanes
+    bool first, last; // the item's segments are the block's first / last (uniform)
+};
+
+// Segments per packed tail item for a tail of R rows over nxs segments; 1 =
+// no packing.
+inline int tail_pack(int R, int nxs) {
+    const int p = 64 / (R + 2);
+    return (p >= 2 && nxs >= 4) ? p : 1;
+}
+
+__device__ __forceinline__ TileLanes tile_lanes(const BatchBlockDesc& d, int tile, int lane) {
+    TileLanes t;
+    const int body = (d.nyt - 1) * d.nxs;
+    int lanesPerGroup = 64, groups = 1;
+    if (tile < body) {
+        t.yt = tile % (d.nyt - 1);
+        t.xs0 = tile / (d.nyt - 1);
+    } else {
+        const int u = tile - body;
+        t.yt = d.nyt - 1;
+        if (d.pack == 1) {
+            t.xs0 = u;
+        } else if (u < 2) {
+            t.xs0 = u ? d.nxs - 1 : 0;
+        } else {
+            t.xs0 = 1 + (u - 2) * d.pack;
+            groups = min(d.pack, d.nxs - 1 - t.xs0);
+            lanesPerGroup = d.ny - kStripRows * t.yt + 2;
+        }
+    }
+    const int g = lane / lanesPerGroup;
+    t.xs = t.xs0 + g;
+    t.r = lane - g * lanesPerGroup;
+    if (g >= groups) {
+        t.xs = t.xs0;
+        t.r = 63;
+    }
+    t.first = t.xs0 == 0;
+    t.last = t.xs0 == d.nxs - 1;
+    return t;
+}
+
+// columns of every segment of the item: XS, fewer for the last
+__device__ __forceinline__ int tile_columns(const TileLanes& t, int nx, int xs) {
+    return min(xs, nx - t.xs0 * xs);
+}
+
 // ---- sweep: net-update sums of one strip tile ------------------------------
-// Same edge order and arithmetic as the single-block strip sweep: the wave
-// owns rows 62*yt+1 .. 62*yt+62 (lanes 1..62, lanes 0 and 63 are halo) and
-// marches over columns 1+xs*XS .. min(xs*XS+XS, nx).
+// Same edge order and arithmetic as the single-block strip sweep: the lanes
+// of a group own rows 62*yt+1 .. 62*yt+62 (rows r = 1..62; r = 0 and the
+// group's top lane are halo) and march over columns 1+xs*XS .. xs*XS+n.
 template <int S, int XS>
 __device__ __forceinline__ float strip_sums(const dev::SolverCtx<S>& solver, const StateView& v, const SumsView& sums,
-                                            int yt, int xs, int lane) {
-    const int y = kStripRows * yt + lane;
+                                            const TileLanes& t) {
+    const int y = kStripRows * t.yt + t.r;
     const bool rowValid = y <= v.ny + 1;
     const int yc = rowValid ? y : v.ny + 1;
-    const bool owned = (lane >= 1) && (lane <= kStripRows) && (y <= v.ny);
+    const bool owned = (t.r >= 1) && (t.r <= kStripRows) && (y <= v.ny);
     const bool rowInterior = (y >= 1) && (y <= v.ny);
-    const bool hEdgeValid = (lane <= kStripRows) && (y <= v.ny);
-    const int x0 = 1 + xs * XS;
-    const int x1 = min(x0 + XS - 1, v.nx);
+    const bool hEdgeValid = (t.r <= kStripRows) && (y <= v.ny);
+    const int x0 = 1 + t.xs * XS;
+    const int n = tile_columns(t, v.nx, XS);
     size_t o = idx(v, x0 - 1, yc);
     CellQ cq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
     o += v.pitch;
@@ -212,7 +283,8 @@ __device__ __forceinline__ float strip_sums(const dev::SolverCtx<S>& solver, con
     const dev::EdgeOut e = solver.solve(cq.xside(), nq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
-    for (int x = x0; x <= x1; ++x) {
+    for (int k = 0; k < n; ++k) {
+        const int x = x0 + k;
         cq = nq;
         o += v.pitch;
         nq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
@@ -253,7 +325,7 @@ __global__ __launch_bounds__(256) void batch_sweep_kernel(BatchArena A, const Ba
         const int tile = code & ((1 << kTileBits) - 1);
         const BatchBlockDesc& d = D[blk];
         const StateView v = batch_view(A, d, st[blk].cur);
-        const float m = strip_sums<S, XS>(solver, v, batch_sums(A, d, 0), tile % d.nyt, tile / d.nyt, lane);
+        const float m = strip_sums<S, XS>(solver, v, batch_sums(A, d, 0), tile_lanes(d, tile, lane));
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     }
 }
@@ -285,10 +357,11 @@ __global__ __launch_bounds__(256) void batch_apply_kernel(BatchArena A, const Ba
         const float dtdy = dt / v.dy;
         BoundaryInfo bi;
         for (int k = 0; k < 4; ++k) bi.type[k] = d.bnd[k];
-        const int y = kStripRows * (tile % d.nyt) + lane;
-        const bool owned = (lane >= 1) && (lane <= kStripRows) && (y <= v.ny);
-        const int x0 = 1 + (tile / d.nyt) * XS;
-        const int x1 = min(x0 + XS - 1, v.nx);
+        const TileLanes t = tile_lanes(d, tile, lane);
+        const int y = kStripRows * t.yt + t.r;
+        const bool owned = (t.r >= 1) && (t.r <= kStripRows) && (y <= v.ny);
+        const int x0 = 1 + t.xs * XS;
+        const int x1 = x0 + tile_columns(t, v.nx, XS) - 1;
         if (owned) {
             for (int x = x0; x <= x1; ++x) {
                 const size_t c = idx(v, x, y);
@@ -317,21 +390,21 @@ struct BQ3 {
 template <int S, int XS>
 __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, const StateView& v, const StateView& o,
                                             const SumsView& sIn, const SumsView& sOut, bool pend, float dt,
-                                            const int* bnd, int yt, int xs, int lane) {
+                                            const int* bnd, const TileLanes& t) {
     const int nx = v.nx, ny = v.ny;
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;
-    const int y = kStripRows * yt + lane;
+    const int y = kStripRows * t.yt + t.r;
     const bool rowValid = y <= ny + 1;
     const int yc = rowValid ? y : ny + 1;
-    const bool owned = (lane >= 1) && (lane <= kStripRows) && (y <= ny);
+    const bool owned = (t.r >= 1) && (t.r <= kStripRows) && (y <= ny);
     const bool rowInterior = (y >= 1) && (y <= ny);
-    const bool hEdgeValid = (lane <= kStripRows) && (y <= ny);
+    const bool hEdgeValid = (t.r <= kStripRows) && (y <= ny);
     const bool rowBottom = (y == 0);
     const bool rowTop = (y == ny + 1);
-    const bool carryRow = owned || (rowBottom && lane == 0) || (rowTop && lane >= 1);
-    const int x0 = 1 + xs * XS;
-    const int x1 = min(x0 + XS - 1, nx);
+    const bool carryRow = owned || (rowBottom && t.r == 0) || (rowTop && t.r >= 1);
+    const int x0 = 1 + t.xs * XS;
+    const int n = tile_columns(t, nx, XS);
     const int tB = bnd[BND_BOTTOM], tT = bnd[BND_TOP], tL = bnd[BND_LEFT], tR = bnd[BND_RIGHT];
     auto ow = [](int t) { return t == OUTFLOW || t == WALL; };
 
@@ -369,19 +442,21 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
 
     BQ3 cur = fix_rows(load_updated(x0));
     BQ3 prev = load_updated(x0 - 1);
-    if (x0 == 1) prev = fix_column(prev, cur, tL);
-    if (x0 == 1 && carryRow) store(0, prev);
+    if (t.first) prev = fix_column(prev, cur, tL);  // x0 == 1
+    if (t.first && carryRow) store(0, prev);
     CellQ pq = cell_q(prev.h, prev.hu, prev.hv, v.b[idx(v, x0 - 1, yc)]);
     CellQ cq = cell_q(cur.h, cur.hu, cur.hv, v.b[idx(v, x0, yc)]);
     const dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
-    for (int x = x0; x <= x1; ++x) {
+    for (int k = 0; k < n; ++k) {
+        const int x = x0 + k;
+        const bool rightGhost = t.last && k == n - 1;  // x + 1 == nx + 1
         BQ3 nxt = fix_rows(load_updated(x + 1));
-        if (x + 1 == nx + 1) nxt = fix_column(nxt, cur, tR);
+        if (rightGhost) nxt = fix_column(nxt, cur, tR);
         const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, v.b[idx(v, x + 1, yc)]);
         if (carryRow) store(x, cur);
-        if (x + 1 == nx + 1 && carryRow) store(nx + 1, nxt);
+        if (rightGhost && carryRow) store(nx + 1, nxt);
         const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
         const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
@@ -426,7 +501,7 @@ __global__ __launch_bounds__(256) void batch_fused_kernel(BatchArena A, const Ba
         const float dt = st[blk].maxTimestep;
         const float m = fused_tile<S, XS>(solver, batch_view(A, d, cur), batch_view(A, d, cur ^ 1),
                                           batch_sums(A, d, sidx), batch_sums(A, d, sidx ^ 1), pend, dt, d.bnd,
-                                          tile % d.nyt, tile / d.nyt, lane);
+                                          tile_lanes(d, tile, lane));
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     }
 }

@@ -1,7 +1,7 @@
-// swe_batch.hip -- batched engine of libswe_hip.so: control kernels of the
-// driver iteration (ghost exchange, LTS checks, queue fill, dt rounding), the
-// persistent apply, hipGraph capture / replay, the autotuner and the C ABI
-// of include/swe_hip_batch.h.
+// swe_batch.hip -- batched engine of libswe_hip.so: the edges kernel of the
+// driver iteration (ghost exchange, LTS checks, queue fill), the persistent
+// apply, start / end-of-run kernels, hipGraph capture / replay, the autotuner
+// and the C ABI of include/swe_hip_batch.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,34 +74,47 @@ __global__ __launch_bounds__(kCtlThreads) void batch_init_kernel(BatchArena A, c
     }
 }
 
-// The ghost-line kernels run one workgroup per (block, side, chunk of the
-// line): the lines of a block are short and strided (bottom / top), so
-// spreading them keeps each thread at one or two elements.
+// The lines launch runs one workgroup per (block, side, piece of the side):
+// the lines of a block are short and strided (bottom / top), so spreading
+// them keeps each thread at one or two elements.
 constexpr int kSideChunks = 4;
 
-// ---- ghost: setGhostLayer of every block, CONNECT_WITHIN_RANK part
-// (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304; the OUTFLOW / WALL part
-// is written with the updated cells).  Grid (blocks, 4 sides, kSideChunks).
-__global__ __launch_bounds__(kCtlThreads) void batch_ghost_kernel(BatchArena A, const BatchBlockDesc* D,
-                                                                  BatchBlockState* st, BatchGlobals* G) {
+// ---- lines: the ghost-line data of an iteration ----------------------------
+// Per (block, side, piece) workgroup of a CONNECT_WITHIN_RANK side:
+//   global dt: setGhostLayer's in-rank copy
+//     (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304): the neighbour's
+//     copy layer, with its pending update applied, into the ghost line;
+//   LTS: the same copy into the side's receive line when the side receives
+//     this iteration (bit `side` of BatchBlockState::recv), then
+//     checkAllGhostlayers' ghost-line step (SWE_Block.hh:375-436): copy when
+//     the time stamps agree, old + (new - old) * (t_me / t_neighbour) when
+//     the neighbour is ahead, nothing when it is behind.
+// Which sides receive, their time stamps and the block's own time were fixed
+// by the previous round (lts_transitions).  Grid (blocks, 4, kSideChunks).
+__global__ __launch_bounds__(kCtlThreads) void batch_lines_kernel(BatchArena A, const BatchBlockDesc* D,
+                                                                  const BatchBlockState* st, const BatchGlobals* G) {
+    if (G->skip) return;
     const int blk = blockIdx.x, side = blockIdx.y, chunk = blockIdx.z;
-    const int stop = G->stop;
-    if (blk == 0 && side == 0 && chunk == 0 && threadIdx.x == 0) {
-        G->count = 0;
-        G->head = 0;
-        G->headApply = 0;
-        G->skip = stop;
-    }
-    if (stop) return;
     const BatchBlockDesc& d = D[blk];
     if (d.bnd[side] != CONNECT_WITHIN_RANK) return;
+    const BatchBlockState& s = st[blk];
     const bool lts = G->mode == BATCH_LTS;
-    BatchBlockState& s = st[blk];
-    if (lts && !lts_receivable(s, side)) return;  // s.gl / s.notified are only read here
+    const bool receive = !lts || ((s.recv >> side) & 1);
+    int act = 0;  // LTS: 1 copy, 2 interpolate
+    float w = 0.f;
+    if (lts) {
+        const float bt = s.borderTs[side], tot = s.total;
+        if ((double)fabsf(bt - tot) <= DBL_EPSILON) {
+            act = 1;
+        } else if (bt > tot) {
+            act = 2;
+            w = tot / bt;  // interpolation weight t_me / t_neighbour
+        }
+        if (!receive && !act) return;
+    }
     const int nb = d.nbr[side];
     const BatchBlockDesc& dn = D[nb];
     const BatchBlockState& sn = st[nb];
-    if (lts && chunk == 0 && threadIdx.x == 0) s.borderTs[side] = lts_total(G->T, sn);
     long long so, go;
     int ss, gs, n, nn;
     side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
@@ -121,114 +134,151 @@ __global__ __launch_bounds__(kCtlThreads) void batch_ghost_kernel(BatchArena A, 
         if (npend) q = update_cell(q.h, q.hu, q.hv, ndtdx, ndtdy, ps.sx[c], ps.sy[c], ps.su[c], ps.sv[c]);
         return q;
     };
-    const int stride = blockDim.x * gridDim.z;
-    if (lts) {
-        float* L = A.lines + d.line[side];
-        for (int i = chunk * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const Cell q = value(i);
-            L[i] = q.h;
-            L[n + i] = q.hu;
-            L[2 * n + i] = q.hv;
-        }
-    } else {
-        float* gh = A.h[s.cur] + d.off + go;
-        float* ghu = A.hu[s.cur] + d.off + go;
-        float* ghv = A.hv[s.cur] + d.off + go;
-        for (int i = chunk * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const Cell q = value(i);
-            gh[(long long)i * gs] = q.h;
-            ghu[(long long)i * gs] = q.hu;
-            ghv[(long long)i * gs] = q.hv;
-        }
-    }
-}
-
-// ---- check, part 1: the ghost-line copy or interpolation of
-// checkAllGhostlayers (SWE_Block.hh:375-436), old + (new - old) * (t_me /
-// t_neighbour).  Grid (blocks, 4 sides, kSideChunks); LTS only.
-__global__ __launch_bounds__(kCtlThreads) void batch_check_sides_kernel(BatchArena A, const BatchBlockDesc* D,
-                                                                        const BatchBlockState* st, BatchGlobals* G) {
-    if (G->skip || G->mode != BATCH_LTS) return;
-    const int blk = blockIdx.x, side = blockIdx.y, chunk = blockIdx.z;
-    const BatchBlockDesc& d = D[blk];
-    if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) return;
-    const BatchBlockState& s = st[blk];
-    const float tot = lts_total(G->T, s);
-    const float bt = s.borderTs[side];
-    int act = 0;  // 1 copy, 2 interpolate
-    float w = 0.f;
-    if ((double)fabsf(bt - tot) <= DBL_EPSILON) {
-        act = 1;
-    } else if (bt > tot) {
-        act = 2;
-        w = tot / bt;
-    }
-    if (!act) return;
-    long long go;
-    int gs, n;
-    side_line(d, side, 1, go, gs, n);
-    const float* L = A.lines + d.line[side];
     float* f[3] = {A.h[s.cur] + d.off + go, A.hu[s.cur] + d.off + go, A.hv[s.cur] + d.off + go};
-    for (int i = chunk * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.z) {
-        for (int c = 0; c < 3; ++c) {
-            float& g = f[c][(long long)i * gs];
-            const float nv = L[c * n + i];
-            g = (act == 1) ? nv : (g + (nv - g) * w);
+    const int i0 = chunk * blockDim.x + threadIdx.x, stride = blockDim.x * gridDim.z;
+    if (!lts) {
+        for (int i = i0; i < n; i += stride) {
+            const Cell q = value(i);
+            f[0][(long long)i * gs] = q.h;
+            f[1][(long long)i * gs] = q.hu;
+            f[2][(long long)i * gs] = q.hv;
         }
+        return;
+    }
+    float* L = A.lines + d.line[side];
+    for (int i = i0; i < n; i += stride) {
+        float nv[3];
+        if (receive) {
+            const Cell q = value(i);
+            L[i] = nv[0] = q.h;
+            L[n + i] = nv[1] = q.hu;
+            L[2 * n + i] = nv[2] = q.hv;
+        } else {
+            for (int c = 0; c < 3; ++c) nv[c] = L[c * n + i];
+        }
+        if (act)
+            for (int c = 0; c < 3; ++c) {
+                float& g = f[c][(long long)i * gs];
+                g = (act == 1) ? nv[c] : (g + (nv[c] - g) * w);
+            }
     }
 }
 
-// ---- check, part 2: the state transitions of checkAllGhostlayers
-// (SWE_Block.hh:375-436: ghost-layer states, notifiedLastTimestep, the
-// sub-step counter) and the queue fill with the tiles of every block in sync.
-// One workgroup per block.
-__global__ __launch_bounds__(kCtlThreads) void batch_check_state_kernel(const BatchBlockDesc* D, BatchBlockState* st,
-                                                                        BatchGlobals* G, int* items) {
-    if (G->skip) return;
-    const int blk = blockIdx.x;
-    const BatchBlockDesc& d = D[blk];
-    BatchBlockState& s = st[blk];
-    __shared__ int activeS, base;
-    if (threadIdx.x == 0) {
-        if (G->mode == BATCH_LTS) {
-            const float duration = G->duration;
-            const float tot = lts_total(G->T, s);
-            for (int side = 0; side < 4; ++side) {
-                if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) continue;
-                const float bt = s.borderTs[side];
-                if ((double)fabsf(bt - tot) <= DBL_EPSILON)
-                    s.gl[side] = GL_SYNC;
-                else if (bt > tot)
-                    s.gl[side] = GL_INTER;
-                else
-                    s.gl[side] = GL_UNVALID;
-                if (bt >= duration) s.gl[side] = GL_DONE;
-            }
-            if ((double)fabsf(duration - tot) <= DBL_EPSILON || tot >= duration) s.notified = 1;
-            if (lts_in_sync(s))
-                for (int side = 0; side < 4; ++side)
-                    if (s.gl[side] != GL_DONE) s.gl[side] = (s.gl[side] == GL_INTER) ? GL_INTER : GL_NEXT;
-            if (lts_in_sync(s) && s.stepSizeCounter < s.stepSize) s.stepSizeCounter++;
-            s.active = lts_in_sync(s) ? 1 : 0;
-        } else {
-            s.active = 1;
-        }
-        activeS = s.active;
+// ---- LTS bookkeeping of one block at the start of an iteration -------------
+// All that setGhostLayer / receiveGhostLayer and checkAllGhostlayers decide
+// at the start of an iteration follows from the blocks' times at the end of
+// the previous one (BatchBlockState::total): which sides receive and the
+// neighbour's time stamp they take (SWE_Block.hh:311-374), then the
+// ghost-layer states, notifiedLastTimestep, the sub-step counter and whether
+// the block is in sync (SWE_Block.hh:375-436).  The lines launch moves the
+// data afterwards.
+__device__ void lts_transitions(const BatchBlockDesc& d, BatchBlockState& s, const BatchBlockState* st,
+                                float duration) {
+    const float tot = s.total;
+    int recv = 0;
+    for (int side = 0; side < 4; ++side) {
+        if (d.bnd[side] != CONNECT_WITHIN_RANK || !lts_receivable(s, side)) continue;
+        recv |= 1 << side;
+        s.borderTs[side] = st[d.nbr[side]].total;
+    }
+    s.recv = recv;
+    for (int side = 0; side < 4; ++side) {
+        if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) continue;
+        const float bt = s.borderTs[side];
+        if ((double)fabsf(bt - tot) <= DBL_EPSILON)
+            s.gl[side] = GL_SYNC;
+        else if (bt > tot)
+            s.gl[side] = GL_INTER;
+        else
+            s.gl[side] = GL_UNVALID;
+        if (bt >= duration) s.gl[side] = GL_DONE;
+    }
+    if ((double)fabsf(duration - tot) <= DBL_EPSILON || tot >= duration) s.notified = 1;
+    if (lts_in_sync(s))
+        for (int side = 0; side < 4; ++side)
+            if (s.gl[side] != GL_DONE) s.gl[side] = (s.gl[side] == GL_INTER) ? GL_INTER : GL_NEXT;
+    if (lts_in_sync(s) && s.stepSizeCounter < s.stepSize) s.stepSizeCounter++;
+    s.active = lts_in_sync(s) ? 1 : 0;
+}
+
+// exclusive prefix sum over the workgroup (all threads, full waves)
+__device__ int wg_exclusive_scan(int v, int& total) {
+    __shared__ int sWave[kCtlThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sWave[wave] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        const int t = sWave[k];
+        if (k < wave) before += t;
+        all += t;
     }
     __syncthreads();
-    if (!activeS) return;
-    if (threadIdx.x == 0) base = atomicAdd(&G->count, d.ntiles);
+    total = all;
+    return before + x - v;
+}
+
+// ---- work queue of an iteration: the tiles of every active block -----------
+// One workgroup: a prefix sum over the blocks gives each block's first item,
+// then each wave writes the items of every fourth block.
+__device__ void fill_queue(const BatchBlockDesc* D, const BatchBlockState* st, BatchGlobals* G, int* items,
+                           bool all) {
+    __shared__ int sBase[kMaxBatchBlocks];
+    const int nb = G->nblocks;
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
+    auto queued = [&](int b) { return (all || st[b].active) ? D[b].ntiles : 0; };
+    int mine = 0;
+    for (int b = b0; b < b1; ++b) mine += queued(b);
+    int total;
+    int run = wg_exclusive_scan(mine, total);
+    for (int b = b0; b < b1; ++b) {
+        const int q = queued(b);
+        sBase[b] = q ? run : -1;
+        run += q;
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < d.ntiles; i += blockDim.x) items[base + i] = (blk << kTileBits) | i;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    for (int b = wave; b < nb; b += waves) {
+        const int base = sBase[b];
+        if (base < 0) continue;
+        const int nt = D[b].ntiles;
+        for (int i = lane; i < nt; i += 64) items[base + i] = (b << kTileBits) | i;
+    }
+    if (threadIdx.x == 0) {
+        G->count = total;
+        G->head = 0;
+        G->headApply = 0;
+        if (all) G->allItems = total;
+    }
+}
+
+// the queue of the next iteration and the iteration's skip flag
+__device__ void next_queue(const BatchBlockDesc* D, const BatchBlockState* st, BatchGlobals* G, int* items) {
+    if (G->mode == BATCH_LTS) {
+        fill_queue(D, st, G, items, false);
+    } else if (threadIdx.x == 0) {  // every block computes every iteration: the queue stays as filled
+        G->count = G->allItems;
+        G->head = 0;
+        G->headApply = 0;
+    }
+    if (threadIdx.x == 0) G->skip = G->stop;
 }
 
 // ---- round: per-block dt (SWE_DimensionalSplittingMpi.cpp:448-463), the
 // global minimum (swe_mpi_overdecomp.cpp:297-307), the completion test
 // (hasMaxLocalTimestep, SWE_Block.hh:355-358, swe_mpi_overdecomp.cpp:327-342)
-// and resetStepSizeCounter (SWE_Block.hh:530-535).  One workgroup.
+// and resetStepSizeCounter (SWE_Block.hh:530-535); then every block's time,
+// the next iteration's LTS bookkeeping and, with the fused strategy, its
+// queue.  One workgroup.
 __global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlockDesc* D, BatchBlockState* st,
-                                                                  BatchGlobals* G, float* dtLog, int* iterLog,
-                                                                  int* hostStat) {
+                                                                  BatchGlobals* G, int* items, float* dtLog,
+                                                                  int* iterLog, int* hostStat) {
     if (G->skip) return;
     const int mode = G->mode;
     const float T = G->T;
@@ -301,6 +351,34 @@ __global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlo
             s.stepSize = 0;
             s.timestepCounter += 1;
         }
+        if (mode == BATCH_LTS) s.total = lts_total(T, s);
+    }
+    if (mode == BATCH_LTS) {
+        __syncthreads();  // every block's total is written
+        const float duration = G->duration;
+        for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) lts_transitions(D[blk], st[blk], st, duration);
+    }
+    if (fused) {
+        __syncthreads();
+        next_queue(D, st, G, items);
+    }
+}
+
+// ---- store strategy: the next queue, after the apply launch ------------------
+__global__ __launch_bounds__(kCtlThreads) void batch_fill_kernel(const BatchBlockDesc* D, const BatchBlockState* st,
+                                                                 BatchGlobals* G, int* items) {
+    if (G->skip) return;
+    next_queue(D, st, G, items);
+}
+
+// ---- after an upload: the first iteration's bookkeeping ----------------------
+__global__ __launch_bounds__(kCtlThreads) void batch_prepare_kernel(const BatchBlockDesc* D, BatchBlockState* st,
+                                                                    int nblocks, int mode, float duration) {
+    for (int blk = threadIdx.x; blk < nblocks; blk += blockDim.x) {
+        if (mode == BATCH_LTS)
+            lts_transitions(D[blk], st[blk], st, duration);
+        else
+            st[blk].active = 1;
     }
 }
 
@@ -389,16 +467,23 @@ __global__ void batch_flags_kernel(BatchBlockState* st, int nblocks) {
     }
 }
 
-// start of a run: `target` more units
-__global__ void batch_begin_kernel(BatchGlobals* G, int target, int logCap, int* hostStat) {
-    G->target = target;
-    G->done = 0;
-    G->stop = target <= 0 ? 1 : 0;
-    G->itersUnit = 0;
-    G->logCap = logCap;
-    hostStat[0] = G->stop;
-    hostStat[1] = 0;
-    hostStat[2] = 0;
+// start of a run: `target` more units, the queue of the first iteration (the
+// tiling may have changed since the last run).  One workgroup.
+__global__ __launch_bounds__(kCtlThreads) void batch_begin_kernel(const BatchBlockDesc* D, const BatchBlockState* st,
+                                                                  BatchGlobals* G, int* items, int target, int logCap,
+                                                                  int* hostStat) {
+    if (threadIdx.x == 0) {
+        G->target = target;
+        G->done = 0;
+        G->stop = target <= 0 ? 1 : 0;
+        G->skip = G->stop;
+        G->itersUnit = 0;
+        G->logCap = logCap;
+        hostStat[0] = G->stop;
+        hostStat[1] = 0;
+        hostStat[2] = 0;
+    }
+    fill_queue(D, st, G, items, G->mode != BATCH_LTS);
 }
 
 template <int XS>
@@ -480,7 +565,9 @@ int tiles_for(swe_hip_batch* k, int xs, bool upload) {
     long long count = 0;
     for (auto& d : k->desc) {
         d.nyt = (d.ny + kStripRows - 1) / kStripRows;
-        d.ntiles = d.nyt * ((d.nx + xs - 1) / xs);
+        d.nxs = (d.nx + xs - 1) / xs;
+        d.pack = tail_pack(d.ny - kStripRows * (d.nyt - 1), d.nxs);
+        d.ntiles = (d.nyt - 1) * d.nxs + (d.pack == 1 ? d.nxs : 2 + (d.nxs - 2 + d.pack - 1) / d.pack);
         count += d.ntiles;
     }
     if (count > k->itemCap) {
@@ -504,37 +591,29 @@ int default_groups(swe_hip_batch* k) {
 
 // the launches of one driver iteration
 int enqueue_iteration(swe_hip_batch* k) {
-    const dim3 blk(kCtlThreads);
-    const dim3 sides(k->nblocks, 4, kSideChunks);
-    hipLaunchKernelGGL(batch_ghost_kernel, sides, blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
-    if (k->mode == BATCH_LTS)
-        hipLaunchKernelGGL(batch_check_sides_kernel, sides, blk, 0, k->stream, k->A, k->dDesc, k->dState, k->dG);
-    hipLaunchKernelGGL(batch_check_state_kernel, dim3(k->nblocks), blk, 0, k->stream, k->dDesc, k->dState, k->dG,
-                       k->dItems);
+    const dim3 ctl(kCtlThreads);
+    if (k->mode != BATCH_ENSEMBLE)
+        hipLaunchKernelGGL(batch_lines_kernel, dim3(k->nblocks, 4, kSideChunks), ctl, 0, k->stream, k->A, k->dDesc,
+                           k->dState, k->dG);
     const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
-    if (k->fused) {
-        int rc = launch_fused(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
-        if (rc) return rc;
-        hipLaunchKernelGGL(batch_round_kernel, dim3(1), blk, 0, k->stream, k->dDesc, k->dState, k->dG, k->dDtLog,
-                           k->dIterLog, k->dHostStat);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(e, "batch iteration launch");
-        return 0;
-    }
-    int rc = launch_sweep(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+    int rc = k->fused ? launch_fused(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream)
+                      : launch_sweep(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(batch_round_kernel, dim3(1), blk, 0, k->stream, k->dDesc, k->dState, k->dG, k->dDtLog,
-                       k->dIterLog, k->dHostStat);
-    switch (k->xs) {
-        case 4:
-            launch_apply<4>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
-            break;
-        case 16:
-            launch_apply<16>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
-            break;
-        default:
-            launch_apply<8>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
-            break;
+    hipLaunchKernelGGL(batch_round_kernel, dim3(1), ctl, 0, k->stream, k->dDesc, k->dState, k->dG, k->dItems,
+                       k->dDtLog, k->dIterLog, k->dHostStat);
+    if (!k->fused) {
+        switch (k->xs) {
+            case 4:
+                launch_apply<4>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+                break;
+            case 16:
+                launch_apply<16>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+                break;
+            default:
+                launch_apply<8>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+                break;
+        }
+        hipLaunchKernelGGL(batch_fill_kernel, dim3(1), ctl, 0, k->stream, k->dDesc, k->dState, k->dG, k->dItems);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch iteration launch");
@@ -600,7 +679,8 @@ int ensure_log(swe_hip_batch* k, int units) {
 
 // Advance until `units` are done; returns the iterations replayed.
 int advance(swe_hip_batch* k, int units, long long maxIter, long long* itersOut) {
-    hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(1), 0, k->stream, k->dG, units, k->logCap, k->dHostStat);
+    hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(kCtlThreads), 0, k->stream, k->dDesc, k->dState, k->dG,
+                       k->dItems, units, k->logCap, k->dHostStat);
     HIP_TRY(hipGetLastError());
     long long iters = 0;
     if (k->mode != BATCH_LTS) {
@@ -842,6 +922,8 @@ int swe_hip_batch_upload(swe_hip_batch* k, const float* h, const float* hu, cons
     g.fused = k->fused;
     HIP_TRY(hipMemcpyAsync(k->dG, &g, sizeof(g), hipMemcpyHostToDevice, k->stream));
     hipLaunchKernelGGL(batch_init_kernel, dim3(k->nblocks), dim3(kCtlThreads), 0, k->stream, k->A, k->dDesc);
+    hipLaunchKernelGGL(batch_prepare_kernel, dim3(1), dim3(kCtlThreads), 0, k->stream, k->dDesc, k->dState,
+                       k->nblocks, k->mode, k->duration);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(k->stream));
     k->uploaded = true;
@@ -1021,8 +1103,8 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
             rc = swe_hip_batch_set_tuning(k, xsC[a], gC[g] * k->cus, std::min(iterations, 256), -1, -1);
             if (rc) break;
             // warm-up chunk (also captures the graph), then the timed chunks
-            hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(1), 0, k->stream, k->dG, units, k->logCap,
-                               k->dHostStat);
+            hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(kCtlThreads), 0, k->stream, k->dDesc, k->dState,
+                               k->dG, k->dItems, units, k->logCap, k->dHostStat);
             rc = enqueue_chunk(k);
             if (rc) break;
             e = hipEventRecord(e0, k->stream);
