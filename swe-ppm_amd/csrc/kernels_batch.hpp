@@ -200,65 +200,9 @@ __device__ __forceinline__ int next_items(int* head, int* sBase) {
     return base;
 }
 
-// ---- tiles -----------------------------------------------------------------
-// A block's work items: first one item per (strip, column segment) for the
-// strips below the last one, then the last strip (the tail, R = ny - 62 *
-// (nyt - 1) rows).  When R is small the tail's inner column segments (full
-// XS columns, away from the left and right boundary columns) share waves:
-// `pack` segments per item, each on a group of R + 2 lanes (R rows with a
-// halo lane on either side) laid side by side in the wave.  Every lane
-// computes what the lane of an unpacked strip computes for its row and
-// column, so the packing does not change any result.
-struct TileLanes {
-    int yt;           // strip (uniform)
-    int xs0;          // first column segment of the item (uniform)
-    int xs;           // column segment of this lane's group
-    int r;            // row of the lane in the strip: 0 and R+1 are halo rows; 63 on idle lanes
-    bool first, last; // the item's segments are the block's first / last (uniform)
-};
-
-// Segments per packed tail item for a tail of R rows over nxs segments; 1 =
-// no packing.
-inline int tail_pack(int R, int nxs) {
-    const int p = 64 / (R + 2);
-    return (p >= 2 && nxs >= 4) ? p : 1;
-}
-
+// ---- tiles of a block (strip_tiling / tile_lanes) ---------------------------
 __device__ __forceinline__ TileLanes tile_lanes(const BatchBlockDesc& d, int tile, int lane) {
-    TileLanes t;
-    const int body = (d.nyt - 1) * d.nxs;
-    int lanesPerGroup = 64, groups = 1;
-    if (tile < body) {
-        t.yt = tile % (d.nyt - 1);
-        t.xs0 = tile / (d.nyt - 1);
-    } else {
-        const int u = tile - body;
-        t.yt = d.nyt - 1;
-        if (d.pack == 1) {
-            t.xs0 = u;
-        } else if (u < 2) {
-            t.xs0 = u ? d.nxs - 1 : 0;
-        } else {
-            t.xs0 = 1 + (u - 2) * d.pack;
-            groups = min(d.pack, d.nxs - 1 - t.xs0);
-            lanesPerGroup = d.ny - kStripRows * t.yt + 2;
-        }
-    }
-    const int g = lane / lanesPerGroup;
-    t.xs = t.xs0 + g;
-    t.r = lane - g * lanesPerGroup;
-    if (g >= groups) {
-        t.xs = t.xs0;
-        t.r = 63;
-    }
-    t.first = t.xs0 == 0;
-    t.last = t.xs0 == d.nxs - 1;
-    return t;
-}
-
-// columns of every segment of the item: XS, fewer for the last
-__device__ __forceinline__ int tile_columns(const TileLanes& t, int nx, int xs) {
-    return min(xs, nx - t.xs0 * xs);
+    return tile_lanes(d.nyt, d.nxs, d.pack, d.ny, tile, lane);
 }
 
 // ---- sweep: net-update sums of one strip tile ------------------------------
@@ -274,9 +218,10 @@ __device__ __forceinline__ float strip_sums(const dev::SolverCtx<S>& solver, con
     const bool owned = (t.r >= 1) && (t.r <= kStripRows) && (y <= v.ny);
     const bool rowInterior = (y >= 1) && (y <= v.ny);
     const bool hEdgeValid = (t.r <= kStripRows) && (y <= v.ny);
-    const int x0 = 1 + t.xs * XS;
+    const int x0 = 1 + t.xs0 * XS;  // the item's first column; packed groups add a per-lane offset
     const int n = tile_columns(t, v.nx, XS);
-    size_t o = idx(v, x0 - 1, yc);
+    const size_t colOff = (size_t)(t.xs - t.xs0) * XS * v.pitch;
+    size_t o = idx(v, x0 - 1, yc) + colOff;
     CellQ cq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
     o += v.pitch;
     CellQ nq = cell_q(v.h[o], v.hu[o], v.hv[o], v.b[o]);
@@ -295,7 +240,7 @@ __device__ __forceinline__ float strip_sums(const dev::SolverCtx<S>& solver, con
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);
         if (owned) {
-            const size_t c = idx(v, x, y);
+            const size_t c = idx(v, x, y) + colOff;
             sums.sx[c] = carryH + ev.hL;
             sums.su[c] = carryHu + ev.huL;
             sums.sy[c] = aH + eh.hL;
@@ -403,14 +348,21 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
     const bool rowBottom = (y == 0);
     const bool rowTop = (y == ny + 1);
     const bool carryRow = owned || (rowBottom && t.r == 0) || (rowTop && t.r >= 1);
-    const int x0 = 1 + t.xs * XS;
+    // the item's first column (uniform); a packed group's lanes address their
+    // own segment through a constant per-lane offset
+    const int x0 = 1 + t.xs0 * XS;
     const int n = tile_columns(t, nx, XS);
+    const size_t colOff = (size_t)(t.xs - t.xs0) * XS * v.pitch;
+    const size_t rowC = (size_t)yc + colOff, rowY = (size_t)y + colOff;
+    auto at = [&](int x, size_t row) { return (size_t)x * v.pitch + row; };
     const int tB = bnd[BND_BOTTOM], tT = bnd[BND_TOP], tL = bnd[BND_LEFT], tR = bnd[BND_RIGHT];
     auto ow = [](int t) { return t == OUTFLOW || t == WALL; };
 
     auto load_updated = [&](int x) -> BQ3 {
-        const size_t c = idx(v, x, yc);
+        const size_t c = at(x, rowC);
         BQ3 q{v.h[c], v.hu[c], v.hv[c]};
+        // x is the item's column: a packed item's segments are all inner, so
+        // the test has the same outcome for every lane
         if (pend && x >= 1 && x <= nx && rowInterior) {
             const Cell u = update_cell(q.h, q.hu, q.hv, dtdx, dtdy, sIn.sx[c], sIn.sy[c], sIn.su[c], sIn.sv[c]);
             q = BQ3{u.h, u.hu, u.hv};
@@ -434,7 +386,7 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
         return q;
     };
     auto store = [&](int x, const BQ3& q) {
-        const size_t c = idx(o, x, y);
+        const size_t c = at(x, rowY);
         o.h[c] = q.h;
         o.hu[c] = q.hu;
         o.hv[c] = q.hv;
@@ -444,8 +396,8 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
     BQ3 prev = load_updated(x0 - 1);
     if (t.first) prev = fix_column(prev, cur, tL);  // x0 == 1
     if (t.first && carryRow) store(0, prev);
-    CellQ pq = cell_q(prev.h, prev.hu, prev.hv, v.b[idx(v, x0 - 1, yc)]);
-    CellQ cq = cell_q(cur.h, cur.hu, cur.hv, v.b[idx(v, x0, yc)]);
+    CellQ pq = cell_q(prev.h, prev.hu, prev.hv, v.b[at(x0 - 1, rowC)]);
+    CellQ cq = cell_q(cur.h, cur.hu, cur.hv, v.b[at(x0, rowC)]);
     const dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
@@ -454,7 +406,7 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
         const bool rightGhost = t.last && k == n - 1;  // x + 1 == nx + 1
         BQ3 nxt = fix_rows(load_updated(x + 1));
         if (rightGhost) nxt = fix_column(nxt, cur, tR);
-        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, v.b[idx(v, x + 1, yc)]);
+        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, v.b[at(x + 1, rowC)]);
         if (carryRow) store(x, cur);
         if (rightGhost && carryRow) store(nx + 1, nxt);
         const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
@@ -464,7 +416,7 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);
         if (owned) {
-            const size_t c = idx(v, x, y);
+            const size_t c = at(x, rowY);
             sOut.sx[c] = carryH + ev.hL;
             sOut.su[c] = carryHu + ev.huL;
             sOut.sy[c] = aH + eh.hL;

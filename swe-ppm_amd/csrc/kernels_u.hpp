@@ -41,6 +41,79 @@ enum SweepMode : int { MODE_MAX = 0, MODE_STORE = 1, MODE_UPDATE = 2 };
 constexpr int kStripRows = 62;  // owned rows per wavefront
 constexpr int kWavesPerGroup = 4;
 
+// ---- strip tiles of the batched engine --------------------------------------
+// The strips of kStripRows rows below a block's last strip are cut into
+// column segments of XS columns, one wave per (strip, segment).  The last
+// strip (the tail, R = ny - kStripRows * (nyt - 1) rows) follows: when R is
+// small, its inner segments (full XS columns, away from the left and right
+// boundary columns) share waves, `pack` segments per wave, each on a group of
+// R + 2 lanes (R rows with a halo lane on either side) laid side by side.
+// Every lane computes what the lane of an unpacked strip computes for its row
+// and column, so the packing does not change any result.  (The single-block
+// kernels keep one wave per (strip, segment): on a large block the tail is a
+// small share of the waves, and its clamped lanes make those waves cheap.)
+struct StripTiling {
+    int nyt, nxs, pack, items;
+};
+
+__host__ __device__ inline StripTiling strip_tiling(int nx, int ny, int xs) {
+    StripTiling g;
+    g.nyt = (ny + kStripRows - 1) / kStripRows;
+    g.nxs = (nx + xs - 1) / xs;
+    const int perWave = 64 / (ny - kStripRows * (g.nyt - 1) + 2);
+    g.pack = (perWave >= 2 && g.nxs >= 4) ? perWave : 1;
+    g.items = (g.nyt - 1) * g.nxs + (g.pack == 1 ? g.nxs : 2 + (g.nxs - 2 + g.pack - 1) / g.pack);
+    return g;
+}
+
+struct TileLanes {
+    int yt;            // strip (uniform)
+    int xs0;           // first column segment of the item (uniform)
+    int xs;            // column segment of this lane's group
+    int r;             // row of the lane in the strip: 0 and R+1 are halo rows; 63 on idle lanes
+    bool first, last;  // the item's segments are the block's first / last (uniform)
+};
+
+// item `tile` of a block's strip tiling: order (strip, segment) with the
+// strip fastest below the tail, then the tail's first and last segments,
+// then its packed inner segments
+__device__ __forceinline__ TileLanes tile_lanes(int nyt, int nxs, int pack, int ny, int tile, int lane) {
+    TileLanes t;
+    const int body = (nyt - 1) * nxs;
+    int lanesPerGroup = 64, groups = 1;
+    if (tile < body) {
+        t.yt = tile % (nyt - 1);
+        t.xs0 = tile / (nyt - 1);
+    } else {
+        const int u = tile - body;
+        t.yt = nyt - 1;
+        if (pack == 1) {
+            t.xs0 = u;
+        } else if (u < 2) {
+            t.xs0 = u ? nxs - 1 : 0;
+        } else {
+            t.xs0 = 1 + (u - 2) * pack;
+            groups = min(pack, nxs - 1 - t.xs0);
+            lanesPerGroup = ny - kStripRows * t.yt + 2;
+        }
+    }
+    const int g = lane / lanesPerGroup;
+    t.xs = t.xs0 + g;
+    t.r = lane - g * lanesPerGroup;
+    if (g >= groups) {
+        t.xs = t.xs0;
+        t.r = 63;
+    }
+    t.first = t.xs0 == 0;
+    t.last = t.xs0 == nxs - 1;
+    return t;
+}
+
+// columns of every segment of the item: XS, fewer for the last
+__device__ __forceinline__ int tile_columns(const TileLanes& t, int nx, int xs) {
+    return min(xs, nx - t.xs0 * xs);
+}
+
 struct StateView {
     float* h;
     float* hu;

@@ -564,10 +564,11 @@ void free_graph(swe_hip_batch* k) {
 int tiles_for(swe_hip_batch* k, int xs, bool upload) {
     long long count = 0;
     for (auto& d : k->desc) {
-        d.nyt = (d.ny + kStripRows - 1) / kStripRows;
-        d.nxs = (d.nx + xs - 1) / xs;
-        d.pack = tail_pack(d.ny - kStripRows * (d.nyt - 1), d.nxs);
-        d.ntiles = (d.nyt - 1) * d.nxs + (d.pack == 1 ? d.nxs : 2 + (d.nxs - 2 + d.pack - 1) / d.pack);
+        const StripTiling g = strip_tiling(d.nx, d.ny, xs);
+        d.nyt = g.nyt;
+        d.nxs = g.nxs;
+        d.pack = g.pack;
+        d.ntiles = g.items;
         count += d.ntiles;
     }
     if (count > k->itemCap) {
