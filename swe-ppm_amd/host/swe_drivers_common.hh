@@ -32,6 +32,7 @@ struct DriverArgs {
     std::string dump;    // final states
     std::string dump0;   // initial states
     long max_steps = -1;  // stop after this many steps (global stepping)
+    std::string writer;   // --writer vtk|netcdf ("" = the driver's default)
 };
 
 inline int solver_from_name(const std::string& s) {
@@ -60,6 +61,7 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
                             {"dump", required_argument, nullptr, 5},
                             {"dump-initial", required_argument, nullptr, 6},
                             {"max-steps", required_argument, nullptr, 7},
+                            {"writer", required_argument, nullptr, 8},
                             {nullptr, 0, nullptr, 0}};
     int c;
     while ((c = getopt_long(argc, argv, "t:n:x:y:o:b:l:w:", opts, nullptr)) != -1) {
@@ -108,6 +110,10 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
                 break;
             case 7:
                 a.max_steps = std::atol(optarg);
+                break;
+            case 8:
+                a.writer = optarg;
+                if (a.writer != "vtk" && a.writer != "netcdf") throw std::invalid_argument("--writer: vtk or netcdf");
                 break;
             default:
                 throw std::invalid_argument("bad command line");

@@ -6,6 +6,13 @@
 //
 //   swe_overdecomp_hip -t 80 -n 20 -x 2048 -y 2048 -b 28 [-l 1] [--rows R]
 //                      [--solver augrie] [--input blocks.bin] [--dump out.bin]
+//                      [-w 1 -o base [--writer netcdf|vtk]]
+//
+// With -w 1 every block writes its state at t = 0 and at every checkpoint to
+// its own file <base>_<column><row>, netCDF by default, as the reference's
+// blocks do (SWE_DimensionalSplittingMPIOverdecomp.cpp:98-116,
+// swe_mpi_overdecomp.cpp:254-258, 348-352; file names of
+// tools/help.hh:100-105).
 //
 // Block k sits at column k / rows, row k % rows; neighbours follow
 // getNeighbours (swe_mpi_overdecomp.cpp:41-49).  Without --input the blocks
@@ -22,6 +29,8 @@
 #include "blocks/SWE_DimensionalSplittingHip.hh"
 #include "scenarios/SWE_simple_scenarios.hh"
 #include "swe_drivers_common.hh"
+#include "writer/NetCdfWriter.hh"
+#include "writer/VtkWriter.hh"
 
 namespace {
 
@@ -140,6 +149,27 @@ int main(int argc, char** argv) {
             write_snapshot(a.dump0, s0);
         }
 
+        std::vector<std::unique_ptr<Writer>> writers;
+        auto write_all = [&](float time) {
+            for (int k = 0; k < (int)writers.size(); k++)
+                writers[k]->writeTimeStep(blocks[k]->getWaterHeight(), blocks[k]->getMomentumHorizontal(),
+                                          blocks[k]->getMomentumVertical(), time);
+        };
+        if (a.write) {
+            for (int k = 0; k < total; k++) {
+                const int px = k / rows, py = k % rows;
+                const std::string name = a.out + "_" + std::to_string(px) + std::to_string(py);
+                SWE_DimensionalSplittingHip& b = *blocks[k];
+                if (a.writer == "vtk")
+                    writers.emplace_back(new VtkWriter(name, b.getBathymetry(), {{1, 1, 1, 1}}, b.nx, b.ny, b.dx,
+                                                       b.dy, (int)(b.originX / b.dx), (int)(b.originY / b.dy)));
+                else
+                    writers.emplace_back(new NetCdfWriter(name, b.getBathymetry(), {{1, 1, 1, 1}}, b.nx, b.ny, b.dx,
+                                                          b.dy, b.originX, b.originY));
+            }
+            write_all(0.f);
+        }
+
         float maxLocalTimestep = 0.f;
         if (lts) {
             // swe_mpi_overdecomp.cpp:235-245: the CFL estimate is computed and
@@ -198,6 +228,7 @@ int main(int argc, char** argv) {
                 }
             }
             std::printf("Write timestep (%fs)\n", t);
+            write_all(t);
         }
         long cellUpdates = 0;
         for (int k = 0; k < total; k++) cellUpdates += updates[k] * (long)blocks[k]->nx * blocks[k]->ny;

@@ -3,8 +3,13 @@
 // and the loop setGhostLayer -> computeNumericalFluxes -> getMaxTimestep ->
 // updateUnknowns -> t += dt, with the block's step on the GPU.
 //
-//   swe_simple_hip -t 25 -n 5 -x 256 -y 256 -o out [--solver fwave]
-//                  [--flavour u|s] [--scenario radial|centred|...] [--dump f]
+//   swe_simple_hip -t 25 -n 5 -x 256 -y 256 -o out [-w 1] [--writer vtk|netcdf]
+//                  [--solver fwave] [--flavour u|s] [--scenario radial|centred|...]
+//                  [--dump f]
+//
+// With -w 1 the state is written at t = 0 and at every checkpoint: VTK files
+// <out>.<k>.vts (the default) or one netCDF file <out>.nc, the two writers
+// the reference chooses between at build time (swe_simple.cpp:44-48, 172-185).
 #include <chrono>
 #include <cstdio>
 #include <memory>
@@ -12,6 +17,7 @@
 #include "blocks/SWE_DimensionalSplittingHip.hh"
 #include "scenarios/SWE_simple_scenarios.hh"
 #include "swe_drivers_common.hh"
+#include "writer/NetCdfWriter.hh"
 #include "writer/VtkWriter.hh"
 
 static std::unique_ptr<SWE_Scenario> make_scenario(const std::string& name) {
@@ -43,9 +49,12 @@ int main(int argc, char** argv) {
         sim.initScenario(*scenario, boundaries);
         if (!a.dump0.empty()) write_snapshot(a.dump0, {snapshot_of(sim)});
 
-        std::unique_ptr<VtkWriter> writer;
+        std::unique_ptr<Writer> writer;
         if (a.write) {
-            writer.reset(new VtkWriter(a.out, sim.getBathymetry(), {{1, 1, 1, 1}}, a.nx, a.ny, dx, dy, ox, oy));
+            if (a.writer == "netcdf")
+                writer.reset(new NetCdfWriter(a.out, sim.getBathymetry(), {{1, 1, 1, 1}}, a.nx, a.ny, dx, dy, ox, oy));
+            else
+                writer.reset(new VtkWriter(a.out, sim.getBathymetry(), {{1, 1, 1, 1}}, a.nx, a.ny, dx, dy, ox, oy));
             writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(), 0.f);
         }
 

@@ -105,3 +105,39 @@ def test_overdecomp_lts_matches_oracle(oracle, tmp_path):
     for k, (g, o) in enumerate(zip(s1, res)):
         for a, c in zip((g["h"], g["hu"], g["hv"]), o[:3]):
             assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), f"block {k}"
+
+
+def _assert_nc_matches(path, state, nrec, t_last):
+    """The last record of a driver's netCDF file holds the interior of the
+    state the driver dumped at the same instant, as (y, x) slabs."""
+    from scipy.io import netcdf_file
+
+    with netcdf_file(str(path), "r", mmap=False) as f:
+        v = f.variables
+        assert v["ah"].shape[0] == nrec
+        assert np.float32(v["time"].data[0]) == 0.0
+        assert abs(float(v["time"].data[-1]) - t_last) <= 1e-6 * max(1.0, t_last)  # printed with %f
+        for name, key in (("ah", "h"), ("hu", "hu"), ("hv", "hv")):
+            assert np.array_equal(v[name].data[-1], state[key][1:-1, 1:-1].T), name
+        assert np.array_equal(v["b"].data, state["b"][1:-1, 1:-1].T)
+
+
+def test_swe_simple_driver_writes_netcdf(tmp_path):
+    f1 = tmp_path / "final.bin"
+    out = run([BIN / "swe_simple_hip", "-t", "1000", "-n", "1", "-x", "48", "-y", "40", "--solver", "augrie",
+               "--scenario", "radial", "--max-steps", "25", "--dump", f1, "-w", "1", "--writer", "netcdf", "-o",
+               tmp_path / "run"])
+    t = float([ln for ln in out.splitlines() if ln.startswith("Write timestep")][-1].split("(")[1].rstrip("s)"))
+    _assert_nc_matches(tmp_path / "run.nc", snapshots.read(f1)[0], 2, t)
+
+
+def test_overdecomp_driver_writes_one_netcdf_file_per_block(tmp_path):
+    f1 = tmp_path / "final.bin"
+    out = run([BIN / "swe_overdecomp_hip", "-t", "1000", "-n", "1", "-x", "60", "-y", "40", "-b", "6", "--solver",
+               "augrie", "--max-steps", "12", "--dump", f1, "-w", "1", "-o", tmp_path / "od"])
+    t = float([ln for ln in out.splitlines() if ln.startswith("Write timestep")][-1].split("(")[1].rstrip("s)"))
+    s1 = snapshots.read(f1)
+    rows = 2
+    for k, s in enumerate(s1):
+        c, r = divmod(k, rows)
+        _assert_nc_matches(tmp_path / f"od_{c}{r}.nc", s, 2, t)
