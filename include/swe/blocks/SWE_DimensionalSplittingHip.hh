@@ -30,6 +30,7 @@
 
 #include "../../swe_hip.h"
 #include "blocks/SWE_Block.hh"
+#include "scenarios/SWE_NetCdfScenario.hh"
 #include "tools/Float2DNative.hh"
 
 // The synch* hooks are virtual in this library's SWE_Block; compiled against
@@ -59,6 +60,9 @@ public:
     // scenario / boundaries (host side of the base class, then upload)
     void initScenario(SWE_Scenario& scenario, BoundaryType boundaries[]) override;
     void setBoundaryType(Boundary boundary, BoundaryType type) override;
+    // initScenario of the file-driven scenario with the sampling done on the
+    // device (swe_hip_init_grid_scenario); the same arrays as initScenario
+    void initScenarioOnDevice(const SWE_NetCdfScenario& scenario, BoundaryType boundaries[]);
 
     // in-rank neighbours (SWE_DimensionalSplittingMPIOverdecomp.cpp:119-139)
     void connectLocalNeighbours(std::array<SWE_DimensionalSplittingHip*, 4> neighbourBlocks);

@@ -92,6 +92,32 @@ int swe_hip_apply_bc(swe_hip_block* blk);
 /* applyBoundaryBathymetry (SWE_Block.hh:724-749). */
 int swe_hip_apply_bc_bathymetry(swe_hip_block* blk);
 
+/* A gridded field with cell-centred values: value (i, j) sits at
+ * (x0 + i dx, y0 + j dy) and covers the cell [x0 + (i - 1/2) dx,
+ * x0 + (i + 1/2) dx) x [...]; `values` holds ny rows of nx floats (x fastest,
+ * the (y, x) order of a netCDF variable z(y, x)), in host memory. */
+typedef struct swe_hip_grid {
+    int nx, ny;
+    double x0, y0;
+    double dx, dy;
+    const float* values;
+} swe_hip_grid;
+
+/* initScenario (SWE_Block.hh:650-682) of the file-driven scenario
+ * (SWE_AsagiScenario.hh:42-151), sampled on the device: every cell, ghosts
+ * included, at x = (float)(ox + (i - 1/2) dx), y = (float)(oy + (j - 1/2) dy)
+ * in double, gets
+ *   bathy = the bathymetry grid's value of the cell containing (x, y)
+ *           (indices from floor((x - (x0 - dx/2)) / dx) in double, clamped to
+ *           the grid),
+ *   b     = bathy + (displacement value at (x, y) if (x, y) lies strictly
+ *           inside the displacement grid's float extent, else 0),
+ *   h     = bathy > 0 ? 0 : -bathy,  hu = 0 * h,  hv = 0 * h;
+ * then applyBoundaryConditions and applyBoundaryBathymetry with the block's
+ * boundary types.  displacement may be NULL.  No counterpart entry in the
+ * reference: it replaces the host loop of initScenario for this scenario. */
+int swe_hip_init_grid_scenario(swe_hip_block* blk, const swe_hip_grid* bathymetry, const swe_hip_grid* displacement);
+
 /* computeNumericalFluxes (flavour U: SWE_DimensionalSplittingMpi.cpp:383-456;
  * flavour S: SWE_DimensionalSplitting.cpp:105-179).  Solves every edge,
  * keeps the net updates on the device and returns the block's CFL time step

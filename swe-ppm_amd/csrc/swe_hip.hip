@@ -216,6 +216,65 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
     return SWE_HIP_OK;
 }
 
+// ---- file-driven scenario sampled on the device ------------------------------
+// A grid as the kernel sees it: the host description plus the device copy of
+// its values and the float extent the reference's strict range test uses
+// (SWE_AsagiScenario.hh:100-131).
+struct GridDev {
+    int nx, ny;
+    double x0, y0, dx, dy;
+    float rx0, rx1, ry0, ry1;
+    const float* v;
+};
+
+GridDev grid_dev(const swe_hip_grid& g, const float* dv) {
+    GridDev d;
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.x0 = g.x0;
+    d.y0 = g.y0;
+    d.dx = g.dx;
+    d.dy = g.dy;
+    d.rx0 = (float)(g.x0 - 0.5 * g.dx);
+    d.rx1 = (float)(g.x0 + (g.nx - 0.5) * g.dx);
+    d.ry0 = (float)(g.y0 - 0.5 * g.dy);
+    d.ry1 = (float)(g.y0 + (g.ny - 0.5) * g.dy);
+    d.v = dv;
+    return d;
+}
+
+// the value of the cell containing (x, y), indices clamped to the grid
+__device__ float grid_at(const GridDev& g, float x, float y) {
+    long i = (long)floor(((double)x - (g.x0 - 0.5 * g.dx)) / g.dx);
+    long j = (long)floor(((double)y - (g.y0 - 0.5 * g.dy)) / g.dy);
+    i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
+    j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
+    return g.v[(size_t)j * g.nx + i];
+}
+
+// one thread per cell, ghosts included; cell (i, j) at index i * pitch + j
+__global__ void grid_scenario_kernel(StateView v, float ox, float oy, GridDev bathy, GridDev displ, int hasDispl) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)(v.nx + 2) * v.pitch;
+    if (c >= total) return;
+    const int i = (int)(c / v.pitch), j = (int)(c % v.pitch);
+    // initScenario's sample point (SWE_Block.hh:669-670): float origin + (i - 0.5) * d in double, stored as float
+    const float x = (float)((double)ox + ((double)i - 0.5) * (double)v.dx);
+    const float y = (float)((double)oy + ((double)j - 0.5) * (double)v.dy);
+    const float bv = grid_at(bathy, x, y);
+    float dv = 0.f;
+    if (hasDispl && x > displ.rx0 && x < displ.rx1 && y > displ.ry0 && y < displ.ry1) dv = grid_at(displ, x, y);
+    const float h = (bv > 0.f) ? 0.f : -bv;
+    v.h[c] = h;
+    v.hu[c] = 0.f * h;
+    v.hv[c] = 0.f * h;
+    const_cast<float*>(v.b)[c] = bv + dv;
+}
+
+bool grid_ok(const swe_hip_grid* g) {
+    return g && g->values && g->nx >= 1 && g->ny >= 1 && g->dx > 0.0 && g->dy > 0.0;
+}
+
 }  // namespace
 
 namespace swe {
@@ -355,6 +414,37 @@ int swe_hip_apply_bc(swe_hip_block* k) {
 
 int swe_hip_apply_bc_bathymetry(swe_hip_block* k) {
     if (int r = check(k)) return r;
+    if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_init_grid_scenario(swe_hip_block* k, const swe_hip_grid* bathymetry, const swe_hip_grid* displacement) {
+    if (int r = check(k)) return r;
+    if (!grid_ok(bathymetry) || (displacement && !grid_ok(displacement))) {
+        set_error("swe_hip_init_grid_scenario: a grid needs values, nx, ny >= 1 and positive spacing");
+        return SWE_HIP_EINVAL;
+    }
+    const size_t nb = (size_t)bathymetry->nx * bathymetry->ny;
+    const size_t nd = displacement ? (size_t)displacement->nx * displacement->ny : 0;
+    float* dev = nullptr;
+    HIP_TRY(hipMalloc(&dev, sizeof(float) * (nb + nd + 1)));
+    hipError_t e = hipMemcpyAsync(dev, bathymetry->values, sizeof(float) * nb, hipMemcpyHostToDevice, k->stream);
+    if (e == hipSuccess && displacement)
+        e = hipMemcpyAsync(dev + nb, displacement->values, sizeof(float) * nd, hipMemcpyHostToDevice, k->stream);
+    if (e == hipSuccess) {
+        const GridDev gb = grid_dev(*bathymetry, dev);
+        const GridDev gd = displacement ? grid_dev(*displacement, dev + nb) : gb;
+        const long total = (long)field_elems(k);
+        hipLaunchKernelGGL(grid_scenario_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, k->stream,
+                           view(k, k->cur), k->ox, k->oy, gb, gd, displacement ? 1 : 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_fail(e, "swe_hip_init_grid_scenario");
+    // applyBoundaryConditions, then applyBoundaryBathymetry (SWE_Block.hh:679-681)
+    if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
     if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;
     HIP_TRY(hipStreamSynchronize(k->stream));
     return SWE_HIP_OK;

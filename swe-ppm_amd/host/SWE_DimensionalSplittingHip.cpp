@@ -39,6 +39,30 @@ void SWE_DimensionalSplittingHip::initScenario(SWE_Scenario& scenario, BoundaryT
     synchAfterWrite();
 }
 
+void SWE_DimensionalSplittingHip::initScenarioOnDevice(const SWE_NetCdfScenario& scenario, BoundaryType boundaries[]) {
+    for (int s = 0; s < 4; s++) setBoundaryType(static_cast<Boundary>(s), boundaries[s]);
+    auto grid = [](const GriddedField& g) {
+        swe_hip_grid d;
+        d.nx = g.nx;
+        d.ny = g.ny;
+        d.x0 = g.x0;
+        d.y0 = g.y0;
+        d.dx = g.dx;
+        d.dy = g.dy;
+        d.values = g.values.data();
+        return d;
+    };
+    const swe_hip_grid gb = grid(scenario.bathymetryGrid());
+    swe_hip_grid gd{};
+    const swe_hip_grid* pd = nullptr;
+    if (scenario.displacementGrid()) {
+        gd = grid(*scenario.displacementGrid());
+        pd = &gd;
+    }
+    check(swe_hip_init_grid_scenario(dev_, &gb, pd), "swe_hip_init_grid_scenario");
+    hostStale_ = true;  // the host arrays are filled on the next read
+}
+
 void SWE_DimensionalSplittingHip::setBoundaryType(Boundary boundary, BoundaryType type) {
     boundaryType[boundary] = type;
     // CONNECT and CONNECT_WITHIN_RANK sides are filled by neighbour exchange;

@@ -33,6 +33,9 @@ struct DriverArgs {
     std::string dump0;   // initial states
     long max_steps = -1;  // stop after this many steps (global stepping)
     std::string writer;   // --writer vtk|netcdf ("" = the driver's default)
+    std::string bathymetry;    // --bathymetry-file: file-driven scenario
+    std::string displacement;  // --displacement-file
+    bool hostInit = false;     // --host-init: sample the file-driven scenario on the host
 };
 
 inline int solver_from_name(const std::string& s) {
@@ -62,6 +65,9 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
                             {"dump-initial", required_argument, nullptr, 6},
                             {"max-steps", required_argument, nullptr, 7},
                             {"writer", required_argument, nullptr, 8},
+                            {"bathymetry-file", required_argument, nullptr, 9},
+                            {"displacement-file", required_argument, nullptr, 10},
+                            {"host-init", no_argument, nullptr, 11},
                             {nullptr, 0, nullptr, 0}};
     int c;
     while ((c = getopt_long(argc, argv, "t:n:x:y:o:b:l:w:", opts, nullptr)) != -1) {
@@ -114,6 +120,15 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
             case 8:
                 a.writer = optarg;
                 if (a.writer != "vtk" && a.writer != "netcdf") throw std::invalid_argument("--writer: vtk or netcdf");
+                break;
+            case 9:
+                a.bathymetry = optarg;
+                break;
+            case 10:
+                a.displacement = optarg;
+                break;
+            case 11:
+                a.hostInit = true;
                 break;
             default:
                 throw std::invalid_argument("bad command line");

@@ -5,7 +5,11 @@
 //
 //   swe_simple_hip -t 25 -n 5 -x 256 -y 256 -o out [-w 1] [--writer vtk|netcdf]
 //                  [--solver fwave] [--flavour u|s] [--scenario radial|centred|...]
-//                  [--dump f]
+//                  [-b bathymetry.nc [-d displacement.nc] [--host-init]] [--dump f]
+//
+// -b / -d (--bathymetry-file / --displacement-file) select the file-driven
+// scenario (the reference's ASAGI build, swe_simple.cpp:68-71, 109-113),
+// sampled on the device unless --host-init asks for the host loop.
 //
 // With -w 1 the state is written at t = 0 and at every checkpoint: VTK files
 // <out>.<k>.vts (the default) or one netCDF file <out>.nc, the two writers
@@ -32,8 +36,23 @@ static std::unique_ptr<SWE_Scenario> make_scenario(const std::string& name) {
 
 int main(int argc, char** argv) {
     try {
-        DriverArgs a = parse_args(argc, argv, "radial");
-        std::unique_ptr<SWE_Scenario> scenario = make_scenario(a.scenario);
+        // -b and -d are the reference's short flags for the input files here
+        std::vector<std::string> words(argv, argv + argc);
+        std::vector<char*> args;
+        for (auto& w : words) {
+            if (w == "-b") w = "--bathymetry-file";
+            if (w == "-d") w = "--displacement-file";
+            args.push_back(&w[0]);
+        }
+        DriverArgs a = parse_args((int)args.size(), args.data(), "radial");
+        std::unique_ptr<SWE_Scenario> scenario;
+        SWE_NetCdfScenario* fileScenario = nullptr;
+        if (!a.bathymetry.empty()) {
+            fileScenario = new SWE_NetCdfScenario(a.bathymetry, a.displacement);
+            scenario.reset(fileScenario);
+        } else {
+            scenario = make_scenario(a.scenario);
+        }
         std::vector<float> cp = checkpoint_instants(a.duration, a.checkpoints);
 
         const int width = (int)(scenario->getBoundaryPos(BND_RIGHT) - scenario->getBoundaryPos(BND_LEFT));
@@ -46,7 +65,10 @@ int main(int argc, char** argv) {
         for (int s = 0; s < 4; s++) boundaries[s] = scenario->getBoundaryType(static_cast<Boundary>(s));
 
         SWE_DimensionalSplittingHip sim(a.nx, a.ny, dx, dy, ox, oy, false, a.solver, a.flavour);
-        sim.initScenario(*scenario, boundaries);
+        if (fileScenario && !a.hostInit)
+            sim.initScenarioOnDevice(*fileScenario, boundaries);
+        else
+            sim.initScenario(*scenario, boundaries);
         if (!a.dump0.empty()) write_snapshot(a.dump0, {snapshot_of(sim)});
 
         std::unique_ptr<Writer> writer;

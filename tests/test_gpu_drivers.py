@@ -141,3 +141,48 @@ def test_overdecomp_driver_writes_one_netcdf_file_per_block(tmp_path):
     for k, s in enumerate(s1):
         c, r = divmod(k, rows)
         _assert_nc_matches(tmp_path / f"od_{c}{r}.nc", s, 2, t)
+
+
+def _tsunami_files(tmp_path):
+    """A coast (bathymetry -800 m .. +40 m) on a 40 x 20 grid of 100 m cells
+    and a 3 m sea-floor uplift on a 12 x 8 grid of 50 m cells."""
+    from scipy.io import netcdf_file
+
+    def write(path, xs, ys, z):
+        with netcdf_file(str(path), "w", version=2) as f:
+            f.createDimension("y", len(ys))
+            f.createDimension("x", len(xs))
+            f.createVariable("x", "f4", ("x",))[:] = xs
+            f.createVariable("y", "f4", ("y",))[:] = ys
+            f.createVariable("z", "f4", ("y", "x"))[:] = z
+
+    xs, ys = 50.0 + 100.0 * np.arange(40), 50.0 + 100.0 * np.arange(20)
+    X, Y = np.meshgrid(xs, ys)
+    z = np.float32(-800.0) + np.float32(840.0) * (X / 4000.0) ** 3 + 20.0 * np.sin(Y / 300.0)
+    dxs, dys = 1025.0 + 50.0 * np.arange(12), 725.0 + 50.0 * np.arange(8)
+    DX, DY = np.meshgrid(dxs, dys)
+    dz = 3.0 * np.exp(-((DX - 1300.0) ** 2 + (DY - 900.0) ** 2) / (2 * 150.0 ** 2))
+    write(tmp_path / "bathy.nc", xs, ys, z.astype(np.float32))
+    write(tmp_path / "displ.nc", dxs, dys, dz.astype(np.float32))
+    return tmp_path / "bathy.nc", tmp_path / "displ.nc"
+
+
+def test_file_scenario_sampled_on_device_equals_host_loop(oracle, tmp_path):
+    """swe_hip_init_grid_scenario against SWE_Block::initScenario over the
+    same SWE_NetCdfScenario (ghosts and boundary conditions included), then
+    20 steps against the oracle."""
+    bathy, displ = _tsunami_files(tmp_path)
+    common = ["-t", "1000", "-n", "1", "-x", "60", "-y", "40", "--solver", "augrie", "-b", bathy, "-d", displ]
+    dev0, dev1, host0 = tmp_path / "dev0.bin", tmp_path / "dev1.bin", tmp_path / "host0.bin"
+    run([BIN / "swe_simple_hip"] + common + ["--max-steps", "20", "--dump-initial", dev0, "--dump", dev1])
+    run([BIN / "swe_simple_hip"] + common + ["--max-steps", "1", "--host-init", "--dump-initial", host0])
+    d0, h0 = snapshots.read(dev0)[0], snapshots.read(host0)[0]
+    for k in ("h", "hu", "hv", "b"):
+        assert np.array_equal(d0[k].view(np.uint32), h0[k].view(np.uint32)), k
+    assert (d0["h"] > 0).any() and (d0["h"] == 0).any()  # sea and dry land
+    assert not np.array_equal(d0["b"] + d0["h"], np.zeros_like(d0["b"]))
+    d1 = snapshots.read(dev1)[0]
+    oh, ohu, ohv, _ = oracle.run_u(d0["h"], d0["hu"], d0["hv"], d0["b"], np.float32(4000 / 60), np.float32(2000 / 40),
+                                   [0, 0, 0, 0], 2, 20)
+    for a, c in ((d1["h"], oh), (d1["hu"], ohu), (d1["hv"], ohv)):
+        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1])
