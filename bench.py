@@ -28,6 +28,9 @@ sys.path.insert(0, str(ROOT / "swe-ppm_amd"))
 
 METRIC = "MCell-updates/s, 2048² AugRie FP32 + % HBM peak; batched/surrogate vs plain"
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
+# one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs per CU
+VALU_WAVE_INSTR_PER_CU_CYCLE = 1.0
 BYTES_PER_CELL = 28.0  # read h, hu, hv, b + write h, hu, hv (FP32)
 
 
@@ -41,6 +44,18 @@ def measured_traffic(workload_file: str = "fused_u_augrie_2048_islands.md"):
     if not path.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", path.read_text(), re.M)
+    return int(m.group(1)) if m else None
+
+
+def measured_valu(workload_file: str = "fused_u_augrie_2048_islands.md"):
+    """VALU wave instructions per launch of the dominant kernel (rocprofv3
+    SQ_INSTS_VALU) from the committed counter summary, or None."""
+    import re
+
+    path = ROOT / "profiles" / workload_file
+    if not path.exists():
+        return None
+    m = re.search(r"^valu_wave_instructions_per_launch:\s*(\d+)", path.read_text(), re.M)
     return int(m.group(1)) if m else None
 
 
@@ -339,7 +354,8 @@ def main() -> int:
     pass2_ms = kms[2] / max(kl[2], 1)
     # kernel time per step: every launch of the loop (boundary, passes) / steps
     step_ms_k = (kms[0] + kms[1] + kms[2]) / args.steps
-    achieved = BYTES_PER_CELL * cells / (step_ms_k * 1e-3) / 1e9
+    # the dominant kernel's own average launch time (the fused launch for the fused strategy)
+    achieved = BYTES_PER_CELL * cells / (pass1_ms * 1e-3) / 1e9
     gh, ghu, ghv = blk.download()
     blk.close()
     if not np.all(np.isfinite(gh)):
@@ -351,6 +367,24 @@ def main() -> int:
         cpu = cpu_baseline(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, args.cpu_budget)
 
     traffic = measured_traffic() if (strategy == 2 and n == 2048 and args.math == 0) else None
+    valu = measured_valu() if (strategy == 2 and n == 2048 and args.math == 0 and xseg == 8) else None
+    cus = 256
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            cus = torch.cuda.get_device_properties(0).multi_processor_count
+    except Exception:
+        pass
+    valu_line = None
+    if valu:
+        rate = valu / (pass1_ms * 1e-3) / 1e9
+        peak = cus * VALU_WAVE_INSTR_PER_CU_CYCLE * CLOCK_GHZ
+        valu_line = {"achieved": round(rate, 1), "peak": round(peak, 1), "unit": "G VALU wave-instructions/s",
+                     "frac": round(rate / peak, 4), "instructions_per_launch": valu,
+                     "source": "rocprofv3 SQ_INSTS_VALU of fused_u_kernel on this workload "
+                               "(profiles/fused_u_augrie_2048_islands.md); peak = CUs x 4 SIMDs x 1/4 "
+                               "wave64 instruction per cycle x 2.4 GHz"}
     names = {0: ("sweep_u_kernel<AUGRIE,MODE_MAX>", "sweep_u_kernel<AUGRIE,MODE_UPDATE>"),
              1: ("sweep_u_kernel<AUGRIE,MODE_STORE>", "apply_u_kernel"),
              2: ("fused_u_kernel<AUGRIE> (first launch: sweep_u_kernel<AUGRIE,MODE_STORE>)",
@@ -391,6 +425,7 @@ def main() -> int:
                       f"avg {pass2_ms * 1e3:.2f} us; boundary {kms[0] * 1e3:.1f} us total; "
                       f"{step_ms_k * 1e3:.2f} us of kernel time per step",
             "algorithmic_bytes_per_cell_update": BYTES_PER_CELL,
+            "valu": valu_line,
         },
         "cpu_baseline": cpu,
     }
