@@ -6,7 +6,13 @@
 //
 //   swe_overdecomp_hip -t 80 -n 20 -x 2048 -y 2048 -b 28 [-l 1] [--rows R]
 //                      [--solver augrie] [--input blocks.bin] [--dump out.bin]
-//                      [-w 1 -o base [--writer netcdf|vtk]]
+//                      [-w 1 -o base [--writer netcdf|vtk]] [--batched]
+//
+// --batched hands the whole loop to the batched engine (swe_hip_batch.h):
+// the blocks, scenario, checkpoints and output stay as below, but ghost
+// exchange, the LTS state machine, dt and the updates of every block run on
+// the device, one call per checkpoint interval (LTS) or per step (global dt).
+// The results are bit-identical to the block-by-block loop.
 //
 // With -w 1 every block writes its state at t = 0 and at every checkpoint to
 // its own file <base>_<column><row>, netCDF by default, as the reference's
@@ -26,6 +32,7 @@
 #include <cstdio>
 #include <memory>
 
+#include "../../include/swe_hip_batch.h"
 #include "blocks/SWE_DimensionalSplittingHip.hh"
 #include "scenarios/SWE_simple_scenarios.hh"
 #include "swe_drivers_common.hh"
@@ -57,16 +64,111 @@ struct SnapshotScenarioLoader {
     }
 };
 
+void batch_check(int rc, const char* what) {
+    if (rc != SWE_HIP_OK) throw std::runtime_error(std::string(what) + ": " + swe_hip_last_error());
+}
+
+// The driver loop below, run by the batched engine.  Host arrays of the
+// blocks are the upload source and the download target (writers and dumps
+// read them); the blocks' own device state is not used.
+void run_batched(std::vector<std::unique_ptr<SWE_DimensionalSplittingHip>>& blocks, const std::vector<int>& bnd,
+                 const std::vector<int>& nbr, const DriverArgs& a, bool lts, const std::vector<float>& cp,
+                 std::vector<std::unique_ptr<Writer>>& writers, std::vector<long>& updates, float& t,
+                 long& iterations, long& macro, double& wall) {
+    const int n = (int)blocks.size();
+    std::vector<int> nx(n), ny(n);
+    std::vector<float> dx(n), dy(n);
+    std::vector<size_t> off(n + 1, 0);
+    for (int k = 0; k < n; k++) {
+        blocks[k]->synchBeforeRead();  // host arrays current; nothing marks them stale again
+        nx[k] = blocks[k]->nx;
+        ny[k] = blocks[k]->ny;
+        dx[k] = blocks[k]->dx;
+        dy[k] = blocks[k]->dy;
+        off[k + 1] = off[k] + (size_t)(nx[k] + 2) * (ny[k] + 2);
+    }
+    std::vector<float> f[4];
+    for (auto& v : f) v.resize(off[n]);
+    auto gather = [&]() {
+        for (int k = 0; k < n; k++) {
+            const float* p[4] = {blocks[k]->h.getRawPointer(), blocks[k]->hu.getRawPointer(),
+                                 blocks[k]->hv.getRawPointer(), blocks[k]->b.getRawPointer()};
+            for (int q = 0; q < 4; q++) std::copy(p[q], p[q] + (off[k + 1] - off[k]), f[q].begin() + off[k]);
+        }
+    };
+    auto scatter = [&]() {
+        for (int k = 0; k < n; k++) {
+            float* p[3] = {blocks[k]->h.getRawPointer(), blocks[k]->hu.getRawPointer(), blocks[k]->hv.getRawPointer()};
+            for (int q = 0; q < 3; q++) std::copy(f[q].begin() + off[k], f[q].begin() + off[k + 1], p[q]);
+        }
+    };
+    swe_hip_batch* bt = nullptr;
+    batch_check(swe_hip_batch_create(n, nx.data(), ny.data(), dx.data(), dy.data(), bnd.data(), nbr.data(), a.solver,
+                                      lts ? SWE_BATCH_LTS : SWE_BATCH_GLOBAL, lts ? a.lts : 1.f, a.duration, &bt),
+                "swe_hip_batch_create");
+    std::unique_ptr<swe_hip_batch, int (*)(swe_hip_batch*)> guard(bt, swe_hip_batch_destroy);
+    gather();
+    batch_check(swe_hip_batch_upload(bt, f[0].data(), f[1].data(), f[2].data(), f[3].data()), "swe_hip_batch_upload");
+    bool stop = false;
+    for (size_t i = 0; i < cp.size() && !stop; i++) {
+        // the units this checkpoint interval needs, with the loop's float time
+        auto t0 = std::chrono::steady_clock::now();
+        if (lts) {
+            int units = 0;
+            float tt = t;
+            while (tt < cp[i] && !(a.max_steps > 0 && macro + units >= a.max_steps)) {
+                tt += a.lts;
+                units++;
+            }
+            if (a.max_steps > 0 && macro + units >= a.max_steps) stop = true;
+            if (units > 0) {
+                std::vector<int> its(units);
+                batch_check(swe_hip_batch_run(bt, units, nullptr, its.data(), nullptr), "swe_hip_batch_run");
+                for (int v : its) iterations += v;
+            }
+            t = tt;
+            macro += units;
+        } else {
+            while (t < cp[i]) {
+                float dt = 0.f;
+                batch_check(swe_hip_batch_run(bt, 1, &dt, nullptr, nullptr), "swe_hip_batch_run");
+                t += dt;
+                iterations++;
+                macro++;
+                if (a.max_steps > 0 && macro >= a.max_steps) {
+                    stop = true;
+                    break;
+                }
+            }
+        }
+        wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        batch_check(swe_hip_batch_download(bt, f[0].data(), f[1].data(), f[2].data(), nullptr),
+                    "swe_hip_batch_download");
+        scatter();
+        std::printf("Write timestep (%fs)\n", t);
+        for (int k = 0; k < (int)writers.size(); k++)
+            writers[k]->writeTimeStep(blocks[k]->h, blocks[k]->hu, blocks[k]->hv, t);
+    }
+    std::vector<int64_t> upd(n);
+    batch_check(swe_hip_batch_stats(bt, upd.data(), nullptr), "swe_hip_batch_stats");
+    for (int k = 0; k < n; k++) updates[k] = (long)upd[k];
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     try {
-        // --rows is read here (not a reference flag)
+        // --rows and --batched are read here (not reference flags)
         int rowsArg = -1;
+        bool batched = false;
         std::vector<char*> rest;
         for (int i = 0; i < argc; i++) {
             if (std::string(argv[i]) == "--rows" && i + 1 < argc) {
                 rowsArg = std::atoi(argv[++i]);
+                continue;
+            }
+            if (std::string(argv[i]) == "--batched") {
+                batched = true;
                 continue;
             }
             rest.push_back(argv[i]);
@@ -121,6 +223,7 @@ int main(int argc, char** argv) {
             }
             blocks[k].reset(new SWE_DimensionalSplittingHip(nxl, nyl, dx, dy, ox, oy, lts, a.solver, a.flavour));
         }
+        std::vector<int> allBnd(4 * total), allNbr(4 * total);
         for (int k = 0; k < total; k++) {
             const int px = k / rows, py = k % rows;
             std::array<int, 4> nb = get_neighbours(px, py, cols, rows, k);
@@ -133,6 +236,10 @@ int main(int argc, char** argv) {
                 } else {
                     bnd[s] = scenario->getBoundaryType(static_cast<Boundary>(s));
                 }
+            }
+            for (int s = 0; s < 4; s++) {
+                allBnd[4 * k + s] = bnd[s];
+                allNbr[4 * k + s] = nb[s];
             }
             if (init.empty())
                 blocks[k]->initScenario(*scenario, bnd);
@@ -184,6 +291,11 @@ int main(int argc, char** argv) {
         long iterations = 0, macro = 0;
         double wall = 0.0;
         bool stop = false;
+        if (batched) {
+            if (a.flavour != SWE_FLAVOUR_U) throw std::invalid_argument("--batched runs flavour U");
+            run_batched(blocks, allBnd, allNbr, a, lts, cp, writers, updates, t, iterations, macro, wall);
+            stop = true;
+        }
         for (int i = 0; i < a.checkpoints && !stop; i++) {
             while (t < cp[i]) {
                 auto t0 = std::chrono::steady_clock::now();
