@@ -146,3 +146,74 @@ def test_batch_rejects_inconsistent_neighbours(swe):
             bt.run(1)  # nothing uploaded yet
         with pytest.raises(swe.SweHipError):
             bt.set_tuning(xs=5)
+
+
+EDGE_SIZES = [(1, 1), (1, 70), (70, 1), (200, 1), (200, 2), (33, 63), (64, 124), (9, 126), (300, 30), (37, 62),
+              (8, 64), (100, 125)]
+
+
+def _edge_instance(rng, nx, ny, bnd):
+    """A wet state with dry patches, uneven bed and moving water."""
+    h = rng.uniform(0.5, 6.0, size=(nx + 2, ny + 2)).astype(np.float32)
+    h[rng.uniform(size=h.shape) < 0.1] = 0.0
+    b = rng.uniform(-1.0, 1.0, size=h.shape).astype(np.float32)
+    hu = (h * rng.uniform(-1.0, 1.0, size=h.shape)).astype(np.float32)
+    hv = (h * rng.uniform(-1.0, 1.0, size=h.shape)).astype(np.float32)
+    return dict(nx=nx, ny=ny, dx=float(np.float32(10.0)), dy=float(np.float32(7.5)), bnd=bnd, h=h, hu=hu, hv=hv, b=b)
+
+
+@pytest.mark.parametrize("xs", [4, 8, 16])
+def test_ensemble_edge_shapes_match_oracle(swe, oracle, xs):
+    """Single-row and single-column blocks, heights around the 62-row strip
+    (tails of 1..62 rows, packed several segments to a wave), widths that
+    are not a multiple of xs, WALL sides, dry cells."""
+    rng = np.random.default_rng(17)
+    inst = []
+    for k, (nx, ny) in enumerate(EDGE_SIZES):
+        bnd = [int(v) for v in rng.integers(0, 2, size=4)]  # OUTFLOW or WALL per side
+        inst.append(_edge_instance(rng, nx, ny, bnd))
+    nsteps = 12
+    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+        bt.set_tuning(xs=xs, chunk=5)
+        bt.upload(*[[i[f] for i in inst] for f in ("h", "hu", "hv", "b")])
+        bt.run(nsteps)
+        got = bt.download()
+    want = [oracle.run_u(i["h"], i["hu"], i["hv"], i["b"], np.float32(i["dx"]), np.float32(i["dy"]), i["bnd"],
+                         swe.AUGRIE, nsteps)[:3] for i in inst]
+    _assert_blocks_equal(oracle, got, want, inst, f"edge shapes xs={xs}")
+
+
+@pytest.mark.parametrize("lts", [False, True])
+def test_connected_edge_shapes_match_oracle(swe, oracle, lts):
+    """3 x 3 connected blocks with column widths (1, 45, 200) and row heights
+    (63, 1, 124): thin blocks between wide ones, global dt and LTS."""
+    widths, heights = [1, 45, 200], [63, 1, 124]
+    rng = np.random.default_rng(23)
+    rows, cols = len(heights), len(widths)
+    blocks = []
+    for c in range(cols):
+        for r in range(rows):
+            k = c * rows + r
+            nbr = [k - rows if c > 0 else -1, k + rows if c < cols - 1 else -1, k - 1 if r > 0 else -1,
+                   k + 1 if r < rows - 1 else -1]
+            bnd = [4 if m >= 0 else 0 for m in nbr]
+            nx, ny = widths[c], heights[r]
+            depth = np.float32(rng.uniform(2.0, 20.0))
+            h = np.full((nx + 2, ny + 2), depth, np.float32)
+            h += rng.uniform(0.0, 0.5, size=h.shape).astype(np.float32)
+            z = np.zeros_like(h)
+            blocks.append(dict(nx=nx, ny=ny, bnd=bnd, nbr=nbr, h=h, hu=z.copy(), hv=z.copy(), b=z.copy()))
+    dx = dy = 5.0
+    T, duration, units = 0.5, 2.0, 3
+    mode = swe.BATCH_LTS if lts else swe.BATCH_GLOBAL
+    with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=mode, dx=dx, dy=dy, max_local=T, duration=duration) as bt:
+        bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
+        _, its, _ = bt.run(units if lts else 20)
+        got = bt.download()
+        upd, _ = bt.stats()
+    states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, lts, T, duration if lts else 0.0,
+                                                  units if lts else 20, nthreads=8)
+    assert np.array_equal(upd, oupd)
+    if lts:
+        assert np.array_equal(its, oits)
+    _assert_blocks_equal(oracle, got, states, blocks, f"connected edge shapes lts={lts}")
