@@ -302,8 +302,8 @@ def main() -> int:
                           "launches": 3}))
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the step is measured on one GPU; sharding across GPUs is not implemented)", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus} is not supported: the step runs and is measured on one GPU "
+              "(sharding blocks across GPUs is not implemented)", file=sys.stderr)
         return 2
 
     import swe_ppm_amd as swe
