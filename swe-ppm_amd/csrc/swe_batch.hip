@@ -172,14 +172,15 @@ __global__ __launch_bounds__(kCtlThreads) void batch_lines_kernel(BatchArena A, 
 // ghost-layer states, notifiedLastTimestep, the sub-step counter and whether
 // the block is in sync (SWE_Block.hh:375-436).  The lines launch moves the
 // data afterwards.
-__device__ void lts_transitions(const BatchBlockDesc& d, BatchBlockState& s, const BatchBlockState* st,
+template <typename Desc, typename NeighbourTotal>
+__device__ void lts_transitions(const Desc& d, BatchBlockState& s, const NeighbourTotal& neighbourTotal,
                                 float duration) {
     const float tot = s.total;
     int recv = 0;
     for (int side = 0; side < 4; ++side) {
         if (d.bnd[side] != CONNECT_WITHIN_RANK || !lts_receivable(s, side)) continue;
         recv |= 1 << side;
-        s.borderTs[side] = st[d.nbr[side]].total;
+        s.borderTs[side] = neighbourTotal(d.nbr[side]);
     }
     s.recv = recv;
     for (int side = 0; side < 4; ++side) {
@@ -203,7 +204,7 @@ __device__ void lts_transitions(const BatchBlockDesc& d, BatchBlockState& s, con
 
 // exclusive prefix sum over the workgroup (all threads, full waves)
 __device__ int wg_exclusive_scan(int v, int& total) {
-    __shared__ int sWave[kCtlThreads / 64];
+    __shared__ int sWave[16];  // up to 1024 threads
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = v;
     for (int o = 1; o < 64; o <<= 1) {
@@ -275,26 +276,60 @@ __device__ void next_queue(const BatchBlockDesc* D, const BatchBlockState* st, B
 // (hasMaxLocalTimestep, SWE_Block.hh:355-358, swe_mpi_overdecomp.cpp:327-342)
 // and resetStepSizeCounter (SWE_Block.hh:530-535); then every block's time,
 // the next iteration's LTS bookkeeping and, with the fused strategy, its
-// queue.  One workgroup.
-__global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlockDesc* D, BatchBlockState* st,
-                                                                  BatchGlobals* G, int* items, float* dtLog,
-                                                                  int* iterLog, int* hostStat) {
+// queue.  One workgroup of kRoundThreads; each thread keeps the state of its
+// blocks (blk = thread + r * kRoundThreads) in registers from one load to one
+// store, and what crosses blocks (neighbour times, queue offsets) goes
+// through LDS: the kernel is a short chain of dependent accesses.
+constexpr int kRoundThreads = 1024;
+constexpr int kRoundPer = (kMaxBatchBlocks + kRoundThreads - 1) / kRoundThreads;
+
+struct RoundDesc {
+    float dx, dy;
+    int bnd[4], nbr[4];
+    int ntiles;
+};
+
+__global__ __launch_bounds__(kRoundThreads) void batch_round_kernel(const BatchBlockDesc* D, BatchBlockState* st,
+                                                                    BatchGlobals* G, int* items, float* dtLog,
+                                                                    int* iterLog, int* hostStat) {
     if (G->skip) return;
     const int mode = G->mode;
     const float T = G->T;
     const int nb = G->nblocks;
+    const bool fused = G->fused != 0;
+    const float duration = G->duration;
     __shared__ unsigned int sminBits;
     __shared__ int allSynced, macroEnd;
+    __shared__ float sTotal[kMaxBatchBlocks];
+    __shared__ int sBase[kMaxBatchBlocks];
+    __shared__ int sTiles[kMaxBatchBlocks];
+    BatchBlockState S[kRoundPer];
+    RoundDesc R[kRoundPer];
+#pragma unroll
+    for (int r = 0; r < kRoundPer; ++r) {
+        const int blk = threadIdx.x + r * kRoundThreads;
+        if (blk >= nb) continue;
+        S[r] = st[blk];
+        const BatchBlockDesc& d = D[blk];
+        R[r].dx = d.dx;
+        R[r].dy = d.dy;
+        for (int k = 0; k < 4; ++k) {
+            R[r].bnd[k] = d.bnd[k];
+            R[r].nbr[k] = d.nbr[k];
+        }
+        R[r].ntiles = d.ntiles;
+    }
     if (threadIdx.x == 0) {
         sminBits = __float_as_uint(FLT_MAX);
         allSynced = 1;
     }
     __syncthreads();
-    for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) {
-        BatchBlockState& s = st[blk];
+#pragma unroll
+    for (int r = 0; r < kRoundPer; ++r) {
+        if (threadIdx.x + r * kRoundThreads >= nb) continue;
+        BatchBlockState& s = S[r];
         if (s.active) {
-            const BatchBlockDesc& d = D[blk];
-            float dt = timestep_from_max(__uint_as_float(s.smax), d.dx, d.dy);
+            float dt = timestep_from_max(__uint_as_float(s.smax), R[r].dx, R[r].dy);
             if (mode == BATCH_LTS) {
                 if (s.stepSizeCounter <= 0) s.stepSize = lts_step_size(T, dt);
                 dt = T / s.stepSize;
@@ -332,10 +367,12 @@ __global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlo
         hostStat[2] = G->itersUnit;
     }
     __syncthreads();
-    const bool fused = G->fused != 0;
     const float gdt = __uint_as_float(sminBits);
-    for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) {
-        BatchBlockState& s = st[blk];
+#pragma unroll
+    for (int r = 0; r < kRoundPer; ++r) {
+        const int blk = threadIdx.x + r * kRoundThreads;
+        if (blk >= nb) continue;
+        BatchBlockState& s = S[r];
         if (s.active) {
             // every block of the global loop updates with the minimum (swe_mpi_overdecomp.cpp:305)
             if (mode == BATCH_GLOBAL) s.maxTimestep = gdt;
@@ -351,16 +388,59 @@ __global__ __launch_bounds__(kCtlThreads) void batch_round_kernel(const BatchBlo
             s.stepSize = 0;
             s.timestepCounter += 1;
         }
-        if (mode == BATCH_LTS) s.total = lts_total(T, s);
+        if (mode == BATCH_LTS) sTotal[blk] = s.total = lts_total(T, s);
     }
     if (mode == BATCH_LTS) {
-        __syncthreads();  // every block's total is written
-        const float duration = G->duration;
-        for (int blk = threadIdx.x; blk < nb; blk += blockDim.x) lts_transitions(D[blk], st[blk], st, duration);
+        __syncthreads();  // every block's total is in LDS
+#pragma unroll
+        for (int r = 0; r < kRoundPer; ++r)
+            if (threadIdx.x + r * kRoundThreads < nb)
+                lts_transitions(R[r], S[r], [&](int k) { return sTotal[k]; }, duration);
     }
-    if (fused) {
-        __syncthreads();
-        next_queue(D, st, G, items);
+#pragma unroll
+    for (int r = 0; r < kRoundPer; ++r) {
+        const int blk = threadIdx.x + r * kRoundThreads;
+        if (blk < nb) st[blk] = S[r];
+    }
+    if (!fused) return;  // the store strategy's queue is built after its apply launch
+    if (mode != BATCH_LTS) {  // every block computes every iteration: the queue stays as filled
+        if (threadIdx.x == 0) {
+            G->count = G->allItems;
+            G->head = 0;
+            G->headApply = 0;
+            G->skip = G->stop;
+        }
+        return;
+    }
+    // the next queue: the tiles of the blocks in sync, block by block
+    int mine = 0;
+#pragma unroll
+    for (int r = 0; r < kRoundPer; ++r)
+        if (threadIdx.x + r * kRoundThreads < nb && S[r].active) mine += R[r].ntiles;
+    int total;
+    int run = wg_exclusive_scan(mine, total);
+#pragma unroll
+    for (int r = 0; r < kRoundPer; ++r) {
+        const int blk = threadIdx.x + r * kRoundThreads;
+        if (blk >= nb) continue;
+        const bool q = S[r].active != 0;
+        sBase[blk] = q ? run : -1;
+        sTiles[blk] = R[r].ntiles;
+        if (q) run += R[r].ntiles;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    for (int b = wave; b < nb; b += waves) {
+        const int base = sBase[b];
+        if (base < 0) continue;
+        const int nt = sTiles[b];
+        for (int i = lane; i < nt; i += 64) items[base + i] = (b << kTileBits) | i;
+    }
+    if (threadIdx.x == 0) {
+        G->count = total;
+        G->head = 0;
+        G->headApply = 0;
+        G->skip = G->stop;
     }
 }
 
@@ -376,7 +456,7 @@ __global__ __launch_bounds__(kCtlThreads) void batch_prepare_kernel(const BatchB
                                                                     int nblocks, int mode, float duration) {
     for (int blk = threadIdx.x; blk < nblocks; blk += blockDim.x) {
         if (mode == BATCH_LTS)
-            lts_transitions(D[blk], st[blk], st, duration);
+            lts_transitions(D[blk], st[blk], [&](int k) { return st[k].total; }, duration);
         else
             st[blk].active = 1;
     }
@@ -600,8 +680,8 @@ int enqueue_iteration(swe_hip_batch* k) {
     int rc = k->fused ? launch_fused(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream)
                       : launch_sweep(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(batch_round_kernel, dim3(1), ctl, 0, k->stream, k->dDesc, k->dState, k->dG, k->dItems,
-                       k->dDtLog, k->dIterLog, k->dHostStat);
+    hipLaunchKernelGGL(batch_round_kernel, dim3(1), dim3(kRoundThreads), 0, k->stream, k->dDesc, k->dState, k->dG,
+                       k->dItems, k->dDtLog, k->dIterLog, k->dHostStat);
     if (!k->fused) {
         switch (k->xs) {
             case 4:
