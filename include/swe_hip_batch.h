@@ -77,11 +77,14 @@ int swe_hip_batch_run(swe_hip_batch* bt, int units, float* dt_log, int* iter_log
 int swe_hip_batch_stats(swe_hip_batch* bt, int64_t* updates, float* t);
 
 /* Tuning: xs = strip columns per work item (4, 8 or 16), groups = workgroups
- * of the persistent sweep / apply grids (1..65535), chunk = driver iterations
- * per captured hipGraph (1..256), graph = 1 replays captured graphs, 0
- * launches the kernels one by one, fused = 1 applies each block's update on
- * its next load (one sweep per iteration, the default), 0 runs a separate
- * apply pass; -1 keeps a value.  Results do not depend on the tuning. */
+ * of the persistent sweep / apply grids (1..65535), or 0 for the direct form
+ * of the fused sweep (one wave per possible work item, no queue loop; the
+ * default; the store strategy's grids then take their default size), chunk = driver
+ * iterations per captured hipGraph (1..256), graph = 1 replays captured
+ * graphs, 0 launches the kernels one by one, fused = 1 applies each block's
+ * update on its next load (one sweep per iteration, the default), 0 runs a
+ * separate apply pass; -1 keeps a value.  Results do not depend on the
+ * tuning. */
 int swe_hip_batch_set_tuning(swe_hip_batch* bt, int xs, int groups, int chunk, int graph, int fused);
 int swe_hip_batch_get_tuning(swe_hip_batch* bt, int* xs, int* groups, int* chunk, int* graph, int* fused);
 
@@ -89,7 +92,7 @@ int swe_hip_batch_get_tuning(swe_hip_batch* bt, int* xs, int* groups, int* chunk
  * iterations of the current state, restores the state afterwards and keeps
  * the fastest.  cand_ms (may be NULL, else room for 16 floats) receives the
  * time per iteration of each candidate in the order xs in {4, 8, 16} x
- * groups in {2, 4, 8} x CUs; *ncand their number. */
+ * groups in {2 x CUs, 4 x CUs, 8 x CUs, 0 (direct)}; *ncand their number. */
 int swe_hip_batch_autotune(swe_hip_batch* bt, int iterations, float* cand_ms, int* ncand);
 
 #ifdef __cplusplus

@@ -19,10 +19,14 @@
 //           or, for LTS, into the side's receive line when the side receives
 //           this iteration, followed by checkAllGhostlayers' copy or
 //           interpolation (SWE_Block.hh:375-436).
-//   sweep : waves pull (block, tile) items from the queue and solve the
+//   sweep : waves take (block, tile) items from the queue and solve the
 //           tile's edges: per-cell net-update sums + per-block CFL maximum;
 //           with the fused strategy a block's pending update is applied as
-//           its cells are loaded.
+//           its cells are loaded.  By default the fused sweep's grid has one
+//           wave per possible item and a wave takes the item of its index
+//           (the queue is compact, so the waves past its count leave at
+//           once); a persistent grid that pulls items in a loop is the
+//           alternative (swe_hip_batch_set_tuning).
 //   round : one workgroup: per-block dt (CFL; LTS: getRoundTimestep; global:
 //           minimum over blocks), the driver's completion test
 //           (hasMaxLocalTimestep), resetStepSizeCounter at the end of a macro
@@ -36,9 +40,9 @@
 //           cells, writing the OUTFLOW / WALL ghosts that the next
 //           setGhostLayer would write; a one-workgroup fill launch then
 //           builds the next queue.
-// The sweep and apply grids are persistent (sized to the machine, not to the
-// work): each wave loops until the queue is drained.  No workgroup ever waits
-// for another inside a launch; the launch boundaries order the phases.
+// Persistent grids are sized to the machine, not to the work: each wave
+// loops until the queue is drained.  No workgroup ever waits for another
+// inside a launch; the launch boundaries order the phases.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -430,19 +434,18 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
     return wave_max(smaxv);
 }
 
+// direct = 0: persistent grid, waves pull items until the queue is drained;
+// direct = 1: one wave per possible item (the grid covers every block's
+// tiles), a wave takes item blockIdx.x * kWavesPerGroup + wave or leaves.
 template <int S, int XS>
 __global__ __launch_bounds__(256) void batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
-                                                          BatchGlobals* G, const int* items) {
+                                                          BatchGlobals* G, const int* items, int direct) {
     if (G->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int count = G->count;
     const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? G->mlp : nullptr);
-    __shared__ int sBase;
-    for (;;) {
-        const int item = next_items(&G->head, &sBase) + wave;
-        if (item - wave >= count) break;  // uniform over the workgroup
-        if (item >= count) continue;
+    auto run_item = [&](int item) {
         const int code = __builtin_amdgcn_readfirstlane(items[item]);
         const int blk = code >> kTileBits;
         const int tile = code & ((1 << kTileBits) - 1);
@@ -455,6 +458,18 @@ __global__ __launch_bounds__(256) void batch_fused_kernel(BatchArena A, const Ba
                                           batch_sums(A, d, sidx), batch_sums(A, d, sidx ^ 1), pend, dt, d.bnd,
                                           tile_lanes(d, tile, lane));
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
+    };
+    if (direct) {
+        const int item = blockIdx.x * kWavesPerGroup + wave;  // uniform per wave
+        if (item < count) run_item(item);
+        return;
+    }
+    __shared__ int sBase;
+    for (;;) {
+        const int item = next_items(&G->head, &sBase) + wave;
+        if (item - wave >= count) break;  // uniform over the workgroup
+        if (item >= count) continue;
+        run_item(item);
     }
 }
 

@@ -36,18 +36,18 @@ template int launch_batch_sweep_solver<SWE_SOLVER_ID>(int, int, BatchArena, cons
                                                       BatchGlobals*, const int*, hipStream_t);
 
 template <int S>
-int launch_batch_fused_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
-                              BatchGlobals* G, const int* items, hipStream_t stream) {
+int launch_batch_fused_solver(int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
+                              BatchBlockState* st, BatchGlobals* G, const int* items, hipStream_t stream) {
     const dim3 grid(groups), block(64 * kWavesPerGroup);
     switch (xs) {
         case 4:
-            hipLaunchKernelGGL((batch_fused_kernel<S, 4>), grid, block, 0, stream, A, D, st, G, items);
+            hipLaunchKernelGGL((batch_fused_kernel<S, 4>), grid, block, 0, stream, A, D, st, G, items, direct);
             break;
         case 16:
-            hipLaunchKernelGGL((batch_fused_kernel<S, 16>), grid, block, 0, stream, A, D, st, G, items);
+            hipLaunchKernelGGL((batch_fused_kernel<S, 16>), grid, block, 0, stream, A, D, st, G, items, direct);
             break;
         default:
-            hipLaunchKernelGGL((batch_fused_kernel<S, 8>), grid, block, 0, stream, A, D, st, G, items);
+            hipLaunchKernelGGL((batch_fused_kernel<S, 8>), grid, block, 0, stream, A, D, st, G, items, direct);
             break;
     }
     const hipError_t e = hipGetLastError();
@@ -55,7 +55,7 @@ int launch_batch_fused_solver(int xs, int groups, BatchArena A, const BatchBlock
     return 0;
 }
 
-template int launch_batch_fused_solver<SWE_SOLVER_ID>(int, int, BatchArena, const BatchBlockDesc*, BatchBlockState*,
-                                                      BatchGlobals*, const int*, hipStream_t);
+template int launch_batch_fused_solver<SWE_SOLVER_ID>(int, int, int, BatchArena, const BatchBlockDesc*,
+                                                      BatchBlockState*, BatchGlobals*, const int*, hipStream_t);
 
 }  // namespace swe

@@ -574,19 +574,19 @@ int launch_apply(int groups, BatchArena A, const BatchBlockDesc* D, const BatchB
     return 0;
 }
 
-int launch_fused(int solver, int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
-                 BatchGlobals* G, const int* items, hipStream_t stream) {
+int launch_fused(int solver, int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
+                 BatchBlockState* st, BatchGlobals* G, const int* items, hipStream_t stream) {
     switch (solver) {
         case SOLVER_HLLE:
-            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, direct, A, D, st, G, items, stream);
         case SOLVER_FWAVE:
-            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, direct, A, D, st, G, items, stream);
         case SOLVER_AUGRIE:
-            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, direct, A, D, st, G, items, stream);
         case SOLVER_AUGRIE_MLP:
-            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, direct, A, D, st, G, items, stream);
         default:
-            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, direct, A, D, st, G, items, stream);
     }
 }
 
@@ -626,6 +626,7 @@ struct swe_hip_batch {
     int* hostStat = nullptr;  // pinned, written by the round kernel
     int* dHostStat = nullptr;
     int xs = 8, groups = 0, chunk = 16, useGraph = 1;
+    long long itemsAll = 0;  // work items of all blocks at the current xs
     int cus = 256;
     hipStream_t stream = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -651,6 +652,7 @@ int tiles_for(swe_hip_batch* k, int xs, bool upload) {
         d.ntiles = g.items;
         count += d.ntiles;
     }
+    k->itemsAll = count;
     if (count > k->itemCap) {
         set_error("batch: %lld work items exceed the queue (%d)", count, k->itemCap);
         return SWE_HIP_EINVAL;
@@ -677,21 +679,27 @@ int enqueue_iteration(swe_hip_batch* k) {
         hipLaunchKernelGGL(batch_lines_kernel, dim3(k->nblocks, 4, kSideChunks), ctl, 0, k->stream, k->A, k->dDesc,
                            k->dState, k->dG);
     const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
-    int rc = k->fused ? launch_fused(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream)
-                      : launch_sweep(solver, k->xs, k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+    // groups == 0: the direct form (one wave per possible item) for the fused
+    // sweep; the store strategy's persistent grids then take the default size
+    const bool direct = k->groups == 0;
+    const int persistent = direct ? default_groups(k) : k->groups;
+    const int directGrid = (int)((k->itemsAll + kWavesPerGroup - 1) / kWavesPerGroup);
+    int rc = k->fused ? launch_fused(solver, k->xs, direct ? std::max(1, directGrid) : k->groups, direct ? 1 : 0, k->A,
+                                     k->dDesc, k->dState, k->dG, k->dItems, k->stream)
+                      : launch_sweep(solver, k->xs, persistent, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(batch_round_kernel, dim3(1), dim3(kRoundThreads), 0, k->stream, k->dDesc, k->dState, k->dG,
                        k->dItems, k->dDtLog, k->dIterLog, k->dHostStat);
     if (!k->fused) {
         switch (k->xs) {
             case 4:
-                launch_apply<4>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+                launch_apply<4>(persistent, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
                 break;
             case 16:
-                launch_apply<16>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+                launch_apply<16>(persistent, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
                 break;
             default:
-                launch_apply<8>(k->groups, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
+                launch_apply<8>(persistent, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
                 break;
         }
         hipLaunchKernelGGL(batch_fill_kernel, dim3(1), ctl, 0, k->stream, k->dDesc, k->dState, k->dG, k->dItems);
@@ -949,7 +957,7 @@ int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float*
         swe_hip_batch_destroy(k);
         return rc;
     }
-    k->groups = default_groups(k);
+    k->groups = 0;  // the direct form: fastest on the LTS tiling and the ensemble (swe_hip_batch_autotune)
     *out = k;
     return SWE_HIP_OK;
 }
@@ -1079,7 +1087,7 @@ int swe_hip_batch_stats(swe_hip_batch* k, int64_t* updates, float* t) {
 }
 
 int swe_hip_batch_set_tuning(swe_hip_batch* k, int xs, int groups, int chunk, int graph, int fused) {
-    if (!k || (xs != -1 && xs != 4 && xs != 8 && xs != 16) || (groups != -1 && (groups < 1 || groups > 65535)) ||
+    if (!k || (xs != -1 && xs != 4 && xs != 8 && xs != 16) || (groups != -1 && (groups < 0 || groups > 65535)) ||
         (chunk != -1 && (chunk < 1 || chunk > 256)) || (graph != -1 && graph != 0 && graph != 1) ||
         (fused != -1 && fused != 0 && fused != 1)) {
         set_error("swe_hip_batch_set_tuning: invalid argument (xs=%d groups=%d chunk=%d graph=%d fused=%d)", xs, groups,
@@ -1171,7 +1179,7 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
     };
     if (e == hipSuccess) e = copy_all(true);
     const int xsC[3] = {4, 8, 16};
-    const int gC[3] = {2, 4, 8};
+    const int gC[4] = {2, 4, 8, 0};  // x CUs; 0 = the direct form
     const int oldXs = k->xs, oldGroups = k->groups, oldChunk = k->chunk;
     int bestXs = oldXs, bestGroups = oldGroups, n = 0, rc = 0;
     float best = FLT_MAX;
@@ -1180,7 +1188,7 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const int units = 1 << 30;  // never reached: the timed iterations run through macro-step ends
     for (int a = 0; a < 3 && e == hipSuccess && rc == 0; ++a) {
-        for (int g = 0; g < 3 && e == hipSuccess && rc == 0; ++g) {
+        for (int g = 0; g < 4 && e == hipSuccess && rc == 0; ++g) {
             rc = swe_hip_batch_set_tuning(k, xsC[a], gC[g] * k->cus, std::min(iterations, 256), -1, -1);
             if (rc) break;
             // warm-up chunk (also captures the graph), then the timed chunks

@@ -76,7 +76,7 @@ template <int S>
 int launch_batch_sweep_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
                               BatchGlobals* G, const int* items, hipStream_t stream);
 template <int S>
-int launch_batch_fused_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
-                              BatchGlobals* G, const int* items, hipStream_t stream);
+int launch_batch_fused_solver(int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
+                              BatchBlockState* st, BatchGlobals* G, const int* items, hipStream_t stream);
 
 }  // namespace swe

@@ -30,15 +30,17 @@ def _assert_blocks_equal(oracle, got, want, blocks, what):
             assert np.array_equal(a, c), f"{what}: block {k} {name} ghost cells differ"
 
 
-@pytest.mark.parametrize("xs,chunk,graph,fused", [(8, 16, 1, 1), (4, 7, 1, 1), (16, 5, 0, 1), (8, 16, 1, 0),
-                                                  (16, 5, 0, 0)])
-def test_ensemble_matches_independent_oracle_runs(swe, oracle, xs, chunk, graph, fused):
+# groups: -1 the default (the direct form), 0 direct, > 0 a persistent grid of that many workgroups
+@pytest.mark.parametrize("xs,chunk,graph,fused,groups", [(8, 16, 1, 1, -1), (4, 7, 1, 1, 512), (16, 5, 0, 1, 2048),
+                                                         (8, 16, 1, 0, -1), (16, 5, 0, 0, 300), (8, 16, 1, 1, 0),
+                                                         (4, 5, 0, 0, 0)])
+def test_ensemble_matches_independent_oracle_runs(swe, oracle, xs, chunk, graph, fused, groups):
     from swe_ppm_amd import scenarios
 
     inst = scenarios.ensemble_instances(count=24, seed=7, nmin=8, nmax=140)
     nsteps = 37
     with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
-        bt.set_tuning(xs=xs, chunk=chunk, graph=graph, fused=fused)
+        bt.set_tuning(xs=xs, groups=groups, chunk=chunk, graph=graph, fused=fused)
         bt.upload([i["h"] for i in inst], [i["hu"] for i in inst], [i["hv"] for i in inst], [i["b"] for i in inst])
         dts, its, ms = bt.run(nsteps)
         got = bt.download()
@@ -72,14 +74,14 @@ def test_ensemble_runs_continue_across_calls(swe, oracle):
     _assert_blocks_equal(oracle, got, want, inst, "ensemble 1+4+9 steps")
 
 
-@pytest.mark.parametrize("solver,fused", [(2, 1), (3, 1), (2, 0)])
-def test_global_dt_blocks_match_oracle(swe, oracle, solver, fused):
+@pytest.mark.parametrize("solver,fused,groups", [(2, 1, -1), (3, 1, 600), (2, 0, -1)])
+def test_global_dt_blocks_match_oracle(swe, oracle, solver, fused, groups):
     from swe_ppm_amd import scenarios
 
     blocks, dx, dy = scenarios.lts_tiling(n=256)
     nsteps = 25
     with swe.SweBatch(blocks, solver=solver, mode=swe.BATCH_GLOBAL, dx=dx, dy=dy) as bt:
-        bt.set_tuning(chunk=6, fused=fused)
+        bt.set_tuning(groups=groups, chunk=6, fused=fused)
         bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
         dts, its, _ = bt.run(nsteps)
         got = bt.download()
@@ -90,9 +92,10 @@ def test_global_dt_blocks_match_oracle(swe, oracle, solver, fused):
     _assert_blocks_equal(oracle, got, states, blocks, "global dt")
 
 
-@pytest.mark.parametrize("T,duration,nmacro,fused", [(1.0, 3.0, 3, 1), (1.0, 2.5, 3, 1), (0.5, 1.2, 3, 1),
-                                                     (1.0, 3.0, 3, 0), (0.5, 1.2, 3, 0)])
-def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro, fused):
+@pytest.mark.parametrize("T,duration,nmacro,fused,groups", [(1.0, 3.0, 3, 1, -1), (1.0, 2.5, 3, 1, 1024),
+                                                            (0.5, 1.2, 3, 1, -1), (1.0, 3.0, 3, 0, -1),
+                                                            (0.5, 1.2, 3, 0, 700), (1.0, 2.5, 3, 1, 0)])
+def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro, fused, groups):
     """Local time stepping over 28 uneven blocks: copies, interpolated ghost
     layers, power-of-two sub-steps, blocks reaching the end time mid-step."""
     from swe_ppm_amd import scenarios
@@ -100,7 +103,7 @@ def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro, fused):
     blocks, dx, dy = scenarios.lts_tiling(n=256)
     with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=swe.BATCH_LTS, dx=dx, dy=dy, max_local=T,
                       duration=duration) as bt:
-        bt.set_tuning(chunk=5, fused=fused)
+        bt.set_tuning(groups=groups, chunk=5, fused=fused)
         bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
         _, its, _ = bt.run(nmacro)
         got = bt.download()
@@ -124,7 +127,7 @@ def test_lts_macro_steps_split_over_calls_and_autotune(swe, oracle):
         bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
         _, its1, _ = bt.run(1)
         cand = bt.autotune(iterations=8)
-        assert len(cand) == 9 and np.all(cand > 0)
+        assert len(cand) == 12 and np.all(cand > 0)
         _, its2, _ = bt.run(2)
         got = bt.download()
     states, _, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 0.5, 10.0, 3, nthreads=8)
