@@ -437,8 +437,12 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
 // direct = 0: persistent grid, waves pull items until the queue is drained;
 // direct = 1: one wave per possible item (the grid covers every block's
 // tiles), a wave takes item blockIdx.x * kWavesPerGroup + wave or leaves.
+// Seven waves per SIMD (at most 72 VGPRs, no spills) rather than the six the
+// register count would give: an LTS iteration has about one item per wave
+// slot, so more waves in flight hide more latency (measured 2-4% faster; the
+// single-block kernel, two items per slot, gains nothing from it).
 template <int S, int XS>
-__global__ __launch_bounds__(256) void batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
                                                           BatchGlobals* G, const int* items, int direct) {
     if (G->skip) return;
     const int lane = threadIdx.x & 63;
