@@ -1,0 +1,113 @@
+"""Parity at the full sizes of the measured workloads, through the C ABI.
+
+- 2048^2 islands (AugRie): bit-identical to the oracle over 100 steps; over
+  500 steps the interior mass is conserved (the dam-break front stays far
+  from the outflow boundaries).
+- 4096^2 trench with a sea-surface hump (AugRie): bit-identical over 40
+  steps.  The same trench at rest (no hump) stays at rest: h unchanged, no
+  y-momentum, and x-momentum only from FP32 round-off of the hydrostatic
+  balance (the oracle shows the same at 256^2).
+- the 28-block LTS tiling of 2048^2: one macro step bit-identical to the
+  oracle's overdecomposed LTS loop.
+- all 512 ensemble instances: 5 steps bit-identical to 512 oracle runs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_batch import _assert_blocks_equal
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _run(swe, state, steps, solver):
+    """steps of the default (fused) loop; returns the state and the dts"""
+    h, hu, hv, b, dx, dy, bnd = state
+    n, m = h.shape[0] - 2, h.shape[1] - 2
+    with swe.SweBlock(n, m, dx, dy, solver=solver) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        _, dts = blk.run(steps)
+        out = blk.download()
+    return out, dts
+
+
+def test_islands_2048_bitwise_and_mass(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    st = scenarios.dam_break_islands(n=2048, seed=2048)
+    h, hu, hv, b, dx, dy, bnd = st
+    print("2048^2 islands, 100 steps GPU and oracle", flush=True)
+    (gh, ghu, ghv), gdts = _run(swe, st, 100, swe.AUGRIE)
+    oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, 100, nthreads=THREADS)
+    assert np.array_equal(np.asarray(gdts, np.float32), odts)
+    for name, a, c in (("h", gh, oh), ("hu", ghu, ohu), ("hv", ghv, ohv)):
+        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
+    print("2048^2 islands, 500 steps GPU", flush=True)
+    (gh5, _, _), _ = _run(swe, st, 500, swe.AUGRIE)
+    m0 = scenarios.interior_mass(h, dx, dy)
+    m1 = scenarios.interior_mass(gh5, dx, dy)
+    assert abs(m1 - m0) / m0 < 1e-5
+    assert np.any(gh5[1:-1, 1:-1] == 0.0), "the islands stay dry"
+
+
+def test_trench_4096_bitwise(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    st = scenarios.trench_hump(n=4096, seed=4096)
+    h, hu, hv, b, dx, dy, bnd = st
+    print("4096^2 trench, 40 steps GPU and oracle", flush=True)
+    (gh, ghu, ghv), gdts = _run(swe, st, 40, swe.AUGRIE)
+    oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, 40, nthreads=THREADS)
+    assert np.array_equal(np.asarray(gdts, np.float32), odts)
+    for name, a, c in (("h", gh, oh), ("hu", ghu, ohu), ("hv", ghv, ohv)):
+        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
+
+
+def test_trench_4096_lake_at_rest(swe):
+    from swe_ppm_amd import scenarios
+
+    _, _, _, b, dx, dy, bnd = scenarios.trench_hump(n=4096, seed=4096)
+    h0 = np.maximum(-b, 0.0).astype(np.float32)
+    z = np.zeros_like(h0)
+    print("4096^2 trench at rest, 100 steps", flush=True)
+    (gh, ghu, ghv), _ = _run(swe, (h0, z, z.copy(), b, dx, dy, bnd), 100, swe.AUGRIE)
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.all(ghv[inner] == 0.0)
+    assert np.max(np.abs(gh[inner] - h0[inner]) / h0[inner]) <= 1e-6
+    assert np.max(np.abs(ghu[inner]) / h0[inner]) <= 1e-4  # a velocity of at most 1e-4 m/s
+
+
+def test_lts_tiling_2048_one_macro_step(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    blocks, dx, dy = scenarios.lts_tiling(seed=28, n=2048)
+    print("28-block LTS tiling of 2048^2, 1 macro step", flush=True)
+    with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=swe.BATCH_LTS, dx=dx, dy=dy, max_local=1.0,
+                      duration=80.0) as bt:
+        bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
+        _, its, _ = bt.run(1)
+        got = bt.download()
+        upd, _ = bt.stats()
+    states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 1.0, 80.0, 1, nthreads=THREADS)
+    assert np.array_equal(its, oits)
+    assert np.array_equal(upd, oupd)
+    _assert_blocks_equal(oracle, got, states, blocks, "LTS 2048^2 tiling")
+
+
+def test_ensemble_512_instances(swe, oracle):
+    from swe_ppm_amd import scenarios
+
+    inst = scenarios.ensemble_instances(count=512, seed=512)
+    nsteps = 5
+    print("512-instance ensemble, 5 steps", flush=True)
+    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+        bt.upload(*[[i[f] for i in inst] for f in ("h", "hu", "hv", "b")])
+        bt.run(nsteps)
+        got = bt.download()
+    want = [oracle.run_u(i["h"], i["hu"], i["hv"], i["b"], i["dx"], i["dy"], i["bnd"], swe.AUGRIE, nsteps)[:3]
+            for i in inst]
+    _assert_blocks_equal(oracle, got, want, inst, "512-instance ensemble")
