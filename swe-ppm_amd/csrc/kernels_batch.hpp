@@ -440,10 +440,13 @@ __device__ __forceinline__ float fused_tile(const dev::SolverCtx<S>& solver, con
 // Seven waves per SIMD (at most 72 VGPRs, no spills) rather than the six the
 // register count would give: an LTS iteration has about one item per wave
 // slot, so more waves in flight hide more latency (measured 2-4% faster; the
-// single-block kernel, two items per slot, gains nothing from it).
+// single-block kernel, two items per slot, gains nothing from it).  The
+// learned-middle-state solver keeps its own register budget (its network
+// call needs far more).
 template <int S, int XS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
-                                                          BatchGlobals* G, const int* items, int direct) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == SOLVER_AUGRIE_MLP ? 1 : 7, 8))) void
+batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, BatchGlobals* G, const int* items,
+                   int direct) {
     if (G->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
