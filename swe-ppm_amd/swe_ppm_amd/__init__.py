@@ -33,12 +33,19 @@ class SweHipError(RuntimeError):
     pass
 
 
+class _Grid(C.Structure):
+    """swe_hip_grid (include/swe_hip.h)"""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("x0", C.c_double), ("y0", C.c_double), ("dx", C.c_double),
+                ("dy", C.c_double), ("values", C.POINTER(C.c_float))]
+
+
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise SweHipError(
             f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(str(LIB_PATH))
     fp = C.POINTER(C.c_float)
+    gp = C.POINTER(_Grid)
     ip = C.POINTER(C.c_int)
     vp = C.c_void_p
     sig = {
@@ -52,6 +59,7 @@ def _load() -> C.CDLL:
         "swe_hip_set_boundary": (C.c_int, [vp, C.c_int, C.c_int]),
         "swe_hip_apply_bc": (C.c_int, [vp]),
         "swe_hip_apply_bc_bathymetry": (C.c_int, [vp]),
+        "swe_hip_init_grid_scenario": (C.c_int, [vp, gp, gp]),
         "swe_hip_fluxes": (C.c_int, [vp, fp]),
         "swe_hip_update": (C.c_int, [vp, C.c_float]),
         "swe_hip_run": (C.c_int, [vp, C.c_int, fp, fp]),
@@ -156,6 +164,26 @@ class SweBlock:
     def upload(self, h, hu, hv, b) -> None:
         arrs = [_f32(a, self.shape) for a in (h, hu, hv, b)]
         _check(lib.swe_hip_upload(self._h, *[_fptr(a) for a in arrs]), "upload")
+
+    def init_grid_scenario(self, bathymetry, displacement=None) -> None:
+        """The file-driven scenario sampled on the device
+        (swe_hip_init_grid_scenario): each grid is (values[ny, nx], x0, y0,
+        dx, dy) with cell-centred values, the first centre at (x0, y0).  Set
+        the boundaries first; they are applied afterwards."""
+        keep = []
+
+        def grid(g):
+            values, x0, y0, dx, dy = g
+            v = np.ascontiguousarray(values, dtype=np.float32)
+            if v.ndim != 2:
+                raise ValueError("grid values must be 2-D (ny, nx)")
+            keep.append(v)
+            return _Grid(v.shape[1], v.shape[0], x0, y0, dx, dy, _fptr(v))
+
+        gb = grid(bathymetry)
+        gd = grid(displacement) if displacement is not None else None
+        _check(lib.swe_hip_init_grid_scenario(self._h, C.byref(gb), C.byref(gd) if gd is not None else None),
+               "init_grid_scenario")
 
     def download(self, with_b: bool = False):
         out = [np.empty(self.shape, np.float32) for _ in range(4 if with_b else 3)]

@@ -167,3 +167,34 @@ def test_invalid_arguments_fail_loudly(swe):
             blk.set_boundaries([2, 0, 0, 0])  # INFLOW: the reference asserts on it
         with pytest.raises(swe.SweHipError):
             blk.set_tuning(7, -1)
+
+
+def test_grid_scenario_known_answer_and_errors(swe):
+    """swe_hip_init_grid_scenario on a 2 x 2 bathymetry grid of 50 m cells
+    over [0, 100]^2, sampled by a 4 x 4 block of 25 m cells: each quarter of
+    the block takes one grid value (interior) and the OUTFLOW ghosts copy
+    their neighbours.  A displacement grid over [25, 75]^2 adds 1 m to the
+    four central cells only."""
+    bathy = (np.array([[-10.0, -20.0], [-30.0, 5.0]], np.float32), 25.0, 25.0, 50.0, 50.0)  # rows are y
+    displ = (np.ones((1, 1), np.float32), 50.0, 50.0, 50.0, 50.0)
+    with swe.SweBlock(4, 4, 25.0, 25.0) as blk:
+        blk.set_boundaries([0, 0, 0, 0])
+        blk.init_grid_scenario(bathy, displ)
+        h, hu, hv, b = blk.download(with_b=True)
+    want_b = np.zeros((4, 4), np.float32)
+    want_h = np.zeros((4, 4), np.float32)
+    for x in range(4):  # interior cell x + 1, centre at 12.5 + 25 x
+        for y in range(4):
+            bv = bathy[0][y // 2, x // 2]
+            inside = 1 <= x <= 2 and 1 <= y <= 2  # centres 37.5 and 62.5 lie strictly inside (25, 75)
+            want_b[x, y] = bv + (1.0 if inside else 0.0)
+            want_h[x, y] = 0.0 if bv > 0 else -bv
+    assert np.array_equal(b[1:5, 1:5], want_b)
+    assert np.array_equal(h[1:5, 1:5], want_h)
+    assert not np.any(hu) and not np.any(hv)
+    assert np.array_equal(b[0, 1:5], b[1, 1:5]) and np.array_equal(h[5, 1:5], h[4, 1:5])  # OUTFLOW ghosts
+    with swe.SweBlock(4, 4, 25.0, 25.0) as blk:
+        for bad in [(np.ones((2, 2), np.float32), 0.0, 0.0, 0.0, 1.0), (np.ones((2, 2), np.float32), 0.0, 0.0, 1.0, -1.0),
+                    (np.ones((0, 2), np.float32), 0.0, 0.0, 1.0, 1.0)]:
+            with pytest.raises(swe.SweHipError):
+                blk.init_grid_scenario(bad)
