@@ -220,3 +220,55 @@ def test_connected_edge_shapes_match_oracle(swe, oracle, lts):
     if lts:
         assert np.array_equal(its, oits)
     _assert_blocks_equal(oracle, got, states, blocks, f"connected edge shapes lts={lts}")
+
+
+def test_ensemble_at_the_block_limit(swe, oracle):
+    """2047 instances (the limit): the round keeps two blocks per thread."""
+    rng = np.random.default_rng(2047)
+    inst = []
+    for k in range(2047):
+        nx, ny = (int(v) for v in rng.integers(2, 14, size=2))
+        inst.append(_edge_instance(rng, nx, ny, [int(v) for v in rng.integers(0, 2, size=4)]))
+    nsteps = 4
+    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+        bt.upload(*[[i[f] for i in inst] for f in ("h", "hu", "hv", "b")])
+        bt.run(nsteps)
+        got = bt.download()
+        upd, _ = bt.stats()
+    assert np.all(upd == nsteps)
+    want = [oracle.run_u(i["h"], i["hu"], i["hv"], i["b"], np.float32(i["dx"]), np.float32(i["dy"]), i["bnd"],
+                         swe.AUGRIE, nsteps)[:3] for i in inst]
+    _assert_blocks_equal(oracle, got, want, inst, "2047 instances")
+    with pytest.raises(swe.SweHipError):
+        swe.SweBatch(inst + inst[:1], solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE)
+
+
+def test_lts_with_more_blocks_than_round_threads(swe, oracle):
+    """32 x 40 connected blocks (1280 > 1024 round threads) of uneven depth:
+    neighbour times cross between the round's two register slots."""
+    cols, rows = 32, 40
+    rng = np.random.default_rng(1280)
+    widths = [int(v) for v in rng.integers(4, 12, size=cols)]
+    heights = [int(v) for v in rng.integers(4, 12, size=rows)]
+    blocks = []
+    for c in range(cols):
+        for r in range(rows):
+            k = c * rows + r
+            nbr = [k - rows if c > 0 else -1, k + rows if c < cols - 1 else -1, k - 1 if r > 0 else -1,
+                   k + 1 if r < rows - 1 else -1]
+            bnd = [4 if m >= 0 else 0 for m in nbr]
+            nx, ny = widths[c], heights[r]
+            h = np.full((nx + 2, ny + 2), np.float32(rng.uniform(2.0, 40.0)), np.float32)
+            z = np.zeros_like(h)
+            blocks.append(dict(nx=nx, ny=ny, bnd=bnd, nbr=nbr, h=h, hu=z.copy(), hv=z.copy(), b=z.copy()))
+    dx = dy = 2.0
+    with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=swe.BATCH_LTS, dx=dx, dy=dy, max_local=0.5, duration=1.0) as bt:
+        bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
+        _, its, _ = bt.run(2)
+        got = bt.download()
+        upd, _ = bt.stats()
+    states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 0.5, 1.0, 2, nthreads=8)
+    assert np.array_equal(its, oits)
+    assert np.array_equal(upd, oupd)
+    assert len(set(upd.tolist())) > 1
+    _assert_blocks_equal(oracle, got, states, blocks, "1280 LTS blocks")
