@@ -248,6 +248,34 @@ def surrogate_compare(swe, state, weights, steps):
             "weights": str(Path(weights).name)}
 
 
+def batched_surrogate_ensemble(swe, scenarios, weights, plain_value: float, count: int = 512, seed: int = 512,
+                               steps: int = 100):
+    """The 512-instance ensemble through the batched engine with the learned
+    middle state attached (same instances and tuning defaults as the exact
+    ensemble line), against the plain 2048^2 line."""
+    from swe_ppm_amd import surrogate
+
+    inst = scenarios.ensemble_instances(count=count, seed=seed)
+    fields = [[i[f] for i in inst] for f in ("h", "hu", "hv", "b")]
+    mlp = surrogate.load_default(path=weights)
+    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+        bt.set_surrogate(mlp)
+        bt.upload(*fields)
+        bt.run(5)  # warm-up
+        bt.upload(*fields)
+        _, _, ms = bt.run(steps)
+        h = bt.download(packed=True)[0]
+    mlp.close()
+    if not np.all(np.isfinite(h)):
+        raise RuntimeError("non-finite ensemble state (learned middle state)")
+    updates = int(sum(i["nx"] * i["ny"] for i in inst)) * steps
+    value = updates / (ms * 1e-3) / 1e6
+    return {"workload": f"the {count}-instance ensemble (seed {seed}), AugRie FP32 with the learned middle state, "
+                        f"{steps} steps each",
+            "value": round(value, 3), "unit": "MCell-updates/s", "elapsed_ms": round(ms, 3),
+            "vs_plain": round(value / plain_value, 4), "weights": str(Path(weights).name)}
+
+
 def surrogate_trench(swe, scenarios, weights, n: int = 4096, steps: int = 1000, seed: int = 4096):
     """The n x n trench (depth 10..4000 m, 2 m hump), plain and learned."""
     out = {"workload": f"{n}x{n} trench (depth 10 m far field, 4000 m on the axis, Gaussian profile) with a 2 m "
@@ -439,7 +467,8 @@ def main() -> int:
         if Path(weights).exists():
             line["surrogate"] = {
                 "trench": surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps),
-                "islands": surrogate_islands(swe, scenarios, weights, n=n, steps=args.steps, seed=args.seed)}
+                "islands": surrogate_islands(swe, scenarios, weights, n=n, steps=args.steps, seed=args.seed),
+                "batched_ensemble_512": batched_surrogate_ensemble(swe, scenarios, weights, mcells)}
         else:
             line["surrogate"] = {"skipped": f"no trained network at {weights}"}
     print(json.dumps(line))
