@@ -248,6 +248,27 @@ def surrogate_compare(swe, state, weights, steps):
             "weights": str(Path(weights).name)}
 
 
+def solver_lines(swe, state, steps: int = 200):
+    """The headline workload with each solver of the plain path (fused
+    strategy, IEEE): MCell-updates/s and the algorithmic HBM fraction, to show
+    where the same kernel stands when the Riemann solver costs less."""
+    h, hu, hv, b, dx, dy, bnd = state
+    n = h.shape[0] - 2
+    out = {}
+    for name, sid in (("hlle", swe.HLLE), ("fwave", swe.FWAVE), ("hybrid", swe.HYBRID), ("augrie", swe.AUGRIE)):
+        with swe.SweBlock(n, n, dx, dy, solver=sid) as blk:
+            blk.set_boundaries(bnd)
+            blk.upload(h, hu, hv, b)
+            blk.run(10, want_dts=False)  # warm-up
+            blk.upload(h, hu, hv, b)
+            _, total_ms, _, _ = timed_run(swe, blk, steps, per_kernel=False)
+        v = n * n * steps / (total_ms * 1e-3) / 1e6
+        out[name] = {"value": round(v, 1), "ms_per_step": round(total_ms / steps, 5),
+                     "hbm_frac_algorithmic": round(BYTES_PER_CELL * v * 1e6 / 1e9 / HBM_PEAK_GBS, 4)}
+    return {"workload": f"{n}x{n} islands dam break (the headline workload), {steps} steps per solver",
+            "unit": "MCell-updates/s", "solvers": out}
+
+
 def batched_surrogate_ensemble(swe, scenarios, weights, plain_value: float, count: int = 512, seed: int = 512,
                                steps: int = 100):
     """The 512-instance ensemble through the batched engine with the learned
@@ -310,6 +331,7 @@ def main() -> int:
     ap.add_argument("--lts-duration", type=float, default=80.0, help="simulated seconds of the LTS workload")
     ap.add_argument("--ens-steps", type=int, default=300, help="steps of the ensemble workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-middle-state line")
+    ap.add_argument("--no-solvers", action="store_true", help="skip the per-solver lines")
     ap.add_argument("--surrogate-weights", type=Path, default=None,
                     help="trained network (.npy); default: the packaged weights")
     ap.add_argument("--trench-n", type=int, default=4096)
@@ -457,6 +479,8 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_solvers:
+        line["solvers"] = solver_lines(swe, (h, hu, hv, b, dx, dy, bnd))
     if not args.no_batched:
         lts = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
         ens = batched_ensemble(swe, scenarios, not args.no_autotune, steps=args.ens_steps)
