@@ -279,9 +279,9 @@ __device__ __forceinline__ Cell update_cell(float h, float hu, float hv, float d
 
 // One loaded cell: state plus the quantities its edges share.
 struct CellQ {
-    float h, hu, hv, b, u, v, c, sq;
-    __device__ __forceinline__ dev::Side xside() const { return dev::Side{h, hu, b, u, c, sq}; }
-    __device__ __forceinline__ dev::Side yside() const { return dev::Side{h, hv, b, v, c, sq}; }
+    float h, hu, hv, b, u, v, c, sq, inv;
+    __device__ __forceinline__ dev::Side xside() const { return dev::Side{h, hu, b, u, c, sq, inv}; }
+    __device__ __forceinline__ dev::Side yside() const { return dev::Side{h, hv, b, v, c, sq, inv}; }
     // the y-side of the cell held by lane + 1 (the cell above), by shuffle
     __device__ __forceinline__ dev::Side yside_from_lane_above() const {
         dev::Side s;
@@ -291,6 +291,7 @@ struct CellQ {
         s.u = __shfl_down(v, 1, 64);
         s.c = __shfl_down(c, 1, 64);
         s.sq = __shfl_down(sq, 1, 64);
+        s.inv = __shfl_down(inv, 1, 64);
         return s;
     }
 };
@@ -305,6 +306,7 @@ __device__ __forceinline__ CellQ cell_q(float h, float hu, float hv, float b) {
     q.v = hv / h;
     q.c = sqrtf(kGravity * h);
     q.sq = sqrtf(h);
+    q.inv = 1.f / h;
     return q;
 }
 

@@ -200,7 +200,8 @@ struct Bracket {
     bool newton, shocks;
 };
 
-__device__ __forceinline__ Bracket classify(float hL, float hR, float uL, float uR, float cL, float cR) {
+__device__ __forceinline__ Bracket classify(float hL, float hR, float uL, float uR, float cL, float cR, float invL,
+                                            float invR) {
     using namespace dev;
     Bracket k;
     k.hMin = smin(hL, hR);
@@ -208,8 +209,8 @@ __device__ __forceinline__ Bracket classify(float hL, float hR, float uL, float 
     k.du = uR - uL;
     const float cMin = (hR < hL) ? cR : cL;
     k.cMax = (hL < hR) ? cR : cL;
-    k.invMin = 1.f / k.hMin;
-    k.invMax = 1.f / k.hMax;
+    k.invMin = (hR < hL) ? invR : invL;
+    k.invMax = (hL < hR) ? invR : invL;
     const float fMin = k.du + 2.f * (cMin - k.cMax);
     const float fMax = k.du + (k.hMax - k.hMin) * sqrtf(0.5f * kGravity * (k.invMax + k.invMin));
     k.newton = !(k.hMin < kDryTol) && !(fMin > 0.f);
@@ -284,11 +285,12 @@ __device__ __forceinline__ dev::Middle finish(const Bracket& k, const Eval& e, f
 // edges try its guess; what still fails runs the exact iteration.
 // *how: 0 exact middle_state, 1 linearised guess accepted, 2 network guess.
 __device__ __forceinline__ dev::Middle learned_middle(const unsigned* __restrict__ F, float hL, float hR, float uL,
-                                                      float uR, float cL, float cR, float bL, float bR, int* how) {
+                                                      float uR, float cL, float cR, float invL, float invR, float bL,
+                                                      float bR, int* how) {
     // nothing moves across the learned part: two edge solves interleaved by
     // the scheduler would each hold their network state at once
     __builtin_amdgcn_sched_barrier(0);
-    const Bracket k = classify(hL, hR, uL, uR, cL, cR);
+    const Bracket k = classify(hL, hR, uL, uR, cL, cR, invL, invR);
     float x[kIn], hs, cs;
     features(k.newton ? hL : 1.f, k.newton ? hR : 1.f, k.newton ? uL : 0.f, k.newton ? uR : 0.f, bL, bR, x, hs, cs);
     float l0, l1;
@@ -309,23 +311,23 @@ __device__ __forceinline__ dev::Middle learned_middle(const unsigned* __restrict
     *how = used;
     __builtin_amdgcn_sched_barrier(0);
     if (used) return finish(k, ev, hL, hR, uL, uR, cL, cR);
-    return dev::middle_state(hL, hR, uL, uR, cL, cR);
+    return dev::middle_state(hL, hR, uL, uR, cL, cR, invL, invR);
 }
 
 struct LearnedMiddle {
     const unsigned* F;
     __device__ __forceinline__ dev::Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR,
-                                                      float bL, float bR) const {
+                                                      float invL, float invR, float bL, float bR) const {
         int how;
-        return learned_middle(F, hL, hR, uL, uR, cL, cR, bL, bR, &how);
+        return learned_middle(F, hL, hR, uL, uR, cL, cR, invL, invR, bL, bR, &how);
     }
 };
 
 }  // namespace mlp
 
 namespace dev {
-template <>
-struct SolverCtx<SOLVER_AUGRIE_MLP> {
+template <bool ReuseFirst>
+struct SolverCtx<SOLVER_AUGRIE_MLP, ReuseFirst> {
     const unsigned* F;
     __device__ __forceinline__ explicit SolverCtx(const unsigned* frags) : F(frags) {}
     __device__ __forceinline__ EdgeOut solve(const Side& L, const Side& R) const {

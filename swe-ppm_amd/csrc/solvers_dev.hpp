@@ -42,9 +42,10 @@ struct EdgeOut {
 // IEEE results each edge would compute itself.
 struct Side {
     float h, hu, b;
-    float u;   // hu / h (normal velocity; meaningless for dry cells)
-    float c;   // sqrt(g h)
-    float sq;  // sqrt(h)
+    float u;    // hu / h (normal velocity; meaningless for dry cells)
+    float c;    // sqrt(g h)
+    float sq;   // sqrt(h)
+    float inv;  // 1 / h (the augmented solver's middle state divides by both depths)
 };
 
 __device__ __forceinline__ Side cell_side(float h, float hu, float b) {
@@ -55,6 +56,7 @@ __device__ __forceinline__ Side cell_side(float h, float hu, float b) {
     s.u = hu / h;
     s.c = sqrtf(kGravity * h);
     s.sq = sqrtf(h);
+    s.inv = 1.f / h;
     return s;
 }
 
@@ -92,12 +94,14 @@ __device__ __forceinline__ Mirrored mirror_dry(const Side& L, const Side& R) {
     m.L.u = m.dryL ? -R.u : L.u;
     m.L.c = m.dryL ? R.c : L.c;
     m.L.sq = m.dryL ? R.sq : L.sq;
+    m.L.inv = m.dryL ? R.inv : L.inv;
     m.R.h = m.mirrorR ? L.h : R.h;
     m.R.hu = m.mirrorR ? -L.hu : R.hu;
     m.R.b = m.mirrorR ? L.b : R.b;
     m.R.u = m.mirrorR ? -L.u : R.u;
     m.R.c = m.mirrorR ? L.c : R.c;
     m.R.sq = m.mirrorR ? L.sq : R.sq;
+    m.R.inv = m.mirrorR ? L.inv : R.inv;
     return m;
 }
 
@@ -143,13 +147,24 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // lane, and its two square roots share one instruction slot.  The
 // derivative's two quotients share one division.  phi is increasing and
 // concave on the search interval, so the iterates increase monotonically.
-// cL, cR are sqrt(g hL), sqrt(g hR).
+// cL, cR are sqrt(g hL), sqrt(g hR); invL, invR are 1 / hL, 1 / hR, computed
+// once per cell.
+// ReuseFirst: the first iteration starts at hMax or hMin, where 1/h and one of
+// the two curve square roots are already known (the bracket's own quantities);
+// with ReuseFirst it takes them instead of recomputing them, so it costs one
+// square root and no division.  The values are the ones the loop computes, so
+// both forms give identical results.  The compiler peels the first iteration
+// for it, a second copy of the loop body: that pays in the batched kernels
+// (register budget fixed at 72 for seven waves per SIMD) and costs time in the
+// single-block kernels, which therefore recompute.
 // ---------------------------------------------------------------------------
 struct Middle {
     float hm, s1m, s2m;
 };
 
-__device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, float uR, float cL, float cR) {
+template <bool ReuseFirst = false>
+__device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, float uR, float cL, float cR, float invL,
+                                               float invR) {
     const float hMin = smin(hL, hR);
     const float hMax = smax(hL, hR);
     const float du = uR - uL;
@@ -162,12 +177,13 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
     }
     const float cMin = (hR < hL) ? cR : cL;
     const float cMax = (hL < hR) ? cR : cL;
-    const float invMin = 1.f / hMin;
-    const float invMax = 1.f / hMax;
+    const float invMin = (hR < hL) ? invR : invL;  // 1 / hMin
+    const float invMax = (hL < hR) ? invR : invL;  // 1 / hMax
     const float halfG = 0.5f * kGravity;
     const float quarterG = 0.25f * kGravity;
     const float fMin = du + 2.f * (cMin - cMax);
-    const float fMax = du + (hMax - hMin) * sqrtf(halfG * (invMax + invMin));
+    const float gMaxMin = sqrtf(halfG * (invMax + invMin));
+    const float fMax = du + (hMax - hMin) * gMaxMin;
     if (fMin > 0.f) {
         const float a = smax(0.f, -du + 2.f * (cL + cR));
         m.hm = a * a / (16.f * kGravity);
@@ -178,12 +194,23 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
     }
     const bool shocks = fMax <= 0.f;
     float h = shocks ? hMax : hMin;
-    float gMin = 0.f, s2 = 0.f;
+    float invH = 0.f, gMin = 0.f, s2 = 0.f;
+    if constexpr (ReuseFirst) {
+        // at h = hMax the shock curve of hMin is gMaxMin and the new root is
+        // gMax(hMax); at h = hMin the rarefaction root is cMin and the new root
+        // is gMin(hMin)
+        invH = shocks ? invMax : invMin;
+        const float r0 = sqrtf(halfG * (invH + invH));
+        gMin = shocks ? gMaxMin : r0;
+        s2 = shocks ? r0 : cMin;
+    }
     for (int it = 0; it < kMaxNewton; ++it) {
-        const float invH = 1.f / h;
-        gMin = sqrtf(halfG * (invH + invMin));
-        // shocks: gMax(h); shock+rarefaction: sqrt(g h)
-        s2 = sqrtf(shocks ? halfG * (invH + invMax) : kGravity * h);
+        if (!ReuseFirst || it > 0) {
+            invH = 1.f / h;
+            gMin = sqrtf(halfG * (invH + invMin));
+            // shocks: gMax(h); shock+rarefaction: sqrt(g h)
+            s2 = sqrtf(shocks ? halfG * (invH + invMax) : kGravity * h);
+        }
         const float t2 = shocks ? (h - hMax) * s2 : 2.f * (s2 - cMax);
         const float phi = (du + (h - hMin) * gMin) + t2;
         const float q = shocks ? s2 : 1.f;
@@ -220,10 +247,11 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
 // Riemann problem of a wet-wet or inundation edge; the exact Newton
 // middle_state is the default, the learned surrogate plugs in here.
 // ---------------------------------------------------------------------------
+template <bool ReuseFirst>
 struct ExactMiddle {
-    __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR, float,
-                                                 float) const {
-        return middle_state(hL, hR, uL, uR, cL, cR);
+    __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR,
+                                                 float invL, float invR, float, float) const {
+        return middle_state<ReuseFirst>(hL, hR, uL, uR, cL, cR, invL, invR);
     }
 };
 
@@ -231,8 +259,8 @@ struct ExactMiddle {
 // a wet one is either a wall (the wet side's middle state cannot reach its
 // bed: reflect) or inundated (the bed step is cut to the water surface).
 struct AugPrep {
-    float hL, huL, uL, cL, bL;
-    float hR, huR, uR, cR, bR;
+    float hL, huL, uL, cL, bL, invL;
+    float hR, huR, uR, cR, bR, invR;
     bool wallL, wallR, dryBoth;
 };
 
@@ -246,16 +274,18 @@ __device__ __forceinline__ AugPrep augrie_prepare(const Side& Lin, const Side& R
     p.uL = dryLin ? 0.f : Lin.u;
     p.cL = dryLin ? 0.f : Lin.c;
     p.bL = Lin.b;
+    p.invL = Lin.inv;  // a dry side never reaches the divisions
     p.hR = dryRin ? 0.f : Rin.h;
     p.huR = dryRin ? 0.f : Rin.hu;
     p.uR = dryRin ? 0.f : Rin.u;
     p.cR = dryRin ? 0.f : Rin.c;
     p.bR = Rin.b;
+    p.invR = Rin.inv;
     p.wallL = false;
     p.wallR = false;
     if ((dryLin || dryRin) && !p.dryBoth) {
         if (dryRin) {
-            const Middle t = middle_state(p.hL, p.hL, p.uL, -p.uL, p.cL, p.cL);
+            const Middle t = middle_state(p.hL, p.hL, p.uL, -p.uL, p.cL, p.cL, p.invL, p.invL);
             const float hTest = smax(p.hL, t.hm);
             if (hTest + p.bL < p.bR) {
                 p.wallR = true;
@@ -263,12 +293,13 @@ __device__ __forceinline__ AugPrep augrie_prepare(const Side& Lin, const Side& R
                 p.huR = -p.huL;
                 p.uR = -p.uL;
                 p.cR = p.cL;
+                p.invR = p.invL;
                 p.bR = p.bL;
             } else if (p.hL + p.bL < p.bR) {
                 p.bR = p.hL + p.bL;
             }
         } else {
-            const Middle t = middle_state(p.hR, p.hR, -p.uR, p.uR, p.cR, p.cR);
+            const Middle t = middle_state(p.hR, p.hR, -p.uR, p.uR, p.cR, p.cR, p.invR, p.invR);
             const float hTest = smax(p.hR, t.hm);
             if (hTest + p.bR < p.bL) {
                 p.wallL = true;
@@ -276,6 +307,7 @@ __device__ __forceinline__ AugPrep augrie_prepare(const Side& Lin, const Side& R
                 p.huL = -p.huR;
                 p.uL = -p.uR;
                 p.cL = p.cR;
+                p.invL = p.invR;
                 p.bL = p.bR;
             } else if (p.hR + p.bR < p.bL) {
                 p.bL = p.hR + p.bR;
@@ -298,7 +330,7 @@ __device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin,
     const float sE1 = smin(uL - cL, uHat - cHat);
     const float sE2 = smax(uR + cR, uHat + cHat);
 
-    const Middle mid = middle(hL, hR, uL, uR, cL, cR, bL, bR);
+    const Middle mid = middle(hL, hR, uL, uR, cL, cR, p.invL, p.invR, bL, bR);
 
     const float l1 = smin(sE1, mid.s2m);
     const float l3 = smax(sE2, mid.s1m);
@@ -378,7 +410,10 @@ __device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin,
     return o;
 }
 
-__device__ __forceinline__ EdgeOut augrie(const Side& L, const Side& R) { return augrie_with(L, R, ExactMiddle()); }
+template <bool ReuseFirst = false>
+__device__ __forceinline__ EdgeOut augrie(const Side& L, const Side& R) {
+    return augrie_with(L, R, ExactMiddle<ReuseFirst>());
+}
 
 // ---------------------------------------------------------------------------
 // HLLE in fluctuation form, surface-gradient diffusion (well-balanced)
@@ -417,33 +452,47 @@ __device__ __forceinline__ EdgeOut hlle(const Side& Lin, const Side& Rin) {
 template <int S>
 struct EdgeSolver;
 
+// ReuseFirst selects the form of the Newton loop (middle_state); the results
+// are the same.
 template <>
 struct EdgeSolver<SOLVER_FWAVE> {
-    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) { return fwave(L, R); }
+    template <bool ReuseFirst = false>
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) {
+        return fwave(L, R);
+    }
 };
 template <>
 struct EdgeSolver<SOLVER_AUGRIE> {
-    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) { return augrie(L, R); }
+    template <bool ReuseFirst = false>
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) {
+        return augrie<ReuseFirst>(L, R);
+    }
 };
 template <>
 struct EdgeSolver<SOLVER_HLLE> {
-    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) { return hlle(L, R); }
+    template <bool ReuseFirst = false>
+    __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) {
+        return hlle(L, R);
+    }
 };
 template <>
 struct EdgeSolver<SOLVER_HYBRID> {
+    template <bool ReuseFirst = false>
     __device__ __forceinline__ static EdgeOut solve(const Side& L, const Side& R) {
         if (L.h >= kHybridTol && R.h >= kHybridTol) return fwave(L, R);
-        return augrie(L, R);
+        return augrie<ReuseFirst>(L, R);
     }
 };
 
 // Solver with per-kernel state, built once at kernel start from an optional
 // parameter block (the learned middle state keeps its network weights in
 // registers; the exact solvers have no state).
-template <int S>
+template <int S, bool ReuseFirst = false>
 struct SolverCtx {
     __device__ __forceinline__ explicit SolverCtx(const unsigned*) {}
-    __device__ __forceinline__ EdgeOut solve(const Side& L, const Side& R) const { return EdgeSolver<S>::solve(L, R); }
+    __device__ __forceinline__ EdgeOut solve(const Side& L, const Side& R) const {
+        return EdgeSolver<S>::template solve<ReuseFirst>(L, R);
+    }
 };
 
 // Convenience for raw edge states (one-off evaluations).

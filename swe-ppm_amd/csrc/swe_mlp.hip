@@ -77,10 +77,10 @@ __global__ __launch_bounds__(256) void mlp_log_kernel(StateView v, float keep, u
     linear_part(x, l0, l1);
     if (hardOnly) {
         // only edges whose linearised guess fails the residual check
-        const Bracket k = classify(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR);
+        const Bracket k = classify(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR, p.invL, p.invR);
         if (eval_guess(k, l0 * hs).ok) return;
     }
-    const dev::Middle m = dev::middle_state(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR);
+    const dev::Middle m = dev::middle_state(p.hL, p.hR, p.uL, p.uR, p.cL, p.cR, p.invL, p.invR);
     const unsigned long long at = atomicAdd(count, 1ULL);
     if ((long long)at >= cap) return;
     float* o = samples + (size_t)at * kSampleFloats;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void mlp_middle_kernel(const unsigned* __restr
     const float bL = in[4 * n + sc], bR = in[5 * n + sc];
     const float cL = sqrtf(kGravity * hL), cR = sqrtf(kGravity * hR);
     int acc;
-    const dev::Middle m = learned_middle(frags, hL, hR, uL, uR, cL, cR, bL, bR, &acc);
+    const dev::Middle m = learned_middle(frags, hL, hR, uL, uR, cL, cR, 1.f / hL, 1.f / hR, bL, bR, &acc);
     if (s < n) {
         out[s] = m.hm;
         out[n + s] = m.s1m;

@@ -419,6 +419,19 @@ class MiddleStateMLP:
     def clear_samples(self) -> None:
         _check(lib.swe_hip_mlp_clear_samples(self._h), "mlp_clear_samples")
 
+    def sort_samples(self) -> None:
+        """Put the logged samples in a canonical order (rows sorted
+        lexicographically).  Logging appends edges in the order the device's
+        waves reach them, which varies from run to run, and training draws its
+        batches by index: sorting first makes training from the same samples
+        reproducible."""
+        s = self.read_samples()
+        if s.shape[0] < 2:
+            return
+        s = s[np.lexsort(s.T[::-1])]
+        self.clear_samples()
+        self.add_samples(s)
+
     def train(self, steps: int, batch: int = 4096, lr: float = 2e-3, seed: int = 0) -> np.ndarray:
         loss = np.zeros(max(steps, 1), np.float32)
         _check(lib.swe_hip_mlp_train(self._h, int(steps), int(batch), float(lr), int(seed), _fptr(loss)), "mlp_train")

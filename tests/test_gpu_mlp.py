@@ -99,6 +99,7 @@ def _trained_network(swe, seed=0):
             for k in range(24):
                 mlp.log_block(blk, keep=1.0, seed=100 * seed + k, hard_only=True)
                 blk.run(4, want_dts=False)
+    mlp.sort_samples()  # logging order varies between runs; fixed order makes the training reproducible
     loss = mlp.train(1500, batch=4096, lr=3e-3, seed=seed)
     return mlp, loss
 
@@ -116,6 +117,30 @@ def _held_out_edges(oracle):
     hm, s1, s2, kind, iters = oracle.middle_state(hL.astype(np.float64), hR, uL, uR)
     sel = kind >= 2
     return [a[sel].astype(np.float32) for a in (hL, hR, uL, uR, bL, bR)], iters[sel]
+
+
+def test_sorted_samples_do_not_depend_on_the_logging_order(swe):
+    """Logging appends edges in the order the waves reach them; sort_samples
+    gives every run the same rows in the same (lexicographic) order."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=128, seed=5)
+    runs = []
+    for rep in range(2):
+        with swe.SweBlock(128, 128, dx, dy, solver=swe.AUGRIE) as blk, \
+                swe.MiddleStateMLP(capacity=100_000) as mlp:
+            blk.set_boundaries(bnd)
+            blk.upload(h, hu, hv, b)
+            blk.run(20, want_dts=False)
+            n = mlp.log_block(blk, keep=1.0, seed=3)
+            raw = mlp.read_samples()
+            mlp.sort_samples()
+            assert mlp.count == n
+            got = mlp.read_samples()
+        assert n > 1000
+        assert np.array_equal(got, raw[np.lexsort(raw.T[::-1])])  # the same rows, sorted
+        runs.append(got)
+    assert np.array_equal(runs[0], runs[1])
 
 
 def test_training_reduces_loss_and_trained_network_passes_the_residual_check(swe, oracle):
