@@ -169,6 +169,17 @@ def test_invalid_arguments_fail_loudly(swe):
             blk.set_tuning(7, -1)
 
 
+def test_default_strip_width_follows_the_wave_count(swe):
+    """16 columns per strip when a block has at least two waves per wave slot
+    at that width (4096^2: 67 strips x 256 segments on 256 CUs x 24 slots),
+    else 8 (2048^2: 34 x 128 waves at 16)."""
+    for nx, ny, want in ((2048, 2048, 8), (4096, 4096, 16), (4096, 2048, 8), (2048, 4096, 8), (64, 64, 8)):
+        with swe.SweBlock(nx, ny, 1.0, 1.0) as blk:
+            assert blk.get_tuning()[1] == want, (nx, ny)
+            blk.set_tuning(-1, 32)
+            assert blk.get_tuning()[1] == 32
+
+
 def test_grid_scenario_known_answer_and_errors(swe):
     """swe_hip_init_grid_scenario on a 2 x 2 bathymetry grid of 50 m cells
     over [0, 100]^2, sampled by a 4 x 4 block of 25 m cells: each quarter of
