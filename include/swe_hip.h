@@ -193,9 +193,11 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
  * pass), 2 = fused (one launch per step that applies the previous update on
  * load; the default); xseg = x columns per wavefront strip (8, 16 or 32);
  * -1 keeps the current value.  All strategies and widths give identical
- * results.  A new block starts with xseg 16 when that still gives at least
- * two waves per wave slot of the device (nyt * ceil(nx/16) >= 2 * CUs * 24,
- * nyt = ceil(ny/62)), else 8. */
+ * results.  Until the caller sets xseg, a block uses the widest of 32, 16
+ * and 8 that still gives at least two waves per wave slot of the device:
+ * nyt * ceil(nx/xseg) >= 2 * CUs * 4 * w, nyt = ceil(ny/62), with w = 6 waves
+ * per SIMD for the exact solvers and 3 with a learned middle state attached
+ * (swe_hip_set_surrogate re-picks it). */
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
 

@@ -48,6 +48,7 @@ struct swe_hip_block {
     int strategy = 2;  // device loop: 0 recompute, 1 store + apply, 2 fused (apply on load)
     int math = 0;      // 0 IEEE (bit-compatible), 1 fast
     int xseg = 8;      // columns per wavefront strip (default_xseg)
+    bool xsegSet = false;  // set by the caller: kept when a network is attached or detached
     int cur = 0;       // index of the current state buffer
     BoundaryInfo bi{{PASSIVE, PASSIVE, PASSIVE, PASSIVE}};
     float* h[2] = {nullptr, nullptr};
@@ -70,20 +71,23 @@ size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pit
 
 // Default strip width of a flavour-U block.  A wider segment solves and loads
 // fewer halo columns per owned column (17 x-edges and 18 loaded columns per 16
-// owned, against 9 and 10 per 8) but gives fewer waves.  16 is taken when the
-// block still has at least two rounds of waves for the device's wave slots
-// (six per SIMD at the solvers' register use), 8 otherwise: on a 4096^2 block
-// (17 152 waves at 16) the step is 10% faster at 16; on 2048^2 (4 352 waves
-// for 6 144 slots) it is 10% slower.
-int default_xseg(int nx, int ny) {
+// owned, against 9 and 10 per 8) but gives fewer waves.  The widest of 32, 16
+// and 8 is taken that still gives at least two waves per wave slot of the
+// device, the slots counted at the kernel's occupancy: six waves per SIMD for
+// the exact solvers, three with a network attached (its registers).  Measured:
+// exact solver on 4096^2, 428 us per step at 16 against 482 at 8; on 2048^2
+// (4 352 waves at 16 for 6 144 slots) 16 is 10% slower than 8; learned middle
+// state on 4096^2, 614 us at 32, 655 at 16, 872 at 8.
+int default_xseg(int nx, int ny, int wavesPerSimd) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         return 8;
     const long long strips = (ny + kStripRows - 1) / kStripRows;
-    const long long waves16 = strips * ((nx + 15) / 16);
-    const long long slots = (long long)cus * 4 * 6;
-    return waves16 >= 2 * slots ? 16 : 8;
+    const long long slots = (long long)cus * 4 * wavesPerSimd;
+    for (int xs = 32; xs > 8; xs /= 2)
+        if (strips * ((nx + xs - 1) / xs) >= 2 * slots) return xs;
+    return 8;
 }
 
 // AugRie with a network attached runs the learned-middle-state instantiation
@@ -339,7 +343,7 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
     k->oy = oy;
     k->solver = solver;
     k->flavour = flavour;
-    k->xseg = default_xseg(nx, ny);
+    k->xseg = default_xseg(nx, ny, 6);
     const size_t bytes = field_elems(k) * sizeof(float);
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
@@ -786,7 +790,10 @@ int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
         return SWE_HIP_EINVAL;
     }
     if (strategy != -1) k->strategy = strategy;
-    if (xseg != -1) k->xseg = xseg;
+    if (xseg != -1) {
+        k->xseg = xseg;
+        k->xsegSet = true;
+    }
     return SWE_HIP_OK;
 }
 
@@ -829,6 +836,7 @@ int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
     HIP_TRY(hipStreamSynchronize(k->stream));
     HIP_TRY(hipMemcpy(&k->ctl->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
     k->mlp = frags;
+    if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags ? 3 : 6);
     return SWE_HIP_OK;
 }
 

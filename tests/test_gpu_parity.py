@@ -170,14 +170,25 @@ def test_invalid_arguments_fail_loudly(swe):
 
 
 def test_default_strip_width_follows_the_wave_count(swe):
-    """16 columns per strip when a block has at least two waves per wave slot
-    at that width (4096^2: 67 strips x 256 segments on 256 CUs x 24 slots),
-    else 8 (2048^2: 34 x 128 waves at 16)."""
-    for nx, ny, want in ((2048, 2048, 8), (4096, 4096, 16), (4096, 2048, 8), (2048, 4096, 8), (64, 64, 8)):
+    """The widest of 32, 16, 8 columns per strip that leaves at least two waves
+    per wave slot (256 CUs x 4 SIMDs x 6 waves; x 3 with a network attached):
+    4096^2 has 67 strips x 256 segments at 16 (x 128 at 32), 2048^2 34 x 128
+    at 16.  A width the caller set is kept."""
+    from swe_ppm_amd import surrogate
+
+    net = surrogate.load_default()
+    for nx, ny, exact, learned in ((2048, 2048, 8, 8), (4096, 4096, 16, 32), (4096, 2048, 8, 16),
+                                   (2048, 4096, 8, 16), (64, 64, 8, 8)):
         with swe.SweBlock(nx, ny, 1.0, 1.0) as blk:
-            assert blk.get_tuning()[1] == want, (nx, ny)
+            assert blk.get_tuning()[1] == exact, (nx, ny)
+            blk.set_surrogate(net)
+            assert blk.get_tuning()[1] == learned, (nx, ny)
+            blk.set_surrogate(None)
+            assert blk.get_tuning()[1] == exact, (nx, ny)
             blk.set_tuning(-1, 32)
+            blk.set_surrogate(net)
             assert blk.get_tuning()[1] == 32
+    net.close()
 
 
 def test_grid_scenario_known_answer_and_errors(swe):
