@@ -194,9 +194,9 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
  * load; the default); xseg = x columns per wavefront strip (8, 16 or 32);
  * -1 keeps the current value.  All strategies and widths give identical
  * results.  Until the caller sets xseg, a block uses the widest of 32, 16
- * and 8 that still gives at least two waves per wave slot of the device:
- * nyt * ceil(nx/xseg) >= 2 * CUs * 4 * w, nyt = ceil(ny/62), with w = 6 waves
- * per SIMD for the exact solvers and 3 with a learned middle state attached
+ * and 8 that still gives at least two waves per wave slot of the device,
+ * nyt * ceil(nx/xseg) >= 2 * CUs * 24 with nyt = ceil(ny/62); with a learned
+ * middle state attached, 16 when nyt * ceil(nx/16) >= CUs * 12, else 8
  * (swe_hip_set_surrogate re-picks it). */
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);

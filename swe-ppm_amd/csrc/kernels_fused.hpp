@@ -34,7 +34,33 @@ struct FusedIO {
     float* hvn;
     SumsView sIn;   // S(k-1)
     SumsView sOut;  // S(k)
+    uint2* deferQ = nullptr;  // queue of deferred edges (SOLVER_AUGRIE_DEFER only)
 };
+
+// Deferred edges (SOLVER_AUGRIE_DEFER).  An entry is (x, y) of the edge's left
+// or lower cell, with kDeferY set in .x for the y-edge (y|y+1); the x-edge is
+// (x|x+1).  Each edge is queued once, by the lane that owns it: x-edges by the
+// wave whose segment holds x (edge (0|1) by the first segment), y-edges by the
+// lane of row y (edge (0|1) by lane 0 of the first strip); rows 1..ny only
+// for x-edges.  A deferred edge contributes nothing in the sweep (zero parts,
+// zero speed); the second launch adds its parts to the (two-term) sums and its
+// speed to the CFL maximum, so every sum ends as the same two-term IEEE sum.
+constexpr unsigned kDeferY = 0x80000000u;
+
+__device__ __forceinline__ dev::EdgeOut defer_edge(const dev::EdgeOut& e, bool owns, uint2 entry, uint2* q,
+                                                   unsigned* count) {
+    const bool d = e.speed < 0.f;
+    const unsigned long long m = __ballot(d && owns);
+    if (m) {  // uniform over the wave; every lane is active here
+        unsigned base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (unsigned)__popcll(m));
+        base = __builtin_amdgcn_readfirstlane(base);
+        const unsigned below =
+            __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (d && owns) q[base + below] = entry;
+    }
+    return d ? dev::EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f} : e;
+}
 
 struct Q3 {
     float h, hu, hv;
@@ -55,6 +81,7 @@ __global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, B
     const float dt = timestep_from_max(__uint_as_float(ctl->smax[slotIn]), v.dx, v.dy);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->smax[slotClear] = 0u;
+        ctl->defer[slotClear] = 0u;
         ctl->dt = dt;
         ctl->t += dt;
         ctl->steps += 1;
@@ -130,6 +157,8 @@ __global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, B
     CellQ pq = cell_q(prev.h, prev.hu, prev.hv, cb);
     CellQ cq = cell_q(cur.h, cur.hu, cur.hv, nb);
     dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
+    if constexpr (S == SOLVER_AUGRIE_DEFER)
+        e = defer_edge(e, owned && x0 == 1, make_uint2(0u, (unsigned)y), io.deferQ, &ctl->defer[slotOut]);
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
 
@@ -143,9 +172,14 @@ __global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, B
         if (carryRow) store(x, cur);
         if (x + 1 == nx + 1 && carryRow) store(nx + 1, nxt);
 
-        const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
+        dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
+        if constexpr (S == SOLVER_AUGRIE_DEFER)
+            ev = defer_edge(ev, owned, make_uint2((unsigned)x, (unsigned)y), io.deferQ, &ctl->defer[slotOut]);
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
-        const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
+        dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
+        if constexpr (S == SOLVER_AUGRIE_DEFER)
+            eh = defer_edge(eh, owned || y == 0, make_uint2((unsigned)x | kDeferY, (unsigned)y), io.deferQ,
+                            &ctl->defer[slotOut]);
         if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);

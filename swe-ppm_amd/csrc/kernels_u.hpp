@@ -144,6 +144,7 @@ struct DeviceCtl {
     int steps;
     int pad[2];
     const unsigned* mlp;  // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
+    unsigned int defer[3];  // edges queued by the deferring sweep, per CFL slot (SOLVER_AUGRIE_DEFER)
 };
 
 // Which CFL-maximum slot a launch reads or accumulates into, and which slot

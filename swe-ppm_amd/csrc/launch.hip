@@ -74,7 +74,10 @@ __global__ void apply_bc_bathymetry_kernel(float* b, int nx, int ny, int pitch, 
 }
 
 __global__ void ctl_reset_kernel(DeviceCtl* ctl, int par) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->smax[par] = 0u;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->smax[par] = 0u;
+        ctl->defer[par] = 0u;
+    }
 }
 
 __global__ __launch_bounds__(256) void apply_s_kernel(StateView v, SumsView sums, float dt) {

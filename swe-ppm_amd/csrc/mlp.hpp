@@ -315,6 +315,7 @@ __device__ __forceinline__ dev::Middle learned_middle(const unsigned* __restrict
 }
 
 struct LearnedMiddle {
+    static constexpr bool kDefers = false;
     const unsigned* F;
     __device__ __forceinline__ dev::Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR,
                                                       float invL, float invR, float bL, float bR) const {
@@ -323,9 +324,39 @@ struct LearnedMiddle {
     }
 };
 
+// The first stage of learned_middle only: a Newton edge whose linearised
+// guess passes the residual check is finished here; a Newton edge whose guess
+// fails is deferred (negative depth), to be solved by learned_middle in a
+// later launch, which takes the same decisions and so gives the same result.
+// Dry and two-rarefaction edges take the exact closed forms.  No network call
+// and no Newton loop for wet edges stay in the sweep.
+struct DeferredMiddle {
+    static constexpr bool kDefers = true;
+    __device__ __forceinline__ dev::Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR,
+                                                      float invL, float invR, float bL, float bR) const {
+        const Bracket k = classify(hL, hR, uL, uR, cL, cR, invL, invR);
+        float x[kIn], hs, cs;
+        features(k.newton ? hL : 1.f, k.newton ? hR : 1.f, k.newton ? uL : 0.f, k.newton ? uR : 0.f, bL, bR, x, hs, cs);
+        float l0, l1;
+        linear_part(x, l0, l1);
+        const Eval ev = eval_guess(k, l0 * hs);
+        if (k.newton && ev.ok) return finish(k, ev, hL, hR, uL, uR, cL, cR);
+        if (k.newton) return dev::Middle{-1.f, 0.f, 0.f};
+        return dev::middle_state(hL, hR, uL, uR, cL, cR, invL, invR);
+    }
+};
+
 }  // namespace mlp
 
 namespace dev {
+template <bool ReuseFirst>
+struct SolverCtx<SOLVER_AUGRIE_DEFER, ReuseFirst> {
+    __device__ __forceinline__ explicit SolverCtx(const unsigned*) {}
+    __device__ __forceinline__ EdgeOut solve(const Side& L, const Side& R) const {
+        return augrie_with(L, R, mlp::DeferredMiddle{});
+    }
+};
+
 template <bool ReuseFirst>
 struct SolverCtx<SOLVER_AUGRIE_MLP, ReuseFirst> {
     const unsigned* F;

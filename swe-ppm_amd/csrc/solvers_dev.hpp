@@ -249,6 +249,7 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
 // ---------------------------------------------------------------------------
 template <bool ReuseFirst>
 struct ExactMiddle {
+    static constexpr bool kDefers = false;
     __device__ __forceinline__ Middle operator()(float hL, float hR, float uL, float uR, float cL, float cR,
                                                  float invL, float invR, float, float) const {
         return middle_state<ReuseFirst>(hL, hR, uL, uR, cL, cR, invL, invR);
@@ -407,6 +408,10 @@ __device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin,
     acc_wave(s3, z3h, z3hu, o);
     o.speed = smax(fabsf(s1), fabsf(s3));
     if (dryBoth) o = EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f};
+    // a middle function that defers the edge returns a negative depth: the
+    // edge is marked by a negative speed and solved again elsewhere
+    if constexpr (MiddleFn::kDefers)
+        if (mid.hm < 0.f) o.speed = -1.f;
     return o;
 }
 

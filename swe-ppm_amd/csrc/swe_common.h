@@ -13,6 +13,10 @@ enum Solver : int { SOLVER_HLLE = 0, SOLVER_FWAVE = 1, SOLVER_AUGRIE = 2, SOLVER
 // AugRie whose Newton middle state starts from a learned guess (internal id:
 // selected when a block has a surrogate network attached).
 constexpr int SOLVER_AUGRIE_MLP = 4;
+// AugRie with the learned middle state, hard edges deferred: the sweep keeps
+// the edges the linearised guess settles and queues the others for a second
+// launch that runs the network (fused strategy of a single block only).
+constexpr int SOLVER_AUGRIE_DEFER = 5;
 
 // Step flavours: U = every parallel block of the reference (all edges from
 // one state, unsplit update with dry clamp), S = swe_simple's split sweep.

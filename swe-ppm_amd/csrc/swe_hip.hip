@@ -63,6 +63,8 @@ struct swe_hip_block {
     hipStream_t stream = nullptr;
     float fluxDt = -1.f;  // dt of the last flavour-S flux pass (its Q* depends on it)
     const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
+    uint2* deferQ = nullptr;        // deferred-edge queue of the learned middle state's fused loop
+    unsigned deferCap = 0;          // its capacity: every edge of the block
 };
 
 namespace {
@@ -71,20 +73,23 @@ size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pit
 
 // Default strip width of a flavour-U block.  A wider segment solves and loads
 // fewer halo columns per owned column (17 x-edges and 18 loaded columns per 16
-// owned, against 9 and 10 per 8) but gives fewer waves.  The widest of 32, 16
-// and 8 is taken that still gives at least two waves per wave slot of the
-// device, the slots counted at the kernel's occupancy: six waves per SIMD for
-// the exact solvers, three with a network attached (its registers).  Measured:
-// exact solver on 4096^2, 428 us per step at 16 against 482 at 8; on 2048^2
-// (4 352 waves at 16 for 6 144 slots) 16 is 10% slower than 8; learned middle
-// state on 4096^2, 614 us at 32, 655 at 16, 872 at 8.
-int default_xseg(int nx, int ny, int wavesPerSimd) {
+// owned, against 9 and 10 per 8) but gives fewer waves; the device has
+// CUs x 4 SIMDs x 6 wave slots at the sweeps' register use.
+//  - exact solvers: the widest of 32, 16 and 8 that still gives at least two
+//    waves per slot.  Measured: 4096^2, 428 us per step at 16 against 482 at
+//    8; 2048^2 (4 352 waves at 16 for 6 144 slots), 16 is 10% slower than 8.
+//  - learned middle state (its sweep defers the hard edges, so its waves are
+//    lighter and more even): 16 while that gives at least half a wave per
+//    slot, else 8.  Measured: 2048^2, 165 us at 16, 184 at 8, 220 at 32;
+//    4096^2, 435 us at 16, 461 at 32, 503 at 8.
+int default_xseg(int nx, int ny, bool learned) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         return 8;
     const long long strips = (ny + kStripRows - 1) / kStripRows;
-    const long long slots = (long long)cus * 4 * wavesPerSimd;
+    const long long slots = (long long)cus * 4 * 6;
+    if (learned) return 2 * strips * ((nx + 15) / 16) >= slots ? 16 : 8;
     for (int xs = 32; xs > 8; xs /= 2)
         if (strips * ((nx + xs - 1) / xs) >= 2 * slots) return xs;
     return 8;
@@ -140,6 +145,30 @@ int ensure_second_sums(swe_hip_block* k) {
     return 0;
 }
 
+// One entry per edge the fused sweep solves: (nx+1) x-edges on each of ny
+// rows and (ny+1) y-edges on each of nx columns.
+int ensure_defer_queue(swe_hip_block* k) {
+    if (k->deferQ) return 0;
+    const size_t n = (size_t)(k->nx + 1) * k->ny + (size_t)k->nx * (k->ny + 1);
+    if (n > 0xffffffffull) {
+        set_error("the learned middle state's fused loop supports blocks of fewer than 2^32 edges");
+        return SWE_HIP_EINVAL;
+    }
+    HIP_TRY(hipMalloc(&k->deferQ, n * sizeof(uint2)));
+    k->deferCap = (unsigned)n;
+    return 0;
+}
+
+// Workgroups of the deferred-edge launch: one round of waves at the learned
+// solver's three waves per SIMD.
+int defer_groups() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus * 3;
+}
+
 int ensure_dtlog(swe_hip_block* k, int n) {
     if (n <= k->dtlogCap) return 0;
     if (k->dtlog) HIP_TRY(hipFree(k->dtlog));
@@ -175,6 +204,11 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
         if (int r = ensure_second_buffer(k)) return r;
     if (k->strategy == 2)
         if (int r = ensure_second_sums(k)) return r;
+    // the learned middle state's fused loop defers its hard edges to a second launch per step
+    const bool defer = k->strategy == 2 && nsteps > 1 && eff_solver(k) == SOLVER_AUGRIE_MLP;
+    if (defer)
+        if (int r = ensure_defer_queue(k)) return r;
+    const int deferGroups = defer ? defer_groups() : 0;
     float t0 = t_inout ? *t_inout : 0.f;
     HIP_TRY(hipMemcpyAsync(&k->ctl->t, &t0, sizeof(float), hipMemcpyHostToDevice, k->stream));
     if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
@@ -195,9 +229,18 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             FusedIO io{k->h[k->cur], k->hu[k->cur], k->hv[k->cur], k->b, k->h[o], k->hu[o], k->hv[o],
                        set[(s - 1) & 1], set[s & 1]};
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
-            if (int r = launch_fused_u(eff_solver(k), k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl, (s - 1) % 3,
-                                       s % 3, (s + 1) % 3, k->dtlog, s - 1, k->stream))
+            if (defer) {
+                io.deferQ = k->deferQ;
+                if (int r = launch_fused_defer(k->xseg, view(k, k->cur), io, k->bi, k->ctl, (s - 1) % 3, s % 3,
+                                               (s + 1) % 3, k->dtlog, s - 1, k->stream))
+                    return r;
+                if (int r = launch_defer_solve(view(k, o), set[s & 1], k->ctl, s % 3, k->deferQ, k->deferCap,
+                                               deferGroups, k->stream))
+                    return r;
+            } else if (int r = launch_fused_u(eff_solver(k), k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl,
+                                              (s - 1) % 3, s % 3, (s + 1) % 3, k->dtlog, s - 1, k->stream)) {
                 return r;
+            }
             k->cur = o;
         }
         if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 2));
@@ -343,7 +386,7 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
     k->oy = oy;
     k->solver = solver;
     k->flavour = flavour;
-    k->xseg = default_xseg(nx, ny, 6);
+    k->xseg = default_xseg(nx, ny, false);
     const size_t bytes = field_elems(k) * sizeof(float);
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
@@ -384,6 +427,7 @@ int swe_hip_destroy(swe_hip_block* k) {
     }
     if (k->ctl) (void)hipFree(k->ctl);
     if (k->dtlog) (void)hipFree(k->dtlog);
+    if (k->deferQ) (void)hipFree(k->deferQ);
     if (k->stream) (void)hipStreamDestroy(k->stream);
     delete k;
     return SWE_HIP_OK;
@@ -836,7 +880,7 @@ int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
     HIP_TRY(hipStreamSynchronize(k->stream));
     HIP_TRY(hipMemcpy(&k->ctl->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
     k->mlp = frags;
-    if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags ? 3 : 6);
+    if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags != nullptr);
     return SWE_HIP_OK;
 }
 

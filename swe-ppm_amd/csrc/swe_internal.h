@@ -66,6 +66,13 @@ int launch_fused_u_solver(int xseg, StateView v, FusedIO io, BoundaryInfo bi, De
                           int slotClear, float* dtLog, int logIdx, hipStream_t stream);
 int launch_fused_u(int solver, int variant, int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl,
                    int slotIn, int slotOut, int slotClear, float* dtLog, int logIdx, hipStream_t stream);
+// Learned middle state with deferred hard edges (launch_defer.hip): the
+// deferring fused sweep (io.deferQ set), then the launch that solves the
+// edges it queued in ctl->defer[slotOut] and completes the sums of vNew.
+int launch_fused_defer(int xseg, StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl, int slotIn, int slotOut,
+                       int slotClear, float* dtLog, int logIdx, hipStream_t stream);
+int launch_defer_solve(StateView vNew, SumsView sums, DeviceCtl* ctl, int slot, const uint2* queue, unsigned cap,
+                       int groups, hipStream_t stream);
 template <int S, int V>
 int launch_solve_edges_solver(int64_t n, const float* in, float* out, hipStream_t stream);
 template <int S>

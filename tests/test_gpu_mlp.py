@@ -119,6 +119,32 @@ def _held_out_edges(oracle):
     return [a[sel].astype(np.float32) for a in (hL, hR, uL, uR, bL, bR)], iters[sel]
 
 
+@pytest.mark.parametrize("nx,ny,xseg", [(256, 256, 8), (130, 77, 16), (200, 150, 32)])
+def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx, ny, xseg):
+    """With a network attached, the fused loop settles the edges whose
+    linearised guess passes the residual check inside the sweep and queues
+    the others for a second launch (network, check, exact fallback); the store
+    strategy solves every edge inside its sweep.  Both give the same bits."""
+    from swe_ppm_amd import scenarios, surrogate
+
+    net = surrogate.load_default()
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=max(nx, ny), seed=7)
+    h, hu, hv, b = (a[:nx + 2, :ny + 2].copy() for a in (h, hu, hv, b))
+    out = {}
+    for strategy in (1, 2):
+        with swe.SweBlock(nx, ny, dx, dy, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            blk.set_tuning(strategy, xseg)
+            blk.set_surrogate(net)
+            blk.upload(h, hu, hv, b)
+            _, dts = blk.run(40)
+            out[strategy] = (blk.download(), np.asarray(dts))
+    for name, a, c in zip(("h", "hu", "hv"), out[1][0], out[2][0]):
+        assert np.array_equal(a, c), name
+    assert np.array_equal(out[1][1], out[2][1])
+    net.close()
+
+
 def test_sorted_samples_do_not_depend_on_the_logging_order(swe):
     """Logging appends edges in the order the waves reach them; sort_samples
     gives every run the same rows in the same (lexicographic) order."""

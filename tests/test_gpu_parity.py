@@ -171,14 +171,14 @@ def test_invalid_arguments_fail_loudly(swe):
 
 def test_default_strip_width_follows_the_wave_count(swe):
     """The widest of 32, 16, 8 columns per strip that leaves at least two waves
-    per wave slot (256 CUs x 4 SIMDs x 6 waves; x 3 with a network attached):
-    4096^2 has 67 strips x 256 segments at 16 (x 128 at 32), 2048^2 34 x 128
-    at 16.  A width the caller set is kept."""
+    per wave slot (256 CUs x 4 SIMDs x 6 waves): 4096^2 has 67 strips x 256
+    segments at 16 (x 128 at 32), 2048^2 34 x 128 at 16.  With a network
+    attached, 16 from half a wave per slot.  A width the caller set is kept."""
     from swe_ppm_amd import surrogate
 
     net = surrogate.load_default()
-    for nx, ny, exact, learned in ((2048, 2048, 8, 8), (4096, 4096, 16, 32), (4096, 2048, 8, 16),
-                                   (2048, 4096, 8, 16), (64, 64, 8, 8)):
+    for nx, ny, exact, learned in ((2048, 2048, 8, 16), (4096, 4096, 16, 16), (4096, 2048, 8, 16),
+                                   (2048, 4096, 8, 16), (1024, 1024, 8, 8), (64, 64, 8, 8)):
         with swe.SweBlock(nx, ny, 1.0, 1.0) as blk:
             assert blk.get_tuning()[1] == exact, (nx, ny)
             blk.set_surrogate(net)
