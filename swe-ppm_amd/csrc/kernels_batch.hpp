@@ -115,6 +115,7 @@ struct BatchGlobals {
     const unsigned* mlp;   // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
     int fused;             // 1: fused sweep (update applied on the next load), 0: sweep + apply
     int allItems;          // work items of all blocks (the queue of an iteration without LTS)
+    unsigned deferCount;   // edges queued by the deferring sweep this iteration (SOLVER_AUGRIE_DEFER)
 };
 
 __device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d, int buf) {
@@ -345,7 +346,8 @@ struct BQ3 {
 template <int S, int XS>
 __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const StateView& v, const StateView& o,
                                             const SumsView& sIn, const SumsView& sOut, bool pend, float dt,
-                                            const int* bnd, const TileLanes& t) {
+                                            const int* bnd, const TileLanes& t, int blk, uint2* deferQ,
+                                            unsigned* deferCount) {
     const int nx = v.nx, ny = v.ny;
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;
@@ -408,7 +410,11 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
     if (t.first && carryRow) store(0, prev);
     CellQ pq = cell_q(prev.h, prev.hu, prev.hv, v.b[at(x0 - 1, rowC)]);
     CellQ cq = cell_q(cur.h, cur.hu, cur.hv, v.b[at(x0, rowC)]);
-    const dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
+    // deferred-edge entries: the lane's own column, the block in the high bits
+    const unsigned xLane = (unsigned)(t.xs - t.xs0) * XS, blkBits = (unsigned)blk << kDeferBlockShift;
+    dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
+    if constexpr (S == SOLVER_AUGRIE_DEFER)  // edge (0|1): only the first segment's items have t.first
+        e = defer_edge(e, owned && t.first, make_uint2(blkBits, (unsigned)y), deferQ, deferCount);
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = rowInterior ? e.speed : 0.f;
     for (int k = 0; k < n; ++k) {
@@ -419,9 +425,14 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
         const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, v.b[at(x + 1, rowC)]);
         if (carryRow) store(x, cur);
         if (rightGhost && carryRow) store(nx + 1, nxt);
-        const dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
+        dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
+        if constexpr (S == SOLVER_AUGRIE_DEFER)
+            ev = defer_edge(ev, owned, make_uint2(blkBits | ((unsigned)x + xLane), (unsigned)y), deferQ, deferCount);
         if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
-        const dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
+        dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
+        if constexpr (S == SOLVER_AUGRIE_DEFER)
+            eh = defer_edge(eh, owned || y == 0, make_uint2(kDeferY | blkBits | ((unsigned)x + xLane), (unsigned)y),
+                            deferQ, deferCount);
         if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);
@@ -452,7 +463,7 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
 template <int S, int XS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == SOLVER_AUGRIE_MLP ? 1 : 7, 8))) void
 batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, BatchGlobals* G, const int* items,
-                   int direct) {
+                   int direct, uint2* deferQ) {
     if (G->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -469,7 +480,7 @@ batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, B
         const float dt = st[blk].maxTimestep;
         const float m = fused_tile<S, XS>(solver, batch_view(A, d, cur), batch_view(A, d, cur ^ 1),
                                           batch_sums(A, d, sidx), batch_sums(A, d, sidx ^ 1), pend, dt, d.bnd,
-                                          tile_lanes(d, tile, lane));
+                                          tile_lanes(d, tile, lane), blk, deferQ, &G->deferCount);
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     };
     if (direct) {

@@ -172,6 +172,34 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Deferred edges (SOLVER_AUGRIE_DEFER).  An entry is (x, y) of the edge's left
+// or lower cell, with kDeferY set in .x for the y-edge (y|y+1); the x-edge is
+// (x|x+1).  The batched engine keeps the block index in bits kDeferBlockShift
+// and up of .x (blocks < 2^11, x < 2^20).  Each edge is queued once, by the
+// lane that owns it: x-edges by the wave whose segment holds x (edge (0|1) by
+// the first segment), y-edges by the lane of row y (edge (0|1) by lane 0 of
+// the first strip); rows 1..ny only for x-edges.  A deferred edge contributes
+// nothing in the sweep (zero parts, zero speed); a second launch adds its
+// parts to the (two-term) sums and its speed to the CFL maximum, so every sum
+// ends as the same two-term IEEE sum.
+constexpr unsigned kDeferY = 0x80000000u;
+constexpr int kDeferBlockShift = 20;
+
+__device__ __forceinline__ dev::EdgeOut defer_edge(const dev::EdgeOut& e, bool owns, uint2 entry, uint2* q,
+                                                   unsigned* count) {
+    const bool d = e.speed < 0.f;
+    const unsigned long long m = __ballot(d && owns);
+    if (m) {  // uniform over the wave; every lane is active here
+        unsigned base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (unsigned)__popcll(m));
+        base = __builtin_amdgcn_readfirstlane(base);
+        const unsigned below =
+            __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (d && owns) q[base + below] = entry;
+    }
+    return d ? dev::EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f} : e;
+}
+
 // True for the ghost cells that the fused boundary write refreshes after an
 // update: corners, and ghost lines of OUTFLOW/WALL sides.
 __device__ __forceinline__ bool bc_handles(const StateView& v, const BoundaryInfo& bi, int gx, int gy) {

@@ -293,6 +293,7 @@ __global__ __launch_bounds__(kRoundThreads) void batch_round_kernel(const BatchB
                                                                     BatchGlobals* G, int* items, float* dtLog,
                                                                     int* iterLog, int* hostStat) {
     if (G->skip) return;
+    if (threadIdx.x == 0) G->deferCount = 0u;  // the deferred edges of this iteration are solved
     const int mode = G->mode;
     const float T = G->T;
     const int nb = G->nblocks;
@@ -575,18 +576,22 @@ int launch_apply(int groups, BatchArena A, const BatchBlockDesc* D, const BatchB
 }
 
 int launch_fused(int solver, int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
-                 BatchBlockState* st, BatchGlobals* G, const int* items, hipStream_t stream) {
+                 BatchBlockState* st, BatchGlobals* G, const int* items, uint2* deferQ, hipStream_t stream) {
     switch (solver) {
         case SOLVER_HLLE:
-            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, direct, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
         case SOLVER_FWAVE:
-            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, direct, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
         case SOLVER_AUGRIE:
-            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, direct, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
         case SOLVER_AUGRIE_MLP:
-            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, direct, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, direct, A, D, st, G, items, nullptr,
+                                                                stream);
+        case SOLVER_AUGRIE_DEFER:
+            return launch_batch_fused_solver<SOLVER_AUGRIE_DEFER>(xs, groups, direct, A, D, st, G, items, deferQ,
+                                                                  stream);
         default:
-            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, direct, A, D, st, G, items, stream);
+            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
     }
 }
 
@@ -633,6 +638,8 @@ struct swe_hip_batch {
     bool uploaded = false;
     const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
     int fused = 1;                  // 1: fused sweep (update applied on load), 0: sweep + apply
+    uint2* deferQ = nullptr;        // deferred-edge queue of the learned middle state's fused sweep, or null
+    unsigned deferCap = 0;
 };
 
 namespace {
@@ -679,15 +686,24 @@ int enqueue_iteration(swe_hip_batch* k) {
         hipLaunchKernelGGL(batch_lines_kernel, dim3(k->nblocks, 4, kSideChunks), ctl, 0, k->stream, k->A, k->dDesc,
                            k->dState, k->dG);
     const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
+    // the learned middle state's fused sweep defers its hard edges to a second launch
+    const bool defer = k->fused && solver == SOLVER_AUGRIE_MLP && k->deferQ;
     // groups == 0: the direct form (one wave per possible item) for the fused
     // sweep; the store strategy's persistent grids then take the default size
     const bool direct = k->groups == 0;
     const int persistent = direct ? default_groups(k) : k->groups;
     const int directGrid = (int)((k->itemsAll + kWavesPerGroup - 1) / kWavesPerGroup);
-    int rc = k->fused ? launch_fused(solver, k->xs, direct ? std::max(1, directGrid) : k->groups, direct ? 1 : 0, k->A,
-                                     k->dDesc, k->dState, k->dG, k->dItems, k->stream)
+    int rc = k->fused ? launch_fused(defer ? SOLVER_AUGRIE_DEFER : solver, k->xs,
+                                     direct ? std::max(1, directGrid) : k->groups, direct ? 1 : 0, k->A, k->dDesc,
+                                     k->dState, k->dG, k->dItems, k->deferQ, k->stream)
                       : launch_sweep(solver, k->xs, persistent, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
+    if (defer) {
+        // one round of waves at the learned solver's three waves per SIMD
+        rc = launch_batch_defer_solve(k->A, k->dDesc, k->dState, k->dG, k->deferQ, k->deferCap, 3 * k->cus,
+                                      k->stream);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(batch_round_kernel, dim3(1), dim3(kRoundThreads), 0, k->stream, k->dDesc, k->dState, k->dG,
                        k->dItems, k->dDtLog, k->dIterLog, k->dHostStat);
     if (!k->fused) {
@@ -975,6 +991,7 @@ int swe_hip_batch_destroy(swe_hip_batch* k) {
     if (k->dState) (void)hipFree(k->dState);
     if (k->dG) (void)hipFree(k->dG);
     if (k->dItems) (void)hipFree(k->dItems);
+    if (k->deferQ) (void)hipFree(k->deferQ);
     if (k->dDtLog) (void)hipFree(k->dDtLog);
     if (k->dIterLog) (void)hipFree(k->dIterLog);
     if (k->hostStat) (void)hipHostFree(k->hostStat);
@@ -1126,6 +1143,20 @@ int swe_hip_batch_set_surrogate(swe_hip_batch* k, swe_hip_mlp* m) {
     HIP_TRY(hipMemcpy(&k->dG->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
     k->mlp = frags;
     free_graph(k);  // the captured sweep launches the other instantiation
+    if (m && !k->deferQ) {
+        // queue for the fused sweep's deferred edges: one entry per edge it
+        // solves; entries hold x < 2^20 beside the block index (< 2^11)
+        size_t edges = 0;
+        bool fits = k->nblocks <= (1 << (31 - kDeferBlockShift));
+        for (const auto& d : k->desc) {
+            edges += (size_t)(d.nx + 1) * d.ny + (size_t)d.nx * (d.ny + 1);
+            fits = fits && d.nx + 1 < (1 << kDeferBlockShift);
+        }
+        if (fits && edges <= 0xffffffffull) {  // otherwise the sweep solves the hard edges itself
+            HIP_TRY(hipMalloc(&k->deferQ, edges * sizeof(uint2)));
+            k->deferCap = (unsigned)edges;
+        }
+    }
     return SWE_HIP_OK;
 }
 

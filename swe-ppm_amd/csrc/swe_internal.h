@@ -84,6 +84,10 @@ int launch_batch_sweep_solver(int xs, int groups, BatchArena A, const BatchBlock
                               BatchGlobals* G, const int* items, hipStream_t stream);
 template <int S>
 int launch_batch_fused_solver(int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
-                              BatchBlockState* st, BatchGlobals* G, const int* items, hipStream_t stream);
+                              BatchBlockState* st, BatchGlobals* G, const int* items, uint2* deferQ,
+                              hipStream_t stream);
+// The edges the deferring batched sweep queued (G->deferCount of them).
+int launch_batch_defer_solve(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, BatchGlobals* G,
+                             const uint2* queue, unsigned cap, int groups, hipStream_t stream);
 
 }  // namespace swe
