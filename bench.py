@@ -365,8 +365,10 @@ def main() -> int:
 
         weights = _sur.DEFAULT_WEIGHTS
     if args.batched_only == "surrogate":
-        print(json.dumps({"trench": surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps),
-                          "islands": surrogate_islands(swe, scenarios, weights, steps=args.steps)}))
+        trench = surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps)
+        islands = surrogate_islands(swe, scenarios, weights, steps=args.steps)
+        ens = batched_surrogate_ensemble(swe, scenarios, weights, islands["plain"]["value"])
+        print(json.dumps({"trench": trench, "islands": islands, "batched_ensemble_512": ens}))
         return 0
     if args.batched_only:
         if args.batched_only == "lts":

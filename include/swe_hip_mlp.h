@@ -105,7 +105,12 @@ int swe_hip_mlp_solve_edges(swe_hip_mlp* m, int64_t n, const float* hL, const fl
 /* Attach the network to an AugRie block or batch (NULL detaches): its steps
  * then use the learned middle state.  The network must outlive the
  * attachment; later parameter changes take effect at once.  Needs the IEEE
- * arithmetic variant. */
+ * arithmetic variant.  The fused device loops (the default strategy of
+ * swe_hip_run and of the batch) defer the edges whose linearised guess fails
+ * its residual check to a second launch per step that runs the network there,
+ * with the same results as solving them inside the sweep; for that a queue of
+ * 8 bytes per edge of the block (batch: of all blocks) is allocated here (the
+ * batch) or at the first fused run (a block) and kept until destruction. */
 int swe_hip_set_surrogate(swe_hip_block* blk, swe_hip_mlp* m);
 int swe_hip_batch_set_surrogate(swe_hip_batch* bt, swe_hip_mlp* m);
 
