@@ -63,6 +63,11 @@ public:
     // initScenario of the file-driven scenario with the sampling done on the
     // device (swe_hip_init_grid_scenario); the same arrays as initScenario
     void initScenarioOnDevice(const SWE_NetCdfScenario& scenario, BoundaryType boundaries[]);
+    // initScenario of the analytic scenario `scenario` (SWE_SCENARIO_* of
+    // swe_hip.h, the classes of SWE_simple_scenarios.hh) sampled on the device
+    // (swe_hip_init_scenario); the same arrays as initScenario with that
+    // scenario object
+    void initScenarioOnDevice(int scenario, BoundaryType boundaries[]);
 
     // in-rank neighbours (SWE_DimensionalSplittingMPIOverdecomp.cpp:119-139)
     void connectLocalNeighbours(std::array<SWE_DimensionalSplittingHip*, 4> neighbourBlocks);

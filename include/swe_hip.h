@@ -118,6 +118,29 @@ typedef struct swe_hip_grid {
  * reference: it replaces the host loop of initScenario for this scenario. */
 int swe_hip_init_grid_scenario(swe_hip_block* blk, const swe_hip_grid* bathymetry, const swe_hip_grid* displacement);
 
+/* The analytic scenarios of include/swe/scenarios/SWE_simple_scenarios.hh
+ * (the reference's SWE_simple_scenarios.hh:37-285 and the centred flat-bed
+ * dam break), for swe_hip_init_scenario. */
+#define SWE_SCENARIO_HALF_DOMAIN_DRY 0
+#define SWE_SCENARIO_RADIAL_DAM_BREAK 1
+#define SWE_SCENARIO_RADIAL_BATHYMETRY_DAM_BREAK 2
+#define SWE_SCENARIO_BATHYMETRY_DAM_BREAK 3
+#define SWE_SCENARIO_SEA_AT_REST 4
+#define SWE_SCENARIO_SPLASHING_POOL 5
+#define SWE_SCENARIO_SPLASHING_CONE 6
+#define SWE_SCENARIO_CENTRED_DAM_BREAK 7
+
+/* initScenario (SWE_Block.hh:650-682) of one of the analytic scenarios above,
+ * sampled on the device: every cell, ghosts included, at the same sample
+ * point as swe_hip_init_grid_scenario, gets the scenario class's
+ * getBathymetry / getWaterHeight evaluated in single precision with the same
+ * operations (hu = hv = 0 * h), then applyBoundaryConditions and
+ * applyBoundaryBathymetry with the block's boundary types (set them first).
+ * The result equals the host loop over the scenario object bit for bit.  No
+ * counterpart entry in the reference: it replaces the host loop of
+ * initScenario for these scenarios. */
+int swe_hip_init_scenario(swe_hip_block* blk, int scenario);
+
 /* computeNumericalFluxes (flavour U: SWE_DimensionalSplittingMpi.cpp:383-456;
  * flavour S: SWE_DimensionalSplitting.cpp:105-179).  Solves every edge,
  * keeps the net updates on the device and returns the block's CFL time step

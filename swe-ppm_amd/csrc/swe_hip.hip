@@ -336,6 +336,68 @@ __global__ void grid_scenario_kernel(StateView v, float ox, float oy, GridDev ba
     const_cast<float*>(v.b)[c] = bv + dv;
 }
 
+// The analytic scenarios (SWE_simple_scenarios.hh:37-285 of the reference, as
+// restated in include/swe/scenarios/SWE_simple_scenarios.hh): the same
+// single-precision operations as the host classes, so the device samples
+// equal the host loop's.
+__device__ __forceinline__ float scen_radius(float x, float y, float cx, float cy) {
+    return sqrtf((x - cx) * (x - cx) + (y - cy) * (y - cy));
+}
+
+__global__ void analytic_scenario_kernel(StateView v, float ox, float oy, int scenario) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)(v.nx + 2) * v.pitch;
+    if (c >= total) return;
+    const int i = (int)(c / v.pitch), j = (int)(c % v.pitch);
+    const float x = (float)((double)ox + ((double)i - 0.5) * (double)v.dx);
+    const float y = (float)((double)oy + ((double)j - 0.5) * (double)v.dy);
+    float b = 0.f, h = 0.f;
+    switch (scenario) {
+        case SWE_SCENARIO_HALF_DOMAIN_DRY:
+            b = -250.f;
+            h = scen_radius(x, y, 250.f, 250.f) < 100.f ? 15.f : 10.f;
+            break;
+        case SWE_SCENARIO_RADIAL_DAM_BREAK:
+            b = 10.f;
+            h = scen_radius(x, y, 250.f, 250.f) < 100.f ? 15.f : 10.f;
+            break;
+        case SWE_SCENARIO_RADIAL_BATHYMETRY_DAM_BREAK:
+            b = scen_radius(x, y, 1000.f, 1000.f) < 300.f ? 20.f : 10.f;
+            h = scen_radius(x, y, 500.f, 500.f) < 300.f ? 40.f : 15.f;
+            break;
+        case SWE_SCENARIO_BATHYMETRY_DAM_BREAK:
+            b = scen_radius(x, y, 500.f, 500.f) < 50.f ? -255.f : -260.f;
+            h = 260.f;
+            break;
+        case SWE_SCENARIO_SEA_AT_REST: {
+            const bool inner = scen_radius(x, y, 0.5f, 0.5f) < 0.1f;
+            h = inner ? 9.9f : 10.0f;
+            b = inner ? 0.1f : 0.0f;
+            break;
+        }
+        case SWE_SCENARIO_SPLASHING_POOL:
+            b = -250.f;
+            h = 250.0f + (5.0f - (x + y) / 200.f);
+            break;
+        case SWE_SCENARIO_SPLASHING_CONE: {
+            const float r = scen_radius(x, y, 0.5f, 0.5f);
+            float hh = 4.0f - 4.5f * (r / 0.5f);
+            if (r < 0.1f) hh = hh + 1.0f;
+            h = (hh > 0.0f) ? hh : 0.0f;
+            b = 1.0f + 9.0f * ((r < 0.5f) ? r : 0.5f);
+            break;
+        }
+        default:  // SWE_SCENARIO_CENTRED_DAM_BREAK
+            b = 0.f;
+            h = scen_radius(x, y, 500.f, 500.f) < 100.f ? 15.f : 10.f;
+            break;
+    }
+    v.h[c] = h;
+    v.hu[c] = 0.f * h;
+    v.hv[c] = 0.f * h;
+    const_cast<float*>(v.b)[c] = b;
+}
+
 bool grid_ok(const swe_hip_grid* g) {
     return g && g->values && g->nx >= 1 && g->ny >= 1 && g->dx > 0.0 && g->dy > 0.0;
 }
@@ -510,6 +572,23 @@ int swe_hip_init_grid_scenario(swe_hip_block* k, const swe_hip_grid* bathymetry,
     if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
     (void)hipFree(dev);
     if (e != hipSuccess) return hip_fail(e, "swe_hip_init_grid_scenario");
+    // applyBoundaryConditions, then applyBoundaryBathymetry (SWE_Block.hh:679-681)
+    if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
+    if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_init_scenario(swe_hip_block* k, int scenario) {
+    if (int r = check(k)) return r;
+    if (scenario < SWE_SCENARIO_HALF_DOMAIN_DRY || scenario > SWE_SCENARIO_CENTRED_DAM_BREAK) {
+        set_error("swe_hip_init_scenario: unknown scenario %d", scenario);
+        return SWE_HIP_EINVAL;
+    }
+    const long total = (long)field_elems(k);
+    hipLaunchKernelGGL(analytic_scenario_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, k->stream,
+                       view(k, k->cur), k->ox, k->oy, scenario);
+    HIP_TRY(hipGetLastError());
     // applyBoundaryConditions, then applyBoundaryBathymetry (SWE_Block.hh:679-681)
     if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
     if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;

@@ -63,6 +63,12 @@ void SWE_DimensionalSplittingHip::initScenarioOnDevice(const SWE_NetCdfScenario&
     hostStale_ = true;  // the host arrays are filled on the next read
 }
 
+void SWE_DimensionalSplittingHip::initScenarioOnDevice(int scenario, BoundaryType boundaries[]) {
+    for (int s = 0; s < 4; s++) setBoundaryType(static_cast<Boundary>(s), boundaries[s]);
+    check(swe_hip_init_scenario(dev_, scenario), "swe_hip_init_scenario");
+    hostStale_ = true;  // the host arrays are filled on the next read
+}
+
 void SWE_DimensionalSplittingHip::setBoundaryType(Boundary boundary, BoundaryType type) {
     boundaryType[boundary] = type;
     // CONNECT and CONNECT_WITHIN_RANK sides are filled by neighbour exchange;

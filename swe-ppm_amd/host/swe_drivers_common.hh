@@ -35,7 +35,7 @@ struct DriverArgs {
     std::string writer;   // --writer vtk|netcdf ("" = the driver's default)
     std::string bathymetry;    // --bathymetry-file: file-driven scenario
     std::string displacement;  // --displacement-file
-    bool hostInit = false;     // --host-init: sample the file-driven scenario on the host
+    bool hostInit = false;     // --host-init: sample the scenario on the host (initScenario), not the device
 };
 
 inline int solver_from_name(const std::string& s) {

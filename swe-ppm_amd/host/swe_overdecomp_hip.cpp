@@ -241,7 +241,9 @@ int main(int argc, char** argv) {
                 allBnd[4 * k + s] = bnd[s];
                 allNbr[4 * k + s] = nb[s];
             }
-            if (init.empty())
+            if (init.empty() && !a.hostInit)
+                blocks[k]->initScenarioOnDevice(SWE_SCENARIO_RADIAL_BATHYMETRY_DAM_BREAK, bnd);
+            else if (init.empty())
                 blocks[k]->initScenario(*scenario, bnd);
             else
                 SnapshotScenarioLoader::load(*blocks[k], init[k], bnd);

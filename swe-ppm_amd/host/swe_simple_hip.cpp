@@ -24,13 +24,24 @@
 #include "writer/NetCdfWriter.hh"
 #include "writer/VtkWriter.hh"
 
-static std::unique_ptr<SWE_Scenario> make_scenario(const std::string& name) {
-    if (name == "radial") return std::unique_ptr<SWE_Scenario>(new SWE_RadialDamBreakScenario());
-    if (name == "centred") return std::unique_ptr<SWE_Scenario>(new SWE_CentredDamBreakScenario());
-    if (name == "radial-bathymetry") return std::unique_ptr<SWE_Scenario>(new SWE_RadialBathymetryDamBreakScenario());
-    if (name == "bathymetry") return std::unique_ptr<SWE_Scenario>(new SWE_BathymetryDamBreakScenario());
-    if (name == "half-dry") return std::unique_ptr<SWE_Scenario>(new SWE_HalfDomainDry());
-    if (name == "splashing-pool") return std::unique_ptr<SWE_Scenario>(new SWE_SplashingPoolScenario());
+// the scenario object and its device-sampling id (SWE_SCENARIO_*)
+static std::unique_ptr<SWE_Scenario> make_scenario(const std::string& name, int* id) {
+    if (name == "radial") return *id = SWE_SCENARIO_RADIAL_DAM_BREAK, std::unique_ptr<SWE_Scenario>(new SWE_RadialDamBreakScenario());
+    if (name == "centred")
+        return *id = SWE_SCENARIO_CENTRED_DAM_BREAK, std::unique_ptr<SWE_Scenario>(new SWE_CentredDamBreakScenario());
+    if (name == "radial-bathymetry")
+        return *id = SWE_SCENARIO_RADIAL_BATHYMETRY_DAM_BREAK,
+               std::unique_ptr<SWE_Scenario>(new SWE_RadialBathymetryDamBreakScenario());
+    if (name == "bathymetry")
+        return *id = SWE_SCENARIO_BATHYMETRY_DAM_BREAK,
+               std::unique_ptr<SWE_Scenario>(new SWE_BathymetryDamBreakScenario());
+    if (name == "half-dry") return *id = SWE_SCENARIO_HALF_DOMAIN_DRY, std::unique_ptr<SWE_Scenario>(new SWE_HalfDomainDry());
+    if (name == "splashing-pool")
+        return *id = SWE_SCENARIO_SPLASHING_POOL, std::unique_ptr<SWE_Scenario>(new SWE_SplashingPoolScenario());
+    if (name == "splashing-cone")
+        return *id = SWE_SCENARIO_SPLASHING_CONE, std::unique_ptr<SWE_Scenario>(new SWE_SplashingConeScenario());
+    if (name == "sea-at-rest")
+        return *id = SWE_SCENARIO_SEA_AT_REST, std::unique_ptr<SWE_Scenario>(new SWE_SeaAtRestScenario());
     throw std::invalid_argument("unknown scenario " + name);
 }
 
@@ -47,11 +58,12 @@ int main(int argc, char** argv) {
         DriverArgs a = parse_args((int)args.size(), args.data(), "radial");
         std::unique_ptr<SWE_Scenario> scenario;
         SWE_NetCdfScenario* fileScenario = nullptr;
+        int scenarioId = -1;
         if (!a.bathymetry.empty()) {
             fileScenario = new SWE_NetCdfScenario(a.bathymetry, a.displacement);
             scenario.reset(fileScenario);
         } else {
-            scenario = make_scenario(a.scenario);
+            scenario = make_scenario(a.scenario, &scenarioId);
         }
         std::vector<float> cp = checkpoint_instants(a.duration, a.checkpoints);
 
@@ -65,8 +77,11 @@ int main(int argc, char** argv) {
         for (int s = 0; s < 4; s++) boundaries[s] = scenario->getBoundaryType(static_cast<Boundary>(s));
 
         SWE_DimensionalSplittingHip sim(a.nx, a.ny, dx, dy, ox, oy, false, a.solver, a.flavour);
+        // sampled on the device unless --host-init asks for the host loop
         if (fileScenario && !a.hostInit)
             sim.initScenarioOnDevice(*fileScenario, boundaries);
+        else if (scenarioId >= 0 && !a.hostInit)
+            sim.initScenarioOnDevice(scenarioId, boundaries);
         else
             sim.initScenario(*scenario, boundaries);
         if (!a.dump0.empty()) write_snapshot(a.dump0, {snapshot_of(sim)});

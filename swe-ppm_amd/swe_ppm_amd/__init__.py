@@ -21,6 +21,10 @@ LIB_PATH = PKG_ROOT / "libswe_hip.so"
 
 # solvers (WAVE_PROPAGATION_SOLVER ids of SWE_DimensionalSplittingMpi.hh:42-50, + Hybrid)
 HLLE, FWAVE, AUGRIE, HYBRID = 0, 1, 2, 3
+# analytic scenarios of swe_hip_init_scenario (include/swe_hip.h)
+(SCENARIO_HALF_DOMAIN_DRY, SCENARIO_RADIAL_DAM_BREAK, SCENARIO_RADIAL_BATHYMETRY_DAM_BREAK,
+ SCENARIO_BATHYMETRY_DAM_BREAK, SCENARIO_SEA_AT_REST, SCENARIO_SPLASHING_POOL, SCENARIO_SPLASHING_CONE,
+ SCENARIO_CENTRED_DAM_BREAK) = range(8)
 SOLVER_NAMES = {"hlle": HLLE, "fwave": FWAVE, "augrie": AUGRIE, "hybrid": HYBRID}
 # step flavours
 FLAVOUR_U, FLAVOUR_S = 0, 1
@@ -60,6 +64,7 @@ def _load() -> C.CDLL:
         "swe_hip_apply_bc": (C.c_int, [vp]),
         "swe_hip_apply_bc_bathymetry": (C.c_int, [vp]),
         "swe_hip_init_grid_scenario": (C.c_int, [vp, gp, gp]),
+        "swe_hip_init_scenario": (C.c_int, [vp, C.c_int]),
         "swe_hip_fluxes": (C.c_int, [vp, fp]),
         "swe_hip_update": (C.c_int, [vp, C.c_float]),
         "swe_hip_run": (C.c_int, [vp, C.c_int, fp, fp]),
@@ -184,6 +189,12 @@ class SweBlock:
         gd = grid(displacement) if displacement is not None else None
         _check(lib.swe_hip_init_grid_scenario(self._h, C.byref(gb), C.byref(gd) if gd is not None else None),
                "init_grid_scenario")
+
+    def init_scenario(self, scenario: int) -> None:
+        """One of the analytic scenarios (SCENARIO_*) sampled on the device
+        (swe_hip_init_scenario).  Set the boundaries first; they are applied
+        afterwards."""
+        _check(lib.swe_hip_init_scenario(self._h, int(scenario)), "init_scenario")
 
     def download(self, with_b: bool = False):
         out = [np.empty(self.shape, np.float32) for _ in range(4 if with_b else 3)]

@@ -188,6 +188,34 @@ def test_file_scenario_sampled_on_device_equals_host_loop(oracle, tmp_path):
         assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1])
 
 
+ANALYTIC = ["radial", "centred", "radial-bathymetry", "bathymetry", "half-dry", "splashing-pool", "splashing-cone",
+            "sea-at-rest"]
+
+
+def test_analytic_scenarios_sampled_on_device_equal_host_loop(tmp_path):
+    """swe_hip_init_scenario against SWE_Block::initScenario over the same
+    scenario objects (ghosts and boundary conditions included), bit for bit:
+    every analytic scenario of the single-block driver, and the six blocks at
+    their own origins of the overdecomposed driver."""
+    for name in ANALYTIC:
+        dev, host = tmp_path / f"{name}_dev.bin", tmp_path / f"{name}_host.bin"
+        common = [BIN / "swe_simple_hip", "-t", "1", "-n", "1", "-x", "70", "-y", "45", "--solver", "augrie",
+                  "--scenario", name, "--max-steps", "1"]
+        run(common + ["--dump-initial", dev])
+        run(common + ["--host-init", "--dump-initial", host])
+        d, h = snapshots.read(dev)[0], snapshots.read(host)[0]
+        for k in ("h", "hu", "hv", "b"):
+            assert np.array_equal(d[k].view(np.uint32), h[k].view(np.uint32)), (name, k)
+    dev, host = tmp_path / "od_dev.bin", tmp_path / "od_host.bin"
+    common = [BIN / "swe_overdecomp_hip", "-t", "1000", "-n", "1", "-x", "150", "-y", "100", "-b", "6", "--solver",
+              "augrie", "--max-steps", "1"]
+    run(common + ["--dump-initial", dev])
+    run(common + ["--host-init", "--dump-initial", host])
+    for k, (d, h) in enumerate(zip(snapshots.read(dev), snapshots.read(host))):
+        for f in ("h", "hu", "hv", "b"):
+            assert np.array_equal(d[f].view(np.uint32), h[f].view(np.uint32)), ("block", k, f)
+
+
 def _summary(out):
     upd = [ln for ln in out.splitlines() if ln.startswith("updates ")][0].split()[1:]
     head = [ln for ln in out.splitlines() if ln.startswith("blocks ")][0].split()

@@ -191,6 +191,22 @@ def test_default_strip_width_follows_the_wave_count(swe):
     net.close()
 
 
+def test_analytic_scenario_known_answer_and_errors(swe):
+    """swe_hip_init_scenario on the radial dam break over [0, 1000]^2 (15 m
+    within 100 m of (250, 250), 10 m elsewhere, b = 10, OUTFLOW): cell
+    centres decide, ghosts copy their neighbours; unknown ids fail loudly."""
+    with swe.SweBlock(10, 10, 100.0, 100.0) as blk:
+        blk.set_boundaries([0, 0, 0, 0])
+        blk.init_scenario(swe.SCENARIO_RADIAL_DAM_BREAK)
+        h, hu, hv, b = blk.download(with_b=True)
+        want = np.full((12, 12), 10.0, np.float32)
+        want[3, 3] = 15.0  # the only centre within 100 m: (250, 250) is the centre of cell (3, 3)
+        assert np.array_equal(h, want)
+        assert np.all(b == 10.0) and np.all(hu == 0.0) and np.all(hv == 0.0)
+        with pytest.raises(swe.SweHipError):
+            blk.init_scenario(8)
+
+
 def test_grid_scenario_known_answer_and_errors(swe):
     """swe_hip_init_grid_scenario on a 2 x 2 bathymetry grid of 50 m cells
     over [0, 100]^2, sampled by a 4 x 4 block of 25 m cells: each quarter of
