@@ -270,10 +270,10 @@ def solver_lines(swe, state, steps: int = 200):
 
 
 def batched_surrogate_ensemble(swe, scenarios, weights, plain_value: float, count: int = 512, seed: int = 512,
-                               steps: int = 100):
+                               steps: int = 100, autotune: bool = True):
     """The 512-instance ensemble through the batched engine with the learned
-    middle state attached (same instances and tuning defaults as the exact
-    ensemble line), against the plain 2048^2 line."""
+    middle state attached (same instances as the exact ensemble line, tuned
+    the same way), against the plain 2048^2 line."""
     from swe_ppm_amd import surrogate
 
     inst = scenarios.ensemble_instances(count=count, seed=seed)
@@ -282,10 +282,13 @@ def batched_surrogate_ensemble(swe, scenarios, weights, plain_value: float, coun
     with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
         bt.set_surrogate(mlp)
         bt.upload(*fields)
+        if autotune:
+            bt.autotune(iterations=16)
         bt.run(5)  # warm-up
         bt.upload(*fields)
         _, _, ms = bt.run(steps)
         h = bt.download(packed=True)[0]
+        tuning = bt.get_tuning()
     mlp.close()
     if not np.all(np.isfinite(h)):
         raise RuntimeError("non-finite ensemble state (learned middle state)")
@@ -294,7 +297,8 @@ def batched_surrogate_ensemble(swe, scenarios, weights, plain_value: float, coun
     return {"workload": f"the {count}-instance ensemble (seed {seed}), AugRie FP32 with the learned middle state, "
                         f"{steps} steps each",
             "value": round(value, 3), "unit": "MCell-updates/s", "elapsed_ms": round(ms, 3),
-            "vs_plain": round(value / plain_value, 4), "weights": str(Path(weights).name)}
+            "vs_plain": round(value / plain_value, 4), "weights": str(Path(weights).name),
+            "tuning": dict(zip(("xs", "groups", "chunk", "graph", "fused"), tuning))}
 
 
 def surrogate_trench(swe, scenarios, weights, n: int = 4096, steps: int = 1000, seed: int = 4096):
@@ -367,7 +371,8 @@ def main() -> int:
     if args.batched_only == "surrogate":
         trench = surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps)
         islands = surrogate_islands(swe, scenarios, weights, steps=args.steps)
-        ens = batched_surrogate_ensemble(swe, scenarios, weights, islands["plain"]["value"])
+        ens = batched_surrogate_ensemble(swe, scenarios, weights, islands["plain"]["value"],
+                                         autotune=not args.no_autotune)
         print(json.dumps({"trench": trench, "islands": islands, "batched_ensemble_512": ens}))
         return 0
     if args.batched_only:
@@ -494,7 +499,8 @@ def main() -> int:
             line["surrogate"] = {
                 "trench": surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps),
                 "islands": surrogate_islands(swe, scenarios, weights, n=n, steps=args.steps, seed=args.seed),
-                "batched_ensemble_512": batched_surrogate_ensemble(swe, scenarios, weights, mcells)}
+                "batched_ensemble_512": batched_surrogate_ensemble(swe, scenarios, weights, mcells,
+                                                                   autotune=not args.no_autotune)}
         else:
             line["surrogate"] = {"skipped": f"no trained network at {weights}"}
     print(json.dumps(line))
