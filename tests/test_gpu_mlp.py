@@ -119,8 +119,9 @@ def _held_out_edges(oracle):
     return [a[sel].astype(np.float32) for a in (hL, hR, uL, uR, bL, bR)], iters[sel]
 
 
-@pytest.mark.parametrize("nx,ny,xseg", [(256, 256, 8), (130, 77, 16), (200, 150, 32)])
-def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx, ny, xseg):
+@pytest.mark.parametrize("nx,ny,xseg,walls", [(256, 256, 8, False), (130, 77, 16, False), (200, 150, 32, False),
+                                              (190, 140, 16, True)])
+def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx, ny, xseg, walls):
     """With a network attached, the fused loop settles the edges whose
     linearised guess passes the residual check inside the sweep and queues
     the others for a second launch (network, check, exact fallback); the store
@@ -130,6 +131,8 @@ def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx
     net = surrogate.load_default()
     h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=max(nx, ny), seed=7)
     h, hu, hv, b = (a[:nx + 2, :ny + 2].copy() for a in (h, hu, hv, b))
+    if walls:  # reflecting sides: the deferred edges at the walls read the mirrored ghosts
+        bnd = [swe.WALL] * 4
     out = {}
     for strategy in (1, 2):
         with swe.SweBlock(nx, ny, dx, dy, solver=swe.AUGRIE) as blk:
