@@ -181,7 +181,8 @@ __device__ __forceinline__ float wave_max(float v) {
 // the first strip); rows 1..ny only for x-edges.  A deferred edge contributes
 // nothing in the sweep (zero parts, zero speed); a second launch adds its
 // parts to the (two-term) sums and its speed to the CFL maximum, so every sum
-// ends as the same two-term IEEE sum.
+// ends as the same two-term IEEE sum (a zero sum may differ in its sign: the
+// sweep stores carry + 0 in place of carry when the other term is deferred).
 constexpr unsigned kDeferY = 0x80000000u;
 constexpr int kDeferBlockShift = 20;
 
