@@ -179,10 +179,11 @@ __device__ __forceinline__ float wave_max(float v) {
 // lane that owns it: x-edges by the wave whose segment holds x (edge (0|1) by
 // the first segment), y-edges by the lane of row y (edge (0|1) by lane 0 of
 // the first strip); rows 1..ny only for x-edges.  A deferred edge contributes
-// nothing in the sweep (zero parts, zero speed); a second launch adds its
-// parts to the (two-term) sums and its speed to the CFL maximum, so every sum
-// ends as the same two-term IEEE sum (a zero sum may differ in its sign: the
-// sweep stores carry + 0 in place of carry when the other term is deferred).
+// nothing in the sweep: its parts are -0 (x + -0 == x for every x, zeros of
+// both signs included, so the sweep stores exactly the other term of each
+// two-term sum) and its speed 0.  A second launch adds its parts to the sums
+// and its speed to the CFL maximum; IEEE addition is commutative, so every sum
+// ends bit for bit as when the edge is solved in the sweep.
 constexpr unsigned kDeferY = 0x80000000u;
 constexpr int kDeferBlockShift = 20;
 
@@ -198,7 +199,7 @@ __device__ __forceinline__ dev::EdgeOut defer_edge(const dev::EdgeOut& e, bool o
             __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         if (d && owns) q[base + below] = entry;
     }
-    return d ? dev::EdgeOut{0.f, 0.f, 0.f, 0.f, 0.f} : e;
+    return d ? dev::EdgeOut{-0.f, -0.f, -0.f, -0.f, 0.f} : e;
 }
 
 // True for the ghost cells that the fused boundary write refreshes after an
