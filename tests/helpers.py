@@ -4,6 +4,16 @@ from __future__ import annotations
 import numpy as np
 
 
+def same_bits(a, b) -> bool:
+    """Equality for the parity tests: float32 arrays compare by their bit
+    patterns (so +0 and -0 differ), other arrays by value."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype == np.float32 and b.dtype == np.float32:
+        return a.shape == b.shape and np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                                                     np.ascontiguousarray(b).view(np.uint32))
+    return np.array_equal(a, b)
+
+
 def rel_linf(a: np.ndarray, b: np.ndarray) -> float:
     """max |a - b| / max |b| over the interior (ghost cells excluded)."""
     ai = np.asarray(a, np.float64)[1:-1, 1:-1]

@@ -11,6 +11,8 @@ leaves the CONNECT_WITHIN_RANK ghost lines as the oracle left them.
 import numpy as np
 import pytest
 
+from helpers import same_bits
+
 pytestmark = pytest.mark.gpu
 
 
@@ -25,9 +27,9 @@ def _assert_blocks_equal(oracle, got, want, blocks, what):
     for k, bl in enumerate(blocks):
         oh, ohu, ohv = _with_bc(oracle, want[k], bl["bnd"])
         for name, a, c in (("h", gh[k], oh), ("hu", ghu[k], ohu), ("hv", ghv[k], ohv)):
-            assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), (
+            assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), (
                 f"{what}: block {k} {name} interior differs (max {np.max(np.abs(a - c)):.3e})")
-            assert np.array_equal(a, c), f"{what}: block {k} {name} ghost cells differ"
+            assert same_bits(a, c), f"{what}: block {k} {name} ghost cells differ"
 
 
 # groups: -1 the default (the direct form), 0 direct, > 0 a persistent grid of that many workgroups
@@ -87,8 +89,8 @@ def test_global_dt_blocks_match_oracle(swe, oracle, solver, fused, groups):
         got = bt.download()
         upd, _ = bt.stats()
     states, oupd, oits, odts = oracle.multiblock_run(blocks, dx, dy, solver, False, 1.0, 0.0, nsteps, nthreads=8)
-    assert np.array_equal(dts, odts)
-    assert np.array_equal(upd, oupd)
+    assert same_bits(dts, odts)
+    assert same_bits(upd, oupd)
     _assert_blocks_equal(oracle, got, states, blocks, "global dt")
 
 
@@ -110,8 +112,8 @@ def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro, fused, groups
         upd, _ = bt.stats()
     states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, T, duration, nmacro,
                                                   nthreads=8)
-    assert np.array_equal(its, oits), (its, oits)
-    assert np.array_equal(upd, oupd)
+    assert same_bits(its, oits), (its, oits)
+    assert same_bits(upd, oupd)
     assert len(set(upd.tolist())) > 1, "blocks must take different numbers of sub-steps"
     _assert_blocks_equal(oracle, got, states, blocks, f"LTS T={T}")
 
@@ -131,7 +133,7 @@ def test_lts_macro_steps_split_over_calls_and_autotune(swe, oracle):
         _, its2, _ = bt.run(2)
         got = bt.download()
     states, _, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 0.5, 10.0, 3, nthreads=8)
-    assert np.array_equal(np.concatenate([its1, its2]), oits)
+    assert same_bits(np.concatenate([its1, its2]), oits)
     _assert_blocks_equal(oracle, got, states, blocks, "LTS split runs")
 
 
@@ -216,9 +218,9 @@ def test_connected_edge_shapes_match_oracle(swe, oracle, lts):
         upd, _ = bt.stats()
     states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, lts, T, duration if lts else 0.0,
                                                   units if lts else 20, nthreads=8)
-    assert np.array_equal(upd, oupd)
+    assert same_bits(upd, oupd)
     if lts:
-        assert np.array_equal(its, oits)
+        assert same_bits(its, oits)
     _assert_blocks_equal(oracle, got, states, blocks, f"connected edge shapes lts={lts}")
 
 
@@ -268,7 +270,7 @@ def test_lts_with_more_blocks_than_round_threads(swe, oracle):
         got = bt.download()
         upd, _ = bt.stats()
     states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 0.5, 1.0, 2, nthreads=8)
-    assert np.array_equal(its, oits)
-    assert np.array_equal(upd, oupd)
+    assert same_bits(its, oits)
+    assert same_bits(upd, oupd)
     assert len(set(upd.tolist())) > 1
     _assert_blocks_equal(oracle, got, states, blocks, "1280 LTS blocks")

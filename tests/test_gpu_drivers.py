@@ -8,6 +8,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from helpers import same_bits
+
 import snapshots
 
 pytestmark = pytest.mark.gpu
@@ -42,7 +44,7 @@ def test_swe_simple_driver_matches_oracle(oracle, tmp_path, solver, flavour):
     else:
         oh, ohu, ohv, _ = oracle.run_s(s0["h"], s0["hu"], s0["hv"], s0["b"], np.float32(dx), bnd, sid, 40)
     for a, c in ((s1["h"], oh), (s1["hu"], ohu), (s1["hv"], ohv)):
-        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1])
+        assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1])
     # the VTK writer produced the t=0 and the checkpoint files
     vts = sorted(tmp_path.glob("vtk.*.vts"))
     assert len(vts) == 2 and "<DataArray Name=\"h\"" in vts[1].read_text()
@@ -73,7 +75,7 @@ def test_overdecomp_global_matches_oracle(oracle, tmp_path):
                                               1.0, 1e30, 30)
     for k, (g, o) in enumerate(zip(s1, res)):
         for a, c in zip((g["h"], g["hu"], g["hv"]), o[:3]):
-            assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), f"block {k}"
+            assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), f"block {k}"
 
 
 def test_overdecomp_lts_matches_oracle(oracle, tmp_path):
@@ -104,7 +106,7 @@ def test_overdecomp_lts_matches_oracle(oracle, tmp_path):
     assert gupd == [int(u) for u in upd], (gupd, upd)
     for k, (g, o) in enumerate(zip(s1, res)):
         for a, c in zip((g["h"], g["hu"], g["hv"]), o[:3]):
-            assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), f"block {k}"
+            assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), f"block {k}"
 
 
 def _assert_nc_matches(path, state, nrec, t_last):
@@ -118,8 +120,8 @@ def _assert_nc_matches(path, state, nrec, t_last):
         assert np.float32(v["time"].data[0]) == 0.0
         assert abs(float(v["time"].data[-1]) - t_last) <= 1e-6 * max(1.0, t_last)  # printed with %f
         for name, key in (("ah", "h"), ("hu", "hu"), ("hv", "hv")):
-            assert np.array_equal(v[name].data[-1], state[key][1:-1, 1:-1].T), name
-        assert np.array_equal(v["b"].data, state["b"][1:-1, 1:-1].T)
+            assert same_bits(v[name].data[-1], state[key][1:-1, 1:-1].T), name
+        assert same_bits(v["b"].data, state["b"][1:-1, 1:-1].T)
 
 
 def test_swe_simple_driver_writes_netcdf(tmp_path):
@@ -178,14 +180,14 @@ def test_file_scenario_sampled_on_device_equals_host_loop(oracle, tmp_path):
     run([BIN / "swe_simple_hip"] + common + ["--max-steps", "1", "--host-init", "--dump-initial", host0])
     d0, h0 = snapshots.read(dev0)[0], snapshots.read(host0)[0]
     for k in ("h", "hu", "hv", "b"):
-        assert np.array_equal(d0[k].view(np.uint32), h0[k].view(np.uint32)), k
+        assert same_bits(d0[k].view(np.uint32), h0[k].view(np.uint32)), k
     assert (d0["h"] > 0).any() and (d0["h"] == 0).any()  # sea and dry land
-    assert not np.array_equal(d0["b"] + d0["h"], np.zeros_like(d0["b"]))
+    assert not same_bits(d0["b"] + d0["h"], np.zeros_like(d0["b"]))
     d1 = snapshots.read(dev1)[0]
     oh, ohu, ohv, _ = oracle.run_u(d0["h"], d0["hu"], d0["hv"], d0["b"], np.float32(4000 / 60), np.float32(2000 / 40),
                                    [0, 0, 0, 0], 2, 20)
     for a, c in ((d1["h"], oh), (d1["hu"], ohu), (d1["hv"], ohv)):
-        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1])
+        assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1])
 
 
 ANALYTIC = ["radial", "centred", "radial-bathymetry", "bathymetry", "half-dry", "splashing-pool", "splashing-cone",
@@ -205,7 +207,7 @@ def test_analytic_scenarios_sampled_on_device_equal_host_loop(tmp_path):
         run(common + ["--host-init", "--dump-initial", host])
         d, h = snapshots.read(dev)[0], snapshots.read(host)[0]
         for k in ("h", "hu", "hv", "b"):
-            assert np.array_equal(d[k].view(np.uint32), h[k].view(np.uint32)), (name, k)
+            assert same_bits(d[k].view(np.uint32), h[k].view(np.uint32)), (name, k)
     dev, host = tmp_path / "od_dev.bin", tmp_path / "od_host.bin"
     common = [BIN / "swe_overdecomp_hip", "-t", "1000", "-n", "1", "-x", "150", "-y", "100", "-b", "6", "--solver",
               "augrie", "--max-steps", "1"]
@@ -213,7 +215,7 @@ def test_analytic_scenarios_sampled_on_device_equal_host_loop(tmp_path):
     run(common + ["--host-init", "--dump-initial", host])
     for k, (d, h) in enumerate(zip(snapshots.read(dev), snapshots.read(host))):
         for f in ("h", "hu", "hv", "b"):
-            assert np.array_equal(d[f].view(np.uint32), h[f].view(np.uint32)), ("block", k, f)
+            assert same_bits(d[f].view(np.uint32), h[f].view(np.uint32)), ("block", k, f)
 
 
 def _summary(out):
@@ -229,9 +231,9 @@ def _assert_same_dumps(p, q):
     update (the next setGhostLayer rewrites them in both)."""
     for k, (g, o) in enumerate(zip(snapshots.read(p), snapshots.read(q))):
         for name in ("h", "hu", "hv"):
-            assert np.array_equal(g[name][1:-1, 1:-1].view(np.uint32), o[name][1:-1, 1:-1].view(np.uint32)), \
+            assert same_bits(g[name][1:-1, 1:-1].view(np.uint32), o[name][1:-1, 1:-1].view(np.uint32)), \
                 f"block {k} {name}"
-        assert np.array_equal(g["b"].view(np.uint32), o["b"].view(np.uint32)), f"block {k} b"
+        assert same_bits(g["b"].view(np.uint32), o["b"].view(np.uint32)), f"block {k} b"
 
 
 @pytest.mark.parametrize("lts", [False, True])

@@ -16,6 +16,8 @@ import os
 import numpy as np
 import pytest
 
+from helpers import same_bits
+
 from test_gpu_batch import _assert_blocks_equal
 
 pytestmark = pytest.mark.gpu
@@ -43,9 +45,9 @@ def test_islands_2048_bitwise_and_mass(swe, oracle):
     print("2048^2 islands, 100 steps GPU and oracle", flush=True)
     (gh, ghu, ghv), gdts = _run(swe, st, 100, swe.AUGRIE)
     oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, 100, nthreads=THREADS)
-    assert np.array_equal(np.asarray(gdts, np.float32), odts)
+    assert same_bits(np.asarray(gdts, np.float32), odts)
     for name, a, c in (("h", gh, oh), ("hu", ghu, ohu), ("hv", ghv, ohv)):
-        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
+        assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
     print("2048^2 islands, 500 steps GPU", flush=True)
     (gh5, _, _), _ = _run(swe, st, 500, swe.AUGRIE)
     m0 = scenarios.interior_mass(h, dx, dy)
@@ -62,9 +64,9 @@ def test_trench_4096_bitwise(swe, oracle):
     print("4096^2 trench, 40 steps GPU and oracle", flush=True)
     (gh, ghu, ghv), gdts = _run(swe, st, 40, swe.AUGRIE)
     oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, 40, nthreads=THREADS)
-    assert np.array_equal(np.asarray(gdts, np.float32), odts)
+    assert same_bits(np.asarray(gdts, np.float32), odts)
     for name, a, c in (("h", gh, oh), ("hu", ghu, ohu), ("hv", ghv, ohv)):
-        assert np.array_equal(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
+        assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
 
 
 def test_trench_4096_lake_at_rest(swe):
@@ -93,8 +95,8 @@ def test_lts_tiling_2048_one_macro_step(swe, oracle):
         got = bt.download()
         upd, _ = bt.stats()
     states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 1.0, 80.0, 1, nthreads=THREADS)
-    assert np.array_equal(its, oits)
-    assert np.array_equal(upd, oupd)
+    assert same_bits(its, oits)
+    assert same_bits(upd, oupd)
     _assert_blocks_equal(oracle, got, states, blocks, "LTS 2048^2 tiling")
 
 

@@ -12,6 +12,8 @@ corrected guess passes the residual check, trained against untrained.
 import numpy as np
 import pytest
 
+from helpers import same_bits
+
 import mlp_oracle as M
 
 pytestmark = pytest.mark.gpu
@@ -57,7 +59,7 @@ def test_feature_function_bitwise(swe):
     bL, bR = rng.normal(0, 5, n), rng.normal(0, 5, n)
     got = swe.mlp_features(hL, hR, uL, uR, bL, bR)
     want, _, _ = M.features(hL, hR, uL, uR, bL, bR)
-    assert np.array_equal(got, want)
+    assert same_bits(got, want)
 
 
 def test_logged_samples_are_exact_middle_states(swe, oracle):
@@ -143,8 +145,8 @@ def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx
             _, dts = blk.run(40)
             out[strategy] = (blk.download(), np.asarray(dts))
     for name, a, c in zip(("h", "hu", "hv"), out[1][0], out[2][0]):
-        assert np.array_equal(a.view(np.uint32), c.view(np.uint32)), name  # bit patterns, signed zeros included
-    assert np.array_equal(out[1][1], out[2][1])
+        assert same_bits(a.view(np.uint32), c.view(np.uint32)), name  # bit patterns, signed zeros included
+    assert same_bits(out[1][1], out[2][1])
     net.close()
 
 
@@ -173,9 +175,9 @@ def test_batched_deferred_hard_edges_match_the_sweep_and_single_blocks(swe):
                 got[fused] = (bt.download(), bt.stats()[0])
         for k in range(len(bl)):
             for f, field in enumerate(("h", "hu", "hv")):
-                assert np.array_equal(got[1][0][f][k].view(np.uint32), got[0][0][f][k].view(np.uint32)), (
+                assert same_bits(got[1][0][f][k].view(np.uint32), got[0][0][f][k].view(np.uint32)), (
                     name, k, field)
-        assert np.array_equal(got[1][1], got[0][1]), name
+        assert same_bits(got[1][1], got[0][1]), name
     for k, i in enumerate(inst):
         with swe.SweBlock(i["nx"], i["ny"], i["dx"], i["dy"], solver=swe.AUGRIE) as blk:
             blk.set_boundaries(i["bnd"])
@@ -185,7 +187,7 @@ def test_batched_deferred_hard_edges_match_the_sweep_and_single_blocks(swe):
             blk.run(30, want_dts=False)
             one = blk.download()
         for f, field in enumerate(("h", "hu", "hv")):
-            assert np.array_equal(one[f][1:-1, 1:-1], results["ensemble"][1][0][f][k][1:-1, 1:-1]), (
+            assert same_bits(one[f][1:-1, 1:-1], results["ensemble"][1][0][f][k][1:-1, 1:-1]), (
                 "single block", k, field)
     net.close()
 
@@ -209,9 +211,9 @@ def test_sorted_samples_do_not_depend_on_the_logging_order(swe):
             assert mlp.count == n
             got = mlp.read_samples()
         assert n > 1000
-        assert np.array_equal(got, raw[np.lexsort(raw.T[::-1])])  # the same rows, sorted
+        assert same_bits(got, raw[np.lexsort(raw.T[::-1])])  # the same rows, sorted
         runs.append(got)
-    assert np.array_equal(runs[0], runs[1])
+    assert same_bits(runs[0], runs[1])
 
 
 def test_training_reduces_loss_and_trained_network_passes_the_residual_check(swe, oracle):
@@ -224,7 +226,7 @@ def test_training_reduces_loss_and_trained_network_passes_the_residual_check(swe
     exact = oracle.middle_state(*[a.astype(np.float64) for a in edges[:4]])
     # acc: 1 linearised guess kept, 2 network guess kept, 0 exact iteration
     lin = acc_t == 1
-    assert np.array_equal(lin, acc_u == 1)  # the linearised guess does not depend on the network
+    assert same_bits(lin, acc_u == 1)  # the linearised guess does not depend on the network
     hard = ~lin
     rescue_t = float(np.mean(acc_t[hard] == 2))
     rescue_u = float(np.mean(acc_u[hard] == 2))

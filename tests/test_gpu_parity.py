@@ -8,7 +8,7 @@ results, time steps and states after N steps are required to be identical
 import numpy as np
 import pytest
 
-from helpers import random_edges, rel_linf
+from helpers import same_bits, random_edges, rel_linf
 
 pytestmark = pytest.mark.gpu
 
@@ -21,7 +21,7 @@ def test_solve_edges_bitwise(swe, oracle, solver, name):
     g = swe.solve_edges(solver, *ins)
     o = oracle.solve_edges(solver, *ins)
     for k, (a, b) in enumerate(zip(g, o)):
-        assert np.array_equal(a, b), f"{name} output {k}: max diff {np.nanmax(np.abs(a - b))}"
+        assert same_bits(a, b), f"{name} output {k}: max diff {np.nanmax(np.abs(a - b))}"
 
 
 def _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, solver, nsteps, strategy=1, xseg=16, flavour=0):
@@ -39,8 +39,8 @@ def _assert_parity(g, o, gdts, odts, tol=1e-5):
     for name, a, b in zip(("h", "hu", "hv"), g, o):
         e = rel_linf(a, b)
         assert e <= tol, f"{name}: rel Linf {e:.3e}"
-        assert np.array_equal(a[1:-1, 1:-1], b[1:-1, 1:-1]), f"{name}: not bit-identical (rel Linf {e:.3e})"
-    assert np.array_equal(gdts, odts)
+        assert same_bits(a[1:-1, 1:-1], b[1:-1, 1:-1]), f"{name}: not bit-identical (rel Linf {e:.3e})"
+    assert same_bits(gdts, odts)
 
 
 def test_c1_fwave_256_200_steps(swe, oracle):
@@ -106,7 +106,7 @@ def test_per_call_api_matches_oracle_step_by_step(swe, oracle):
             assert dt == odts[0]
             gh, ghu, ghv = blk.download()
             for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
-                assert np.array_equal(a, c), f"step {step}"
+                assert same_bits(a, c), f"step {step}"
 
 
 def test_run_ghosts_match_reference_sequence(swe, oracle):
@@ -121,7 +121,7 @@ def test_run_ghosts_match_reference_sequence(swe, oracle):
                 (gh, ghu, ghv), t, _ = _run_gpu(swe, h, hu, hv, b, dx, dy, bnd, swe.FWAVE, nsteps, strategy)
                 oh, ohu, ohv, _ = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.FWAVE, nsteps)
                 for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
-                    assert np.array_equal(a, c), (bnd, strategy, nsteps)
+                    assert same_bits(a, c), (bnd, strategy, nsteps)
 
 
 @pytest.mark.parametrize("solver", [1, 3])
@@ -154,7 +154,7 @@ def test_apply_bc_matches_oracle(swe, oracle):
         oracle.apply_bc(bnd, oh, ohu, ohv)
         oracle.apply_bc_bathymetry(bnd, ob)
         for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv), (gb, ob)):
-            assert np.array_equal(a, c), bnd
+            assert same_bits(a, c), bnd
 
 
 def test_invalid_arguments_fail_loudly(swe):
@@ -201,7 +201,7 @@ def test_analytic_scenario_known_answer_and_errors(swe):
         h, hu, hv, b = blk.download(with_b=True)
         want = np.full((12, 12), 10.0, np.float32)
         want[3, 3] = 15.0  # the only centre within 100 m: (250, 250) is the centre of cell (3, 3)
-        assert np.array_equal(h, want)
+        assert same_bits(h, want)
         assert np.all(b == 10.0) and np.all(hu == 0.0) and np.all(hv == 0.0)
         with pytest.raises(swe.SweHipError):
             blk.init_scenario(8)
@@ -227,10 +227,10 @@ def test_grid_scenario_known_answer_and_errors(swe):
             inside = 1 <= x <= 2 and 1 <= y <= 2  # centres 37.5 and 62.5 lie strictly inside (25, 75)
             want_b[x, y] = bv + (1.0 if inside else 0.0)
             want_h[x, y] = 0.0 if bv > 0 else -bv
-    assert np.array_equal(b[1:5, 1:5], want_b)
-    assert np.array_equal(h[1:5, 1:5], want_h)
+    assert same_bits(b[1:5, 1:5], want_b)
+    assert same_bits(h[1:5, 1:5], want_h)
     assert not np.any(hu) and not np.any(hv)
-    assert np.array_equal(b[0, 1:5], b[1, 1:5]) and np.array_equal(h[5, 1:5], h[4, 1:5])  # OUTFLOW ghosts
+    assert same_bits(b[0, 1:5], b[1, 1:5]) and same_bits(h[5, 1:5], h[4, 1:5])  # OUTFLOW ghosts
     with swe.SweBlock(4, 4, 25.0, 25.0) as blk:
         for bad in [(np.ones((2, 2), np.float32), 0.0, 0.0, 0.0, 1.0), (np.ones((2, 2), np.float32), 0.0, 0.0, 1.0, -1.0),
                     (np.ones((0, 2), np.float32), 0.0, 0.0, 1.0, 1.0)]:
