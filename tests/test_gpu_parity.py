@@ -167,6 +167,8 @@ def test_invalid_arguments_fail_loudly(swe):
             blk.set_boundaries([2, 0, 0, 0])  # INFLOW: the reference asserts on it
         with pytest.raises(swe.SweHipError):
             blk.set_tuning(7, -1)
+        with pytest.raises(swe.SweHipError):
+            blk.set_tuning(-1, 12)  # strip widths are 8, 16 or 32
 
 
 def test_default_strip_width_follows_the_wave_count(swe):
