@@ -431,15 +431,16 @@ class MiddleStateMLP:
         _check(lib.swe_hip_mlp_clear_samples(self._h), "mlp_clear_samples")
 
     def sort_samples(self) -> None:
-        """Put the logged samples in a canonical order (rows sorted
-        lexicographically).  Logging appends edges in the order the device's
-        waves reach them, which varies from run to run, and training draws its
-        batches by index: sorting first makes training from the same samples
-        reproducible."""
+        """Put the logged samples in a canonical order: rows sorted
+        lexicographically by their bit patterns (a total order, so rows equal
+        in value but not in the sign of a zero cannot tie).  Logging appends
+        edges in the order the device's waves reach them, which varies from run
+        to run, and training draws its batches by index: sorting first makes
+        training from the same samples reproducible."""
         s = self.read_samples()
         if s.shape[0] < 2:
             return
-        s = s[np.lexsort(s.T[::-1])]
+        s = s[np.lexsort(s.view(np.uint32).T[::-1])]
         self.clear_samples()
         self.add_samples(s)
 

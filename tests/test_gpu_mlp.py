@@ -192,6 +192,29 @@ def test_batched_deferred_hard_edges_match_the_sweep_and_single_blocks(swe):
     net.close()
 
 
+def test_batched_deferred_hard_edges_on_small_and_thin_blocks(swe):
+    """Deferred hard edges on instances from 1 x 1 to 70 cells per side
+    (single rows and columns, tails of a few rows packed several segments to
+    a wave): the fused sweep with the deferred-edge launch against the store
+    strategy, bitwise."""
+    from swe_ppm_amd import scenarios, surrogate
+
+    net = surrogate.load_default()
+    inst = scenarios.ensemble_instances(count=40, seed=11, nmin=1, nmax=70)
+    got = {}
+    for fused in (1, 0):
+        with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+            bt.set_tuning(fused=fused)
+            bt.set_surrogate(net)
+            bt.upload(*[[i[f] for i in inst] for f in ("h", "hu", "hv", "b")])
+            bt.run(25)
+            got[fused] = bt.download()
+    for k in range(len(inst)):
+        for f, field in enumerate(("h", "hu", "hv")):
+            assert same_bits(got[1][f][k], got[0][f][k]), (k, inst[k]["nx"], inst[k]["ny"], field)
+    net.close()
+
+
 def test_sorted_samples_do_not_depend_on_the_logging_order(swe):
     """Logging appends edges in the order the waves reach them; sort_samples
     gives every run the same rows in the same (lexicographic) order."""
@@ -211,7 +234,7 @@ def test_sorted_samples_do_not_depend_on_the_logging_order(swe):
             assert mlp.count == n
             got = mlp.read_samples()
         assert n > 1000
-        assert same_bits(got, raw[np.lexsort(raw.T[::-1])])  # the same rows, sorted
+        assert same_bits(got, raw[np.lexsort(raw.view(np.uint32).T[::-1])])  # the same rows, sorted by bits
         runs.append(got)
     assert same_bits(runs[0], runs[1])
 
