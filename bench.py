@@ -34,29 +34,33 @@ VALU_WAVE_INSTR_PER_CU_CYCLE = 1.0
 BYTES_PER_CELL = 28.0  # read h, hu, hv, b + write h, hu, hv (FP32)
 
 
-def measured_traffic(workload_file: str = "fused_u_augrie_2048_islands.md"):
+def profile_field(key: str, workload_file: str = "fused_u_augrie_2048_islands.md"):
+    """An integer `key: value` line of a committed rocprofv3 summary, or None."""
+    import re
+
+    path = ROOT / "profiles" / workload_file
+    if not path.exists():
+        return None
+    m = re.search(r"^" + key + r":\s*(\d+)", path.read_text(), re.M)
+    return int(m.group(1)) if m else None
+
+
+def measured_traffic(xseg: int):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     counter summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a plain
-    copy), or None if the summary is absent."""
-    import re
-
-    path = ROOT / "profiles" / workload_file
-    if not path.exists():
+    copy), or None if the summary is absent or profiled another strip width."""
+    if profile_field("profiled_xseg") != xseg:
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", path.read_text(), re.M)
-    return int(m.group(1)) if m else None
+    return profile_field("traffic_bytes_per_launch")
 
 
-def measured_valu(workload_file: str = "fused_u_augrie_2048_islands.md"):
+def measured_valu(xseg: int):
     """VALU wave instructions per launch of the dominant kernel (rocprofv3
-    SQ_INSTS_VALU) from the committed counter summary, or None."""
-    import re
-
-    path = ROOT / "profiles" / workload_file
-    if not path.exists():
+    SQ_INSTS_VALU) from the committed counter summary, or None if absent or
+    profiled at another strip width."""
+    if profile_field("profiled_xseg") != xseg:
         return None
-    m = re.search(r"^valu_wave_instructions_per_launch:\s*(\d+)", path.read_text(), re.M)
-    return int(m.group(1)) if m else None
+    return profile_field("valu_wave_instructions_per_launch")
 
 
 def cpu_threads() -> int:
@@ -423,8 +427,8 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, args.cpu_budget)
 
-    traffic = measured_traffic() if (strategy == 2 and n == 2048 and args.math == 0) else None
-    valu = measured_valu() if (strategy == 2 and n == 2048 and args.math == 0 and xseg == 8) else None
+    traffic = measured_traffic(xseg) if (strategy == 2 and n == 2048 and args.math == 0) else None
+    valu = measured_valu(xseg) if (strategy == 2 and n == 2048 and args.math == 0) else None
     cus = 256
     try:
         import torch

@@ -214,12 +214,12 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
 /* Tuning of the device loop (swe_hip_run): strategy 0 = recompute (max pass
  * + update pass), 1 = store (flux pass writing 4 net-update sums + apply
  * pass), 2 = fused (one launch per step that applies the previous update on
- * load; the default); xseg = x columns per wavefront strip (8, 16 or 32);
- * -1 keeps the current value.  All strategies and widths give identical
- * results.  Until the caller sets xseg, a block uses the widest of 32, 16
- * and 8 that still gives at least two waves per wave slot of the device,
- * nyt * ceil(nx/xseg) >= 2 * CUs * 24 with nyt = ceil(ny/62); with a learned
- * middle state attached, 16 when nyt * ceil(nx/16) >= CUs * 12, else 8
+ * load; the default); xseg = x columns per wavefront strip (8, 10, 12, 16
+ * or 32); -1 keeps the current value.  All strategies and widths give
+ * identical results.  Until the caller sets xseg, a block uses the widest of
+ * 16, 12 and 10 for which nyt * ceil(nx/xseg) / (CUs * 4) reaches the
+ * solver's threshold (nyt = ceil(ny/62); AugRie 12, 4, 1; with a learned
+ * middle state attached 5, 3, 2; HLLE, f-wave, Hybrid 2, 1, 0.5), else 8
  * (swe_hip_set_surrogate re-picks it). */
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);

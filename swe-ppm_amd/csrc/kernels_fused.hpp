@@ -45,9 +45,9 @@ struct Q3 {
 __device__ __forceinline__ bool is_ow(int t) { return t == OUTFLOW || t == WALL; }
 
 template <int S, int XS, int V>
-__global__ __launch_bounds__(256) void fused_u_kernel(StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl,
-                                                      int slotIn, int slotOut, int slotClear, float* dtLog,
-                                                      int logIdx) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves_per_simd(S), 8))) void fused_u_kernel(
+    StateView v, FusedIO io, BoundaryInfo bi, DeviceCtl* ctl, int slotIn, int slotOut, int slotClear, float* dtLog,
+    int logIdx) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
     const int nyt = (v.ny + kStripRows - 1) / kStripRows;

@@ -71,7 +71,11 @@ int launch_fused_defer(int xseg, StateView v, FusedIO io, BoundaryInfo bi, Devic
         hipLaunchKernelGGL((fused_u_kernel<SOLVER_AUGRIE_DEFER, XS, 0>), dim3(groups), dim3(64 * kWavesPerGroup), 0,
                            stream, v, io, bi, ctl, slotIn, slotOut, slotClear, dtLog, logIdx);
     };
-    if (xseg == 16)
+    if (xseg == 10)
+        go(std::integral_constant<int, 10>());
+    else if (xseg == 12)
+        go(std::integral_constant<int, 12>());
+    else if (xseg == 16)
         go(std::integral_constant<int, 16>());
     else if (xseg == 32)
         go(std::integral_constant<int, 32>());

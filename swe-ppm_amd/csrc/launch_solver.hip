@@ -45,6 +45,8 @@ int go_u_xs(int xseg, StateView v, SumsView sums, float* hn, float* hun, float* 
             DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
     switch (xseg) {
         case 8:
+        case 10:  // the fused kernel's widths 10 and 12: the sweeps run 8
+        case 12:
             return go_u<S, MODE, 8, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
         case 32:
             return go_u<S, MODE, 32, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
@@ -97,7 +99,11 @@ int launch_fused_u_solver(int xseg, StateView v, FusedIO io, BoundaryInfo bi, De
         hipLaunchKernelGGL((fused_u_kernel<S, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v, io, bi,
                            ctl, slotIn, slotOut, slotClear, dtLog, logIdx);
     };
-    if (xseg == 16)
+    if (xseg == 10)
+        go(std::integral_constant<int, 10>());
+    else if (xseg == 12)
+        go(std::integral_constant<int, 12>());
+    else if (xseg == 16)
         go(std::integral_constant<int, 16>());
     else if (xseg == 32)
         go(std::integral_constant<int, 32>());

@@ -71,27 +71,40 @@ namespace {
 
 size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pitch; }
 
-// Default strip width of a flavour-U block.  A wider segment solves and loads
-// fewer halo columns per owned column (17 x-edges and 18 loaded columns per 16
-// owned, against 9 and 10 per 8) but gives fewer waves; the device has
-// CUs x 4 SIMDs x 6 wave slots at the sweeps' register use.
-//  - exact solvers: the widest of 32, 16 and 8 that still gives at least two
-//    waves per slot.  Measured: 4096^2, 428 us per step at 16 against 482 at
-//    8; 2048^2 (4 352 waves at 16 for 6 144 slots), 16 is 10% slower than 8.
-//  - learned middle state (its sweep defers the hard edges, so its waves are
-//    lighter and more even): 16 while that gives at least half a wave per
-//    slot, else 8.  Measured: 2048^2, 165 us at 16, 184 at 8, 220 at 32;
-//    4096^2, 435 us at 16, 461 at 32, 503 at 8.
-int default_xseg(int nx, int ny, bool learned) {
+// Default strip width of a flavour-U block.  A wider segment loads and solves
+// fewer halo columns per owned column and pays a wave's fixed start fewer
+// times, but gives fewer waves; narrow segments spread the work more evenly
+// over the SIMDs where its cost varies from edge to edge (AugRie's Newton
+// iteration).  The block takes the widest of 16, 12 and 10 whose busiest SIMD
+// still holds at least the solver's threshold of waves (waves per SIMD =
+// ceil(ny/62) * ceil(nx/xs) / (CUs x 4)), else 8:
+//                                      16     12     10
+//   AugRie (7 waves per SIMD)          12      4      1
+//   learned middle state (deferring)    5      3      2   lighter, even waves
+//   HLLE, f-wave, Hybrid                2      1    0.5   light edges
+// 32 stays available through swe_hip_set_tuning.  Measured, radial dam break
+// over bathymetry, us per step at 8/10/12/16/32 columns:
+//   AugRie 1024^2  73.5 / 72.7 / 80.2 / 99.9 / 176   (picks 10)
+//          2048^2   156 /  145 /  143 /  161 / 215   (12)
+//          3000^2   299 /  266 /  259 /  268 / 309   (12)
+//          4096^2   514 /  423 /  413 /  414 / 467   (16; the trench 4096^2: 424 at 12, 416 at 16)
+//          8192^2  1879 / 1417 / 1387 / 1346 / 1360  (16)
+//   learned 2048^2  181 /  174 /  166 /  172 / 220   (12); 3000^2: 285 at 12, 280 at 16 (16)
+//   HLLE 2048^2     129 /  116 /  105 /   95 /  90   (16); f-wave 134 / 122 / 104 / 96 / 100 (16)
+int default_xseg(int nx, int ny, int solver) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         return 8;
-    const long long strips = (ny + kStripRows - 1) / kStripRows;
-    const long long slots = (long long)cus * 4 * 6;
-    if (learned) return 2 * strips * ((nx + 15) / 16) >= slots ? 16 : 8;
-    for (int xs = 32; xs > 8; xs /= 2)
-        if (strips * ((nx + xs - 1) / xs) >= 2 * slots) return xs;
+    const double strips = (double)((ny + kStripRows - 1) / kStripRows);
+    const double simds = 4.0 * cus;
+    static const double heavy[3] = {12.0, 4.0, 1.0}, learned[3] = {5.0, 3.0, 2.0}, light[3] = {2.0, 1.0, 0.5};
+    const double* t = (solver == SOLVER_AUGRIE || solver == SOLVER_AUGRIE_MLP) ? heavy
+                      : solver == SOLVER_AUGRIE_DEFER                         ? learned
+                                                                              : light;
+    const int widths[3] = {16, 12, 10};
+    for (int i = 0; i < 3; i++)
+        if (strips * ((nx + widths[i] - 1) / widths[i]) / simds >= t[i]) return widths[i];
     return 8;
 }
 
@@ -448,7 +461,7 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
     k->oy = oy;
     k->solver = solver;
     k->flavour = flavour;
-    k->xseg = default_xseg(nx, ny, false);
+    k->xseg = default_xseg(nx, ny, solver);
     const size_t bytes = field_elems(k) * sizeof(float);
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
@@ -908,8 +921,8 @@ int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
         set_error("swe_hip_set_tuning: strategy must be 0, 1 or 2");
         return SWE_HIP_EINVAL;
     }
-    if (xseg != -1 && xseg != 8 && xseg != 16 && xseg != 32) {
-        set_error("swe_hip_set_tuning: xseg must be 8, 16 or 32");
+    if (xseg != -1 && xseg != 8 && xseg != 10 && xseg != 12 && xseg != 16 && xseg != 32) {
+        set_error("swe_hip_set_tuning: xseg must be 8, 10, 12, 16 or 32");
         return SWE_HIP_EINVAL;
     }
     if (strategy != -1) k->strategy = strategy;
@@ -959,7 +972,7 @@ int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
     HIP_TRY(hipStreamSynchronize(k->stream));
     HIP_TRY(hipMemcpy(&k->ctl->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
     k->mlp = frags;
-    if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags != nullptr);
+    if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags ? SOLVER_AUGRIE_DEFER : k->solver);
     return SWE_HIP_OK;
 }
 

@@ -122,7 +122,7 @@ def _held_out_edges(oracle):
 
 
 @pytest.mark.parametrize("nx,ny,xseg,walls", [(256, 256, 8, False), (130, 77, 16, False), (200, 150, 32, False),
-                                              (190, 140, 16, True)])
+                                              (190, 140, 16, True), (203, 131, 10, False), (161, 190, 12, True)])
 def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx, ny, xseg, walls):
     """With a network attached, the fused loop settles the edges whose
     linearised guess passes the residual check inside the sweep and queues

@@ -57,7 +57,8 @@ def test_c1_fwave_256_200_steps(swe, oracle):
     assert abs(t - float(np.sum(gdts, dtype=np.float32))) <= 1e-3 * t
 
 
-@pytest.mark.parametrize("strategy,xseg", [(1, 16), (0, 16), (1, 8), (0, 32), (2, 8), (2, 16), (2, 32)])
+@pytest.mark.parametrize("strategy,xseg", [(1, 16), (0, 16), (1, 8), (0, 32), (2, 8), (2, 16), (2, 32), (2, 10),
+                                           (2, 12), (1, 12)])
 def test_islands_augrie_parity(swe, oracle, strategy, xseg):
     from swe_ppm_amd import scenarios
 
@@ -168,19 +169,24 @@ def test_invalid_arguments_fail_loudly(swe):
         with pytest.raises(swe.SweHipError):
             blk.set_tuning(7, -1)
         with pytest.raises(swe.SweHipError):
-            blk.set_tuning(-1, 12)  # strip widths are 8, 16 or 32
+            blk.set_tuning(-1, 11)  # strip widths are 8, 10, 12, 16 or 32
 
 
 def test_default_strip_width_follows_the_wave_count(swe):
-    """The widest of 32, 16, 8 columns per strip that leaves at least two waves
-    per wave slot (256 CUs x 4 SIMDs x 6 waves): 4096^2 has 67 strips x 256
-    segments at 16 (x 128 at 32), 2048^2 34 x 128 at 16.  With a network
-    attached, 16 from half a wave per slot.  A width the caller set is kept."""
+    """The widest of 16, 12, 10 columns per strip whose busiest SIMD still
+    holds the solver's threshold of waves (256 CUs x 4 SIMDs; AugRie 12, 4, 1
+    waves; learned 5, 3, 2; HLLE, f-wave, Hybrid 2, 1, 0.5), else 8: 4096^2
+    has 67 strips x 256 segments at 16 (16.75 per SIMD), 2048^2 34 x 171 at 12
+    (5.68), 1024^2 17 x 103 at 10 (1.71).  A width the caller set is kept."""
     from swe_ppm_amd import surrogate
 
+    with swe.SweBlock(2048, 2048, 1.0, 1.0, solver=swe.HLLE) as blk:
+        assert blk.get_tuning()[1] == 16
+    with swe.SweBlock(1024, 1024, 1.0, 1.0, solver=swe.FWAVE) as blk:
+        assert blk.get_tuning()[1] == 12
     net = surrogate.load_default()
-    for nx, ny, exact, learned in ((2048, 2048, 8, 16), (4096, 4096, 16, 16), (4096, 2048, 8, 16),
-                                   (2048, 4096, 8, 16), (1024, 1024, 8, 8), (64, 64, 8, 8)):
+    for nx, ny, exact, learned in ((2048, 2048, 12, 12), (4096, 4096, 16, 16), (4096, 2048, 12, 16),
+                                   (2048, 4096, 12, 16), (1024, 1024, 10, 8), (64, 64, 8, 8)):
         with swe.SweBlock(nx, ny, 1.0, 1.0) as blk:
             assert blk.get_tuning()[1] == exact, (nx, ny)
             blk.set_surrogate(net)
