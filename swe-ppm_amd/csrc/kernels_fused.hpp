@@ -64,7 +64,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
         if (dtLog) dtLog[logIdx] = dt;
     }
     if (wave >= nyt * nxs) return;  // uniform per wave
-    const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
+    // the Newton loop's first iteration from the bracket (middle_state<true>):
+    // measured 3% faster at seven waves per SIMD, same results
+    const dev::SolverCtx<S, true> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
 
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;

@@ -154,9 +154,9 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // with ReuseFirst it takes them instead of recomputing them, so it costs one
 // square root and no division.  The values are the ones the loop computes, so
 // both forms give identical results.  The compiler peels the first iteration
-// for it, a second copy of the loop body: that pays in the batched kernels
-// (register budget fixed at 72 for seven waves per SIMD) and costs time in the
-// single-block kernels, which therefore recompute.
+// for it, a second copy of the loop body; under the 72-register budget of
+// seven waves per SIMD that pays (the fused single-block and batched kernels
+// use it; the split-strategy sweeps recompute).
 // ---------------------------------------------------------------------------
 struct Middle {
     float hm, s1m, s2m;
