@@ -145,7 +145,9 @@ void fwave(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, 
 // phi is increasing and concave on each search interval, so Newton from the
 // left end increases monotonically to the root.  The derivative's two
 // quotients share one division:
-//   (h-hMin)/gMin + (h-hMax)/gMax = ((h-hMin) gMax + (h-hMax) gMin) / (gMin gMax).
+//   (h-hMin)/gMin + (h-hMax)/gMax = ((h-hMin) gMax + (h-hMax) gMin) / (gMin gMax) = num / den,
+// and the step phi / dphi is taken with both sides scaled by den, so an
+// iteration divides twice (1/h and the step).
 // After convergence (|step| <= newtonTol h) the middle velocities use the
 // last iteration's gk.
 // ---------------------------------------------------------------------------
@@ -191,8 +193,9 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
                 const T phi = (du + (h - hMin) * gMin) + (h - hMax) * gMax;
                 const T num = (h - hMin) * gMax + (h - hMax) * gMin;
                 const T den = gMin * gMax;
-                const T dphi = (gMin + gMax) - quarterG * invH * invH * (num / den);
-                const T step = phi / dphi;
+                // phi / dphi with dphi = (gMin + gMax) - quarterG invH^2 num / den,
+                // numerator and denominator scaled by den (one division)
+                const T step = (phi * den) / ((gMin + gMax) * den - quarterG * invH * invH * num);
                 h = h - step;
                 if (std::fabs(step) <= C::newtonTol * h) {
                     ++it;
@@ -219,8 +222,7 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
                 const T phi = (du + (h - hMin) * gMin) + T(2) * (ch - cMax);
                 const T num = h - hMin;
                 const T den = gMin;
-                const T dphi = (gMin + ch * invH) - quarterG * invH * invH * (num / den);
-                const T step = phi / dphi;
+                const T step = (phi * den) / ((gMin + ch * invH) * den - quarterG * invH * invH * num);
                 h = h - step;
                 if (std::fabs(step) <= C::newtonTol * h) {
                     ++it;

@@ -145,7 +145,9 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // branch-free Newton loop serves both the two-shock case (from hMax) and the
 // shock+rarefaction case (from hMin): the curve towards hMax is selected per
 // lane, and its two square roots share one instruction slot.  The
-// derivative's two quotients share one division.  phi is increasing and
+// derivative's two quotients share one division, num / den, and the step
+// phi / dphi is taken with both sides scaled by den, so an iteration divides
+// twice (1/h and the step).  phi is increasing and
 // concave on the search interval, so the iterates increase monotonically.
 // cL, cR are sqrt(g hL), sqrt(g hR); invL, invR are 1 / hL, 1 / hR, computed
 // once per cell.
@@ -217,8 +219,9 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
         const float num = (h - hMin) * q + (shocks ? (h - hMax) * gMin : 0.f);
         const float den = gMin * q;
         const float first = gMin + (shocks ? s2 : s2 * invH);
-        const float dphi = first - quarterG * invH * invH * (num / den);
-        const float step = phi / dphi;
+        // phi / (first - quarterG invH^2 num / den) with both terms scaled by
+        // den: one division instead of two
+        const float step = (phi * den) / (first * den - quarterG * invH * invH * num);
         h = h - step;
         if (fabsf(step) <= kNewtonTol * h) break;
     }
