@@ -243,8 +243,8 @@ __device__ __forceinline__ Eval eval_guess(const Bracket& k, float hGuess) {
     const float num = (h - k.hMin) * q + (shocks ? (h - k.hMax) * gMin : 0.f);
     const float den = gMin * q;
     const float first = gMin + (shocks ? s2 : s2 * invH);
-    const float dphi = first - quarterG * invH * invH * (num / den);
-    const float step = phi / dphi;
+    // the exact iteration's step form: phi / dphi with both sides scaled by den
+    const float step = (phi * den) / (first * den - quarterG * invH * invH * num);
     Eval e;
     e.h = h - step;
     e.gMin = gMin;
