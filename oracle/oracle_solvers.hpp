@@ -46,6 +46,9 @@ enum SolverKind { SOLVER_HLLE = 0, SOLVER_FWAVE = 1, SOLVER_AUGRIE = 2, SOLVER_H
 template <typename T>
 struct SolverConsts {
     static constexpr T g = T(9.81);
+    // sqrt(g): a cell's celerity is sqrtG * sqrt(h); in single precision the
+    // correctly rounded square root of 9.81f, as the kernels use it
+    static constexpr T sqrtG = sizeof(T) == sizeof(float) ? T(0x1.90e864p+1f) : T(0x1.90e8639c65e42p+1);
     static constexpr T dryTol = T(0.01);
     static constexpr T zeroTol = T(1e-7);
     static constexpr T newtonTol = T(1e-6);
@@ -95,12 +98,13 @@ void fwave(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, 
         huR = -huL;
         bR = bL;
     }
-    const T uL = huL / hL;
-    const T uR = huR / hR;
+    // a cell's velocity and celerity from 1/h and sqrt(h)
+    const T uL = huL * (T(1) / hL);
+    const T uR = huR * (T(1) / hR);
     const T sqhL = std::sqrt(hL);
     const T sqhR = std::sqrt(hR);
-    const T cL = std::sqrt(C::g * hL);
-    const T cR = std::sqrt(C::g * hR);
+    const T cL = C::sqrtG * sqhL;
+    const T cR = C::sqrtG * sqhR;
     // Roe averages
     const T hRoe = T(0.5) * (hL + hR);
     const T uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
@@ -157,8 +161,8 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
     const T hMin = std::min(hL, hR);
     const T hMax = std::max(hL, hR);
     const T du = uR - uL;
-    const T cL = std::sqrt(C::g * hL);
-    const T cR = std::sqrt(C::g * hR);
+    const T cL = C::sqrtG * std::sqrt(hL);
+    const T cR = C::sqrtG * std::sqrt(hR);
     int it = 0;
     int kind;
     if (hMin < C::dryTol) {
@@ -167,8 +171,8 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
         s2m = s1m;
         kind = 0;
     } else {
-        const T cMin = std::sqrt(C::g * hMin);
-        const T cMax = std::sqrt(C::g * hMax);
+        const T cMin = C::sqrtG * std::sqrt(hMin);
+        const T cMax = C::sqrtG * std::sqrt(hMax);
         const T invMin = T(1) / hMin;
         const T invMax = T(1) / hMax;
         const T halfG = T(0.5) * C::g;
@@ -265,14 +269,14 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
         huL = T(0);
         uL = T(0);
     } else {
-        uL = huL / hL;
+        uL = huL * (T(1) / hL);
     }
     if (hR < C::dryTol) {
         hR = T(0);
         huR = T(0);
         uR = T(0);
     } else {
-        uR = huR / hR;
+        uR = huR * (T(1) / hR);
     }
 
     // wet/dry interface: wall (the dry cell's bathymetry is above anything the
@@ -307,8 +311,8 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
     }
 
     // Einfeldt speeds from the Roe average weighted by the celerities
-    const T cL = std::sqrt(C::g * hL);
-    const T cR = std::sqrt(C::g * hR);
+    const T cL = C::sqrtG * std::sqrt(hL);
+    const T cR = C::sqrtG * std::sqrt(hR);
     const T uHat = (cL * uL + cR * uR) / (cL + cR);
     const T cHat = std::sqrt(C::g * T(0.5) * (hR + hL));
     const T sE1 = std::min(uL - cL, uHat - cHat);
@@ -427,12 +431,12 @@ void hlle(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T
         huR = -huL;
         bR = bL;
     }
-    const T uL = huL / hL;
-    const T uR = huR / hR;
-    const T cL = std::sqrt(C::g * hL);
-    const T cR = std::sqrt(C::g * hR);
+    const T uL = huL * (T(1) / hL);
+    const T uR = huR * (T(1) / hR);
     const T sqhL = std::sqrt(hL);
     const T sqhR = std::sqrt(hR);
+    const T cL = C::sqrtG * sqhL;
+    const T cR = C::sqrtG * sqhR;
     const T uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
     const T cRoe = std::sqrt(C::g * T(0.5) * (hL + hR));
     // HLLE speeds bounded by zero so that each side gets one wave

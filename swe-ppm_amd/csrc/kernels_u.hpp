@@ -333,11 +333,11 @@ __device__ __forceinline__ CellQ cell_q(float h, float hu, float hv, float b) {
     q.hu = hu;
     q.hv = hv;
     q.b = b;
-    q.u = hu / h;
-    q.v = hv / h;
-    q.c = sqrtf(kGravity * h);
-    q.sq = sqrtf(h);
     q.inv = 1.f / h;
+    q.u = hu * q.inv;
+    q.v = hv * q.inv;
+    q.sq = sqrtf(h);
+    q.c = kSqrtGravity * q.sq;
     return q;
 }
 

@@ -42,7 +42,7 @@ struct EdgeOut {
 // IEEE results each edge would compute itself.
 struct Side {
     float h, hu, b;
-    float u;    // hu / h (normal velocity; meaningless for dry cells)
+    float u;    // hu (1/h) (normal velocity; meaningless for dry cells)
     float c;    // sqrt(g h)
     float sq;   // sqrt(h)
     float inv;  // 1 / h (the augmented solver's middle state divides by both depths)
@@ -53,10 +53,10 @@ __device__ __forceinline__ Side cell_side(float h, float hu, float b) {
     s.h = h;
     s.hu = hu;
     s.b = b;
-    s.u = hu / h;
-    s.c = sqrtf(kGravity * h);
-    s.sq = sqrtf(h);
     s.inv = 1.f / h;
+    s.u = hu * s.inv;
+    s.sq = sqrtf(h);
+    s.c = kSqrtGravity * s.sq;
     return s;
 }
 

@@ -43,6 +43,7 @@ enum GhostlayerState : int { GL_DONE = 0, GL_UNVALID = 1, GL_INTER = 2, GL_SYNC 
 // Solver constants (the reference's solver submodule is not vendored; these
 // are the values this library states and its CPU oracle uses).
 constexpr float kGravity = 9.81f;
+constexpr float kSqrtGravity = 0x1.90e864p+1f;  // sqrtf(9.81f), correctly rounded: celerity = kSqrtGravity sqrt(h)
 constexpr float kDryTol = 0.01f;
 constexpr float kZeroTol = 1e-7f;
 constexpr float kNewtonTol = 1e-6f;

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void mlp_middle_kernel(const unsigned* __restr
     const long long sc = s < n ? s : n - 1;
     const float hL = in[sc], hR = in[n + sc], uL = in[2 * n + sc], uR = in[3 * n + sc];
     const float bL = in[4 * n + sc], bR = in[5 * n + sc];
-    const float cL = sqrtf(kGravity * hL), cR = sqrtf(kGravity * hR);
+    const float cL = kSqrtGravity * sqrtf(hL), cR = kSqrtGravity * sqrtf(hR);
     int acc;
     const dev::Middle m = learned_middle(frags, hL, hR, uL, uR, cL, cR, 1.f / hL, 1.f / hR, bL, bR, &acc);
     if (s < n) {
