@@ -344,16 +344,21 @@ __device__ __forceinline__ CellQ cell_q(float h, float hu, float hv, float b) {
 // Strip sweep.  One wave per (y-tile, x-segment); 4 waves per workgroup.
 // dtFixed >= 0 forces dt (the per-call API passes the host's maxTimestep);
 // otherwise dt comes from ctl->smax[par].
+// The max and store sweeps hold the fused kernel's occupancy target; the
+// update sweep of the recompute strategy keeps its natural register use
+// (at seven waves it would spill).
 template <int S, int MODE, int XS, int V>
-__global__ __launch_bounds__(256) void sweep_u_kernel(StateView v, SumsView sums, float* hn, float* hun, float* hvn,
-                                                       BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed,
-                                                       int fuseBc, float* dtLog, int logIdx) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == MODE_UPDATE ? 1 : fused_waves_per_simd(S),
+                                                                     8))) void sweep_u_kernel(
+    StateView v, SumsView sums, float* hn, float* hun, float* hvn, BoundaryInfo bi, DeviceCtl* ctl, Slots sl,
+    float dtFixed, int fuseBc, float* dtLog, int logIdx) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
     const int nyt = (v.ny + kStripRows - 1) / kStripRows;
     const int nxs = (v.nx + XS - 1) / XS;
     if (wave >= nyt * nxs) return;  // uniform per wave
-    const dev::SolverCtx<S> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
+    // the Newton loop's first iteration from the bracket, as in the fused kernel
+    const dev::SolverCtx<S, true> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
     const int yt = wave % nyt;
     const int xs = wave / nyt;
     const int y = kStripRows * yt + lane;  // rows y0-1 .. y0+62 with y0 = 62*yt + 1

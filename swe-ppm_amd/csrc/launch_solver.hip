@@ -45,9 +45,10 @@ int go_u_xs(int xseg, StateView v, SumsView sums, float* hn, float* hun, float* 
             DeviceCtl* ctl, Slots sl, float dtFixed, int fuseBc, float* dtLog, int logIdx, hipStream_t stream) {
     switch (xseg) {
         case 8:
-        case 10:  // the fused kernel's widths 10 and 12: the sweeps run 8
-        case 12:
+        case 10:  // width 10 runs the sweeps at 8
             return go_u<S, MODE, 8, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
+        case 12:
+            return go_u<S, MODE, 12, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
         case 32:
             return go_u<S, MODE, 32, V>(v, sums, hn, hun, hvn, bi, ctl, sl, dtFixed, fuseBc, dtLog, logIdx, stream);
         default:

@@ -157,8 +157,8 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // square root and no division.  The values are the ones the loop computes, so
 // both forms give identical results.  The compiler peels the first iteration
 // for it, a second copy of the loop body; under the 72-register budget of
-// seven waves per SIMD that pays (the fused single-block and batched kernels
-// use it; the split-strategy sweeps recompute).
+// seven waves per SIMD that pays (the single-block sweeps and the batched
+// kernels use it).
 // ---------------------------------------------------------------------------
 struct Middle {
     float hm, s1m, s2m;

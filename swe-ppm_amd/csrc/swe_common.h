@@ -18,7 +18,7 @@ constexpr int SOLVER_AUGRIE_MLP = 4;
 // launch that runs the network (fused strategy of a single block only).
 constexpr int SOLVER_AUGRIE_DEFER = 5;
 
-// Waves per SIMD the fused flavour-U kernel of each solver is compiled for
+// Waves per SIMD the flavour-U sweeps of each solver are compiled for
 // (its amdgpu_waves_per_eu floor; the build fails where the compiler cannot
 // meet it).  A SIMD lane has 512 registers: 7 waves need at most 72 per lane,
 // 6 at most 80.  AugRie, Hybrid and the deferring learned sweep fit 7 (71-72
