@@ -252,10 +252,12 @@ def surrogate_compare(swe, state, weights, steps):
             "weights": str(Path(weights).name)}
 
 
-def solver_lines(swe, state, steps: int = 200):
+def solver_lines(swe, state, steps: int = 200, autotune: bool = True):
     """The headline workload with each solver of the plain path (fused
     strategy, IEEE): MCell-updates/s and the algorithmic HBM fraction, to show
-    where the same kernel stands when the Riemann solver costs less."""
+    where the same kernel stands when the Riemann solver costs less.  Each
+    block's strip width is autotuned on the workload's first 20 steps (the
+    state is restored before the timed run)."""
     h, hu, hv, b, dx, dy, bnd = state
     n = h.shape[0] - 2
     out = {}
@@ -263,13 +265,19 @@ def solver_lines(swe, state, steps: int = 200):
         with swe.SweBlock(n, n, dx, dy, solver=sid) as blk:
             blk.set_boundaries(bnd)
             blk.upload(h, hu, hv, b)
+            default = blk.get_tuning()[1]
+            tune = blk.autotune(20) if autotune else None
             blk.run(10, want_dts=False)  # warm-up
             blk.upload(h, hu, hv, b)
             _, total_ms, _, _ = timed_run(swe, blk, steps, per_kernel=False)
+            xseg = blk.get_tuning()[1]
         v = n * n * steps / (total_ms * 1e-3) / 1e6
         out[name] = {"value": round(v, 1), "ms_per_step": round(total_ms / steps, 5),
-                     "hbm_frac_algorithmic": round(BYTES_PER_CELL * v * 1e6 / 1e9 / HBM_PEAK_GBS, 4)}
-    return {"workload": f"{n}x{n} islands dam break (the headline workload), {steps} steps per solver",
+                     "hbm_frac_algorithmic": round(BYTES_PER_CELL * v * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
+                     "xseg": xseg, "default_xseg": default,
+                     "autotune_ms_per_step": None if tune is None else {k: round(t, 5) for k, t in tune.items()}}
+    return {"workload": f"{n}x{n} islands dam break (the headline workload), {steps} steps per solver, strip "
+                        f"width {'autotuned' if autotune else 'default'}",
             "unit": "MCell-updates/s", "solvers": out}
 
 
@@ -491,7 +499,7 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     if not args.no_solvers:
-        line["solvers"] = solver_lines(swe, (h, hu, hv, b, dx, dy, bnd))
+        line["solvers"] = solver_lines(swe, (h, hu, hv, b, dx, dy, bnd), autotune=not args.no_autotune)
     if not args.no_batched:
         lts = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
         ens = batched_ensemble(swe, scenarios, not args.no_autotune, steps=args.ens_steps)

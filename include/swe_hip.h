@@ -224,6 +224,17 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
 
+/* Strip-width autotuner of a flavour-U block (no counterpart in the
+ * reference, whose CPU loops have no such parameter).  Times `steps` steps of
+ * the device loop (swe_hip_run, current strategy) from the block's current
+ * state at each of the SWE_HIP_TUNE_WIDTHS widths 8, 10, 12, 16, 32, restores
+ * the state (h, hu, hv, the simulated time and step count), and keeps the
+ * fastest width as if set with swe_hip_set_tuning.  ms_per_step (may be
+ * null) receives the per-step times in that order.  Results of later runs do
+ * not depend on the width chosen.  Allocates and frees 3 fields of scratch. */
+#define SWE_HIP_TUNE_WIDTHS 5
+int swe_hip_autotune(swe_hip_block* blk, int steps, float* ms_per_step);
+
 /* Arithmetic variant of the flavour-U kernels:
  *   0 = IEEE: no contraction, correctly rounded division and square root;
  *       results are bit-identical to an IEEE single-precision CPU evaluation

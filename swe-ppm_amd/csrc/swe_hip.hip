@@ -951,6 +951,67 @@ int swe_hip_set_math(swe_hip_block* k, int math) {
     return SWE_HIP_OK;
 }
 
+int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
+    if (int r = check(k)) return r;
+    if (steps < 2 || steps > 10000 || k->flavour != FLAVOUR_U) {
+        set_error("swe_hip_autotune: needs 2 <= steps <= 10000 and flavour U");
+        return SWE_HIP_EINVAL;
+    }
+    // snapshot of what a run changes and a later run reads: the current state
+    // and the control block (the other buffer and the sums are scratch)
+    const size_t bytes = field_elems(k) * sizeof(float);
+    const int cur0 = k->cur;
+    float* save[3] = {nullptr, nullptr, nullptr};
+    DeviceCtl* saveCtl = nullptr;
+    hipError_t e = hipSuccess;
+    for (float*& p : save)
+        if (e == hipSuccess) e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) e = hipMalloc(&saveCtl, sizeof(DeviceCtl));
+    auto copy_state = [&](bool toSave) {
+        k->cur = cur0;
+        float* live[3] = {k->h[cur0], k->hu[cur0], k->hv[cur0]};
+        for (int i = 0; i < 3 && e == hipSuccess; ++i)
+            e = hipMemcpyAsync(toSave ? save[i] : live[i], toSave ? live[i] : save[i], bytes, hipMemcpyDeviceToDevice,
+                               k->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(toSave ? saveCtl : k->ctl, toSave ? k->ctl : saveCtl, sizeof(DeviceCtl),
+                               hipMemcpyDeviceToDevice, k->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
+    };
+    if (e == hipSuccess) copy_state(true);
+    const int widths[SWE_HIP_TUNE_WIDTHS] = {8, 10, 12, 16, 32};
+    const int oldXseg = k->xseg;
+    int best = oldXseg, rc = 0;
+    float bestMs = 0.f;
+    // one untimed run first (first launches of the process, allocations)
+    if (e == hipSuccess) rc = swe_hip_run_timed(k, 2, nullptr, nullptr, nullptr, nullptr);
+    if (e == hipSuccess && rc == 0) copy_state(false);
+    for (int i = 0; i < SWE_HIP_TUNE_WIDTHS && e == hipSuccess && rc == 0; ++i) {
+        k->xseg = widths[i];
+        float total = 0.f;
+        rc = swe_hip_run_timed(k, steps, nullptr, &total, nullptr, nullptr);
+        if (rc) break;
+        const float per = total / steps;
+        if (ms_per_step) ms_per_step[i] = per;
+        if (i == 0 || per < bestMs) {
+            bestMs = per;
+            best = widths[i];
+        }
+        copy_state(false);
+    }
+    for (float* p : save)
+        if (p) (void)hipFree(p);
+    if (saveCtl) (void)hipFree(saveCtl);
+    k->cur = cur0;
+    if (rc || e != hipSuccess) {
+        k->xseg = oldXseg;
+        return rc ? rc : hip_fail(e, "swe_hip_autotune");
+    }
+    k->xseg = best;
+    k->xsegSet = true;
+    return SWE_HIP_OK;
+}
+
 int swe_hip_get_tuning(swe_hip_block* k, int* strategy, int* xseg) {
     if (int r = check(k)) return r;
     if (strategy) *strategy = k->strategy;

@@ -69,6 +69,7 @@ def _load() -> C.CDLL:
         "swe_hip_update": (C.c_int, [vp, C.c_float]),
         "swe_hip_run": (C.c_int, [vp, C.c_int, fp, fp]),
         "swe_hip_run_timed": (C.c_int, [vp, C.c_int, fp, fp, fp, C.POINTER(C.c_int)]),
+        "swe_hip_autotune": (C.c_int, [vp, C.c_int, fp]),
         "swe_hip_synchronize": (C.c_int, [vp]),
         "swe_hip_device_pointers": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                               C.POINTER(C.c_int)]),
@@ -233,6 +234,15 @@ class SweBlock:
 
     def set_tuning(self, strategy: int = -1, xseg: int = -1) -> None:
         _check(lib.swe_hip_set_tuning(self._h, strategy, xseg), "set_tuning")
+
+    TUNE_WIDTHS = (8, 10, 12, 16, 32)
+
+    def autotune(self, steps: int = 20) -> dict:
+        """Time `steps` steps at each strip width from the current state,
+        restore the state and keep the fastest width; {width: ms per step}."""
+        ms = (C.c_float * len(self.TUNE_WIDTHS))()
+        _check(lib.swe_hip_autotune(self._h, int(steps), ms), "autotune")
+        return {w: float(m) for w, m in zip(self.TUNE_WIDTHS, ms)}
 
     def set_math(self, math: int) -> None:
         """0 = IEEE (bit-compatible with the CPU oracle), 1 = fast."""

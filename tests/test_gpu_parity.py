@@ -172,6 +172,35 @@ def test_invalid_arguments_fail_loudly(swe):
             blk.set_tuning(-1, 11)  # strip widths are 8, 10, 12, 16 or 32
 
 
+def test_autotune_restores_the_state_and_keeps_the_results(swe):
+    """swe_hip_autotune times every strip width from the current state, puts
+    the state back bit for bit and keeps the fastest width; a run afterwards
+    gives the same bits as a run on an untuned block."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=300, seed=7)
+    with swe.SweBlock(300, 300, dx, dy, solver=swe.AUGRIE) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        t_ref, _ = blk.run(30)
+        ref = blk.download()
+    with swe.SweBlock(300, 300, dx, dy, solver=swe.AUGRIE) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        before = blk.download()
+        ms = blk.autotune(4)
+        assert sorted(ms) == [8, 10, 12, 16, 32] and all(v > 0.0 for v in ms.values())
+        assert blk.get_tuning()[1] == min(ms, key=ms.get)
+        for a, c in zip(blk.download(), before):
+            assert same_bits(a, c)
+        t, _ = blk.run(30)
+        assert t == t_ref
+        for a, c in zip(blk.download(), ref):
+            assert same_bits(a, c)
+        with pytest.raises(swe.SweHipError):
+            blk.autotune(1)
+
+
 def test_default_strip_width_follows_the_wave_count(swe):
     """The widest of 16, 12, 10 columns per strip whose busiest SIMD still
     holds the solver's threshold of waves (256 CUs x 4 SIMDs; AugRie 12, 4, 1
