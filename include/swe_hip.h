@@ -214,8 +214,9 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
 /* Tuning of the device loop (swe_hip_run): strategy 0 = recompute (max pass
  * + update pass), 1 = store (flux pass writing 4 net-update sums + apply
  * pass), 2 = fused (one launch per step that applies the previous update on
- * load; the default); xseg = x columns per wavefront strip (8, 10, 12, 16
- * or 32); -1 keeps the current value.  All strategies and widths give
+ * load; the default); xseg = x columns per wavefront strip (4, 8, 10, 12,
+ * 16 or 32; the sweeps of strategies 0 and 1 run 4 as 16); -1 keeps the
+ * current value.  All strategies and widths give
  * identical results.  Until the caller sets xseg, a block uses the widest of
  * 16, 12 and 10 for which nyt * ceil(nx/xseg) / (CUs * 4) reaches the
  * solver's threshold (nyt = ceil(ny/62); AugRie 12, 4, 1; with a learned
@@ -227,13 +228,23 @@ int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
 /* Strip-width autotuner of a flavour-U block (no counterpart in the
  * reference, whose CPU loops have no such parameter).  Times `steps` steps of
  * the device loop (swe_hip_run, current strategy) from the block's current
- * state at each of the SWE_HIP_TUNE_WIDTHS widths 8, 10, 12, 16, 32, restores
+ * state at each of the SWE_HIP_TUNE_WIDTHS widths 4, 8, 10, 12, 16, 32, restores
  * the state (h, hu, hv, the simulated time and step count), and keeps the
  * fastest width as if set with swe_hip_set_tuning.  ms_per_step (may be
  * null) receives the per-step times in that order.  Results of later runs do
  * not depend on the width chosen.  Allocates and frees 3 fields of scratch. */
-#define SWE_HIP_TUNE_WIDTHS 5
+#define SWE_HIP_TUNE_WIDTHS 6
 int swe_hip_autotune(swe_hip_block* blk, int steps, float* ms_per_step);
+
+/* Quiet-tile skipping of the fused device loop (strategy 2), on by default.
+ * A strip tile whose net-update sums were all +0 on the last two steps, with
+ * its four neighbours quiet on the last step, would compute the same zero
+ * sums and the same CFL maximum again from an unchanged state; the loop
+ * skips it and carries its maximum forward.  Results are bit-identical with
+ * and without skipping (enable = 0 computes every tile, e.g. for timing the
+ * full sweep).  No reference counterpart: the reference's loops visit every
+ * edge every step. */
+int swe_hip_set_quiet_skip(swe_hip_block* blk, int enable);
 
 /* Arithmetic variant of the flavour-U kernels:
  *   0 = IEEE: no contraction, correctly rounded division and square root;

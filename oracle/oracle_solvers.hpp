@@ -42,6 +42,9 @@
 namespace oracle {
 
 enum SolverKind { SOLVER_HLLE = 0, SOLVER_FWAVE = 1, SOLVER_AUGRIE = 2, SOLVER_HYBRID = 3 };
+// the augmented solver's decomposition without the still-edge path (for the
+// tests that pin that path against it)
+constexpr int SOLVER_AUGRIE_FULL = 12;
 
 template <typename T>
 struct SolverConsts {
@@ -55,7 +58,55 @@ struct SolverConsts {
     static constexpr int maxNewton = 10;
     static constexpr T criticalTol = T(1e-6);
     static constexpr T hybridTol = T(1.0);
+    // still edges of the augmented solver (augrie_still): both depths at
+    // least stillTol and within a factor stillRatio of each other
+    static constexpr T stillTol = T(0.1);
+    static constexpr T stillRatio = T(2);
 };
+
+// ---------------------------------------------------------------------------
+// Still edge of the augmented solver: both cells wet with at least stillTol
+// of water (cells shallower than 0.1 m have their momentum zeroed by the dry
+// clamp of the update, so zero momentum there says nothing), zero normal
+// momentum on both sides, the same free surface h + b in the working
+// precision, and depths within a factor stillRatio.  Such an edge is a lake
+// at rest: its exact Riemann solution has no flux, so the solver returns zero
+// net updates (+0) instead of the round-off of the full decomposition, and
+// still water stays exactly still.
+//
+// Its speed is the one the full solver reports.  At rest uL = uR = 0 and the
+// celerity-weighted average uHat is 0, so the Einfeldt speeds are
+// sE1 = min(-cL, -cHat) and sE2 = max(cR, cHat).  The middle state lies
+// between the two depths (the two-shock case with hL = hR returns hm = hL;
+// otherwise it is a shock towards the shallower side and a rarefaction
+// towards the deeper one, and the Newton iterates rise from hMin to the root
+// below hMax), so with hMax <= 2 hMin, i.e. cMax <= sqrt(2) cMin:
+//   hL >= hR: s2m = 2 cL - cm >= cL > 0 > sE1 and
+//             s1m = 2 cL - 3 cm <= (2 sqrt(2) - 3) cR < 0 < sE2,
+//   hL <  hR: s1m = cm - 2 cR <= -cR < 0 < sE2 and
+//             s2m = 3 cm - 2 cR >= (3 - 2 sqrt(2)) cL > 0 > sE1,
+// hence l1 = sE1, l3 = sE2 and the speed is max(|sE1|, |sE2|) with no middle
+// state.  tests/test_oracle_solvers.py checks the speed against the full
+// solver bit for bit, and the full solver's updates against zero to within
+// its round-off, on random still edges.
+// ---------------------------------------------------------------------------
+template <typename T>
+inline bool augrie_still(T hL, T hR, T huL, T huR, T bL, T bR) {
+    typedef SolverConsts<T> C;
+    return hL >= C::stillTol && hR >= C::stillTol && huL == T(0) && huR == T(0) && hL + bL == hR + bR &&
+           hL <= C::stillRatio * hR && hR <= C::stillRatio * hL;
+}
+
+template <typename T>
+inline T augrie_still_speed(T hL, T hR) {
+    typedef SolverConsts<T> C;
+    const T cL = C::sqrtG * std::sqrt(hL);
+    const T cR = C::sqrtG * std::sqrt(hR);
+    const T cHat = std::sqrt(C::g * T(0.5) * (hR + hL));
+    const T sE1 = std::min(-cL, -cHat);
+    const T sE2 = std::max(cR, cHat);
+    return std::max(std::fabs(sE1), std::fabs(sE2));
+}
 
 // Add one f-wave (its h and hu components) to the side its speed points to;
 // a wave of (near) zero speed is shared equally.
@@ -255,9 +306,11 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
 // deficit), the steady-state wave subtracted for well-balancing, wet/dry
 // walls and inundation.
 // ---------------------------------------------------------------------------
+// augrie_full is the decomposition alone; augrie (below) takes still edges
+// out first.
 template <typename T>
-void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
-            T& maxSpeed) {
+void augrie_full(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
+                 T& maxSpeed) {
     typedef SolverConsts<T> C;
     hNetL = hNetR = huNetL = huNetR = T(0);
     maxSpeed = T(0);
@@ -407,6 +460,17 @@ void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL,
     maxSpeed = std::max(std::fabs(s1), std::fabs(s3));
 }
 
+template <typename T>
+void augrie(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T& huNetR,
+            T& maxSpeed) {
+    if (augrie_still(hL, hR, huL, huR, bL, bR)) {
+        hNetL = hNetR = huNetL = huNetR = T(0);
+        maxSpeed = augrie_still_speed(hL, hR);
+        return;
+    }
+    augrie_full(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+}
+
 // ---------------------------------------------------------------------------
 // HLLE solver in f-wave form (Einfeldt 1988): two waves at the Einfeldt
 // speeds, the HLL middle state splitting the flux jump, with the HLL
@@ -494,6 +558,9 @@ inline void solve_edge(int solver, T hL, T hR, T huL, T huR, T bL, T bR, T& hNet
             break;
         case SOLVER_AUGRIE:
             augrie(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
+            break;
+        case SOLVER_AUGRIE_FULL:
+            augrie_full(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);
             break;
         default:
             hybrid(hL, hR, huL, huR, bL, bR, hNetL, hNetR, huNetL, huNetR, maxSpeed);

@@ -16,6 +16,40 @@
 //
 // Work decomposition and edge order are those of the strip sweep
 // (kernels_u.hpp), so S(k) and M(k) are bit-identical to the two-kernel step.
+//
+// Quiet tiles.  The quiet record works on tiles of one strip (62 rows) by
+// kQuietCols columns.  Launch k records per tile which of its parts had all
+// four sums of every owned cell +0 (bit pattern 0) with no deferred edge --
+// all its cells, its bottom and top owned rows, its first and last columns --
+// and its CFL maximum, speed(k): the maximum over its x-edges on owned rows
+// (those right of its columns; the left halo edge too where the tile starts a
+// run of computed tiles) and its y-edges.  The tile's results depend
+// only on Q(k) over its footprint: its own cells, the last column of its left
+// neighbour, the first column of its right one, the top row of the tile below
+// and the bottom row of the tile above (x-edge speeds count on owned rows
+// only, so the corner cells of the halo feed nothing the tile reports).  If
+// those parts were quiet at k-1, then Q(k) = Q(k-1) over the footprint -- an
+// update by +0 sums leaves h, hu, hv bit for bit unchanged once the state has
+// passed one dry clamp -- so S(k) and speed(k) repeat S(k-1) = +0 and
+// speed(k-1).  If the tile was also quiet at k-2, the buffers launch k writes
+// (state and sums of step k-2) already hold exactly those values.  Such a
+// tile is skipped: it only carries its speed forward and adds it to the CFL
+// maximum.  Launches 1 and 2 of a run never skip (their inputs may predate
+// the first clamp), so every flag read was written earlier in the same run.
+// Results are bit-identical with and without skipping.
+//
+// A wave still owns a segment of XS columns (a whole number of record
+// tiles).  The skip decisions of launch k are taken by quiet_plan_kernel
+// between launches k-1 and k: it carries the skipped tiles' speeds and, per
+// segment with work left, lists the run of record tiles from its first to its
+// last busy one (quiet tiles inside the run are recomputed, to the same
+// values).  The list is compact and in domain order, per group of kPlanGroup
+// segments.  Launch k then runs in list mode: wave w takes the w-th listed
+// run, and waves past the count leave at once.  The busy runs thus take the
+// first waves, which the dispatcher spreads over all CUs, instead of sitting
+// wherever they lie in the domain (with one wave per segment in domain order,
+// the CUs that hold the busy region would set the launch time while the
+// others idle); with every tile busy the launch is the one without a plan.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,6 +57,17 @@
 #include "kernels_u.hpp"
 
 namespace swe {
+
+// Parts of a tile whose owned sums the quiet record tracks (one bit each):
+// all cells, the bottom and top owned rows and the first and last columns --
+// the halo a neighbouring tile reads.
+constexpr unsigned kQuietAll = 1u, kQuietBottom = 2u, kQuietTop = 4u, kQuietLeft = 8u, kQuietRight = 16u;
+// Record tiles are kQuietCols columns wide; a wave's segment of XS columns
+// spans XS / kQuietCols of them.
+constexpr int kQuietCols = 4;
+// Waves the quiet-tile plan aims to fill with busy record tiles before it
+// lengthens the pieces it lists.
+constexpr unsigned kPlanWaves = 2048;
 
 struct FusedIO {
     const float* h;   // Q(k-1)
@@ -35,8 +80,41 @@ struct FusedIO {
     SumsView sIn;   // S(k-1)
     SumsView sOut;  // S(k)
     uint2* deferQ = nullptr;  // queue of deferred edges (SOLVER_AUGRIE_DEFER only)
+    // quiet-tile record: quiet[slot * tiles + t], speed[slot * tiles + t] with
+    // slot = launch index & 1 (null: no record, no skipping)
+    unsigned* quiet = nullptr;
+    float* speed = nullptr;
+    int step = 0;  // launch index within the run (1, 2, ...)
+    // list mode: group g of kPlanGroup tiles lists count[g] tiles to compute
+    // at list[g * kPlanGroup ...]
+    const unsigned* list = nullptr;
+    const unsigned* count = nullptr;
 };
 
+// The list entry of wave w in list mode, or -1 past the last listed entry: a
+// wave scan over the group counts finds the group holding the w-th entry.
+__device__ __forceinline__ int listed_item(const FusedIO& io, int groups, int w) {
+    const int lane = threadIdx.x & 63;
+    for (int base = 0; base < groups; base += 64) {  // uniform per wave
+        const unsigned c = (base + lane < groups) ? io.count[base + lane] : 0u;
+        unsigned incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned o = __shfl_up(incl, off, 64);
+            incl += (lane >= off) ? o : 0u;
+        }
+        const unsigned total = __shfl(incl, 63, 64);
+        if ((unsigned)w < total) {
+            const unsigned long long m = __ballot(incl > (unsigned)w);
+            const int g = __builtin_ctzll(m);
+            const unsigned excl = __shfl(incl - c, g, 64);
+            return (int)__builtin_amdgcn_readfirstlane(
+                io.list[(size_t)(base + g) * kPlanGroup * kPlanPieces + (w - excl)]);
+        }
+        w -= (int)total;
+    }
+    return -1;
+}
 
 struct Q3 {
     float h, hu, hv;
@@ -58,20 +136,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->smax[slotClear] = 0u;
         ctl->defer[slotClear] = 0u;
+        ctl->busy[slotClear] = 0u;
         ctl->dt = dt;
         ctl->t += dt;
         ctl->steps += 1;
         if (dtLog) dtLog[logIdx] = dt;
     }
-    if (wave >= nyt * nxs) return;  // uniform per wave
+    // the in-sweep network variant keeps no record (its registers are at the
+    // three-wave limit; the device loop runs the deferring variant instead),
+    // nor do widths that are not a whole number of record tiles
+    constexpr bool kRecord = S != SOLVER_AUGRIE_MLP && XS % kQuietCols == 0;
+    const int items = nyt * nxs;
+    int item = wave, first = 0, ncols = XS;
+    if (kRecord && io.list) {
+        const int e = listed_item(io, (items + kPlanGroup - 1) / kPlanGroup, wave);
+        if (e < 0) return;  // uniform per wave
+        item = e >> 6;
+        first = (e >> 3) & 7;
+        ncols = ((e & 7) + 1) * kQuietCols;
+    }
+    if (item >= items) return;  // uniform per wave
+    const int yt = item % nyt;
+    const int xs = item / nyt;
+    const int slotNow = (io.step & 1) * nyt * ((nx + kQuietCols - 1) / kQuietCols);
     // the Newton loop's first iteration from the bracket (middle_state<true>):
     // measured 3% faster at seven waves per SIMD, same results
     const dev::SolverCtx<S, true> solver(S == SOLVER_AUGRIE_MLP ? ctl->mlp : nullptr);
 
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;
-    const int yt = wave % nyt;
-    const int xs = wave / nyt;
     const int y = kStripRows * yt + lane;
     const bool rowValid = y <= ny + 1;
     const int yc = rowValid ? y : ny + 1;
@@ -81,8 +174,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
     const bool rowBottom = (y == 0);
     const bool rowTop = (y == ny + 1);
     const bool carryRow = owned || (rowBottom && lane == 0) || (rowTop && lane >= 1);
-    const int x0 = 1 + xs * XS;
-    const int x1 = min(x0 + XS - 1, nx);
+    const int x0 = 1 + xs * XS + first * kQuietCols;
+    const int x1 = min(x0 + ncols - 1, nx);
     const int tB = bi.type[BND_BOTTOM], tT = bi.type[BND_TOP], tL = bi.type[BND_LEFT], tR = bi.type[BND_RIGHT];
 
     // Q(k) of the cell (x, yc): interior cells updated, ghosts as stored
@@ -135,10 +228,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
     CellQ pq = cell_q(prev.h, prev.hu, prev.hv, cb);
     CellQ cq = cell_q(cur.h, cur.hu, cur.hv, nb);
     dev::EdgeOut e = solver.solve(pq.xside(), cq.xside());
-    if constexpr (S == SOLVER_AUGRIE_DEFER)
+    // record tile of the columns being solved: a deferred edge touching it
+    // makes it not quiet; the halo edge (x0-1 | x0) counts for the first
+    bool loud = false;
+    if constexpr (S == SOLVER_AUGRIE_DEFER) {
+        loud = e.speed < 0.f;
         e = defer_edge(e, owned && x0 == 1, make_uint2(0u, (unsigned)y), io.deferQ, &ctl->defer[slotOut]);
+    }
     float carryH = e.hR, carryHu = e.huR;
-    float smaxv = rowInterior ? e.speed : 0.f;
+    // x-edges count on owned rows (each interior row is owned by one strip),
+    // so a record tile's maximum depends on its footprint only
+    float smaxTile = owned ? e.speed : 0.f;
+    // the segment's maximum over its record tiles, wave-uniform (float bits:
+    // non-negative floats order like unsigned integers)
+    unsigned smaxRun = 0u;
+    // kQuiet* bits of the record tile's parts whose owned sums were not all +0
+    unsigned loudParts = 0u;
 
     for (int x = x0; x <= x1; ++x) {
         // column x+1 of Q(k)
@@ -151,31 +256,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
         if (x + 1 == nx + 1 && carryRow) store(nx + 1, nxt);
 
         dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
-        if constexpr (S == SOLVER_AUGRIE_DEFER)
+        bool loudEdge = false;
+        if constexpr (S == SOLVER_AUGRIE_DEFER) {
+            loudEdge = ev.speed < 0.f;
             ev = defer_edge(ev, owned, make_uint2((unsigned)x, (unsigned)y), io.deferQ, &ctl->defer[slotOut]);
-        if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
+        }
+        if (owned) smaxTile = dev::smax(smaxTile, ev.speed);
         dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
-        if constexpr (S == SOLVER_AUGRIE_DEFER)
+        if constexpr (S == SOLVER_AUGRIE_DEFER) {
+            loud = loud || loudEdge || eh.speed < 0.f;
             eh = defer_edge(eh, owned || y == 0, make_uint2((unsigned)x | kDeferY, (unsigned)y), io.deferQ,
                             &ctl->defer[slotOut]);
-        if (hEdgeValid) smaxv = dev::smax(smaxv, eh.speed);
+        }
+        if (hEdgeValid) smaxTile = dev::smax(smaxTile, eh.speed);
         const float aH = __shfl_up(eh.hR, 1, 64);
         const float aHv = __shfl_up(eh.huR, 1, 64);
         if (owned) {
             const size_t c = idx(v, x, y);
-            io.sOut.sx[c] = carryH + ev.hL;
-            io.sOut.su[c] = carryHu + ev.huL;
-            io.sOut.sy[c] = aH + eh.hL;
-            io.sOut.sv[c] = aHv + eh.huL;
+            const float sx = carryH + ev.hL, su = carryHu + ev.huL, sy = aH + eh.hL, sv = aHv + eh.huL;
+            io.sOut.sx[c] = sx;
+            io.sOut.su[c] = su;
+            io.sOut.sy[c] = sy;
+            io.sOut.sv[c] = sv;
+            if constexpr (kRecord)
+                if ((__float_as_uint(sx) | __float_as_uint(su) | __float_as_uint(sy) | __float_as_uint(sv)) != 0u)
+                    loudParts |= kQuietAll | (lane == 1 ? kQuietBottom : 0u) | (lane == kStripRows ? kQuietTop : 0u) |
+                                 (((x - 1) & (kQuietCols - 1)) == 0 ? kQuietLeft : 0u) |
+                                 ((x & (kQuietCols - 1)) == 0 || x == nx ? kQuietRight : 0u);
         }
         carryH = ev.hR;
         carryHu = ev.huR;
         cur = nxt;
         cq = nq;
+        // the end of a record tile: its parts' flags and its maximum.  A tile
+        // reports its right edge (x | x+1) and, first in a run, its left halo
+        // edge; the tile of column 1 always starts a run, so every x-edge is
+        // in some record, and each record's edges lie in its tile's footprint.
+        // A deferred edge at the boundary touches both tiles.
+        if (kRecord && ((x & (kQuietCols - 1)) == 0 || x == x1)) {  // uniform per wave
+            const float tileMax = wave_max(smaxTile);
+            smaxRun = max(smaxRun, __builtin_amdgcn_readfirstlane(__float_as_uint(tileMax)));
+            if (io.quiet) {
+                unsigned quiet = 0u;
+#pragma unroll
+                for (unsigned part = 1u; part <= kQuietRight; part <<= 1)
+                    if (!__any((loudParts & part) != 0u || loud)) quiet |= part;
+                if (lane == 0) {
+                    const int rec = slotNow + yt + nyt * ((x - 1) / kQuietCols);
+                    io.quiet[rec] = quiet;
+                    io.speed[rec] = tileMax;
+                }
+            }
+            smaxTile = 0.f;
+            loud = loudEdge;
+            loudParts = 0u;
+        }
     }
 
-    smaxv = wave_max(smaxv);
-    if (lane == 0) atomicMax(&ctl->smax[slotOut], __float_as_uint(smaxv));
+    if constexpr (!kRecord) smaxRun = __float_as_uint(wave_max(smaxTile));
+    if (lane == 0) atomicMax(&ctl->smax[slotOut], smaxRun);
 }
 
 }  // namespace swe

@@ -40,6 +40,8 @@ enum SweepMode : int { MODE_MAX = 0, MODE_STORE = 1, MODE_UPDATE = 2 };
 
 constexpr int kStripRows = 62;  // owned rows per wavefront
 constexpr int kWavesPerGroup = 4;
+constexpr int kPlanGroup = 1024;  // wave segments per workgroup of the quiet-tile plan
+constexpr int kPlanPieces = 8;    // list entries per segment at most (32 columns in runs of 4)
 
 // ---- strip tiles of the batched engine --------------------------------------
 // The strips of kStripRows rows below a block's last strip are cut into
@@ -145,6 +147,7 @@ struct DeviceCtl {
     int pad[2];
     const unsigned* mlp;  // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
     unsigned int defer[3];  // edges queued by the deferring sweep, per CFL slot (SOLVER_AUGRIE_DEFER)
+    unsigned int busy[3];   // busy record tiles found by the quiet-tile plan of each CFL slot's launch
 };
 
 // Which CFL-maximum slot a launch reads or accumulates into, and which slot

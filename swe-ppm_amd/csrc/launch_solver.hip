@@ -96,11 +96,15 @@ int launch_fused_u_solver(int xseg, StateView v, FusedIO io, BoundaryInfo bi, De
     auto go = [&](auto xsTag) {
         constexpr int XS = decltype(xsTag)::value;
         const int nxs = (v.nx + XS - 1) / XS;
-        const int groups = (nyt * nxs + kWavesPerGroup - 1) / kWavesPerGroup;
+        // list mode may list up to one piece per record tile
+        const int waves = io.list ? nyt * ((v.nx + kQuietCols - 1) / kQuietCols) : nyt * nxs;
+        const int groups = (waves + kWavesPerGroup - 1) / kWavesPerGroup;
         hipLaunchKernelGGL((fused_u_kernel<S, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v, io, bi,
                            ctl, slotIn, slotOut, slotClear, dtLog, logIdx);
     };
-    if (xseg == 10)
+    if (xseg == 4)
+        go(std::integral_constant<int, 4>());
+    else if (xseg == 10)
         go(std::integral_constant<int, 10>());
     else if (xseg == 12)
         go(std::integral_constant<int, 12>());

@@ -321,8 +321,35 @@ __device__ __forceinline__ AugPrep augrie_prepare(const Side& Lin, const Side& R
     return p;
 }
 
+// A still edge: both cells hold at least kStillTol of water, zero normal
+// momentum on both sides, the same free surface h + b in single precision,
+// and depths within a factor kStillRatio.  It is a lake at rest, whose exact
+// Riemann solution carries no flux: the solver returns zero net updates and
+// the speed the full decomposition reports, max(|sE1|, |sE2|) with
+// sE1 = min(-cL, -cHat), sE2 = max(cR, cHat) (at rest u = uHat = 0, and for
+// depths within a factor 2 the middle-state speeds never replace the
+// Einfeldt speeds: s2m > 0 > sE1 and s1m < 0 < sE2).  Still water thus stays
+// exactly still, and a wave whose edges are all still skips the middle state
+// and the decomposition.  In a wave that mixes still and moving edges every
+// lane runs the decomposition and the still lanes take the still result at
+// the end: the learned middle state runs its network for the whole wave
+// and must be reached with every lane active.
+__device__ __forceinline__ bool still_edge(const Side& L, const Side& R) {
+    return L.h >= kStillTol && R.h >= kStillTol && L.hu == 0.f && R.hu == 0.f && L.h + L.b == R.h + R.b &&
+           L.h <= kStillRatio * R.h && R.h <= kStillRatio * L.h;
+}
+
+__device__ __forceinline__ float still_speed(const Side& L, const Side& R) {
+    const float cHat = sqrtf(kGravity * 0.5f * (R.h + L.h));
+    const float sE1 = smin(-L.c, -cHat);
+    const float sE2 = smax(R.c, cHat);
+    return smax(fabsf(sE1), fabsf(sE2));
+}
+
 template <typename MiddleFn>
 __device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin, const MiddleFn& middle) {
+    const bool still = still_edge(Lin, Rin);
+    if (__all(still)) return EdgeOut{0.f, 0.f, 0.f, 0.f, still_speed(Lin, Rin)};
     const AugPrep p = augrie_prepare(Lin, Rin);
     const bool dryBoth = p.dryBoth;
     const bool wallL = p.wallL, wallR = p.wallR;
@@ -415,6 +442,7 @@ __device__ __forceinline__ EdgeOut augrie_with(const Side& Lin, const Side& Rin,
     // edge is marked by a negative speed and solved again elsewhere
     if constexpr (MiddleFn::kDefers)
         if (mid.hm < 0.f) o.speed = -1.f;
+    if (still) o = EdgeOut{0.f, 0.f, 0.f, 0.f, still_speed(Lin, Rin)};
     return o;
 }
 

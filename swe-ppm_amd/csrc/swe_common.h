@@ -50,6 +50,10 @@ constexpr float kNewtonTol = 1e-6f;
 constexpr int kMaxNewton = 10;
 constexpr float kCriticalTol = 1e-6f;
 constexpr float kHybridTol = 1.0f;
+// still edges of the augmented solver: both depths >= kStillTol and within a
+// factor kStillRatio of each other
+constexpr float kStillTol = 0.1f;
+constexpr float kStillRatio = 2.0f;
 
 // Block-level constants (src/Constants.hh:4-6, SWE_DimensionalSplittingMpi.cpp:448-456, 502-515).
 constexpr float kCfl = 0.4f;

@@ -65,6 +65,13 @@ struct swe_hip_block {
     const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
     uint2* deferQ = nullptr;        // deferred-edge queue of the learned middle state's fused loop
     unsigned deferCap = 0;          // its capacity: every edge of the block
+    // quiet-tile record of the fused loop (kernels_fused.hpp): two slots of
+    // one flag and one speed per strip tile, sized for the narrowest width
+    unsigned* quiet = nullptr;
+    float* tspeed = nullptr;
+    unsigned* tlist = nullptr;   // tiles listed for the next fused launch, per plan group
+    unsigned* tcount = nullptr;  // their number per plan group
+    bool skipQuiet = true;  // skip quiet tiles (swe_hip_set_quiet_skip)
 };
 
 namespace {
@@ -172,6 +179,22 @@ int ensure_defer_queue(swe_hip_block* k) {
     return 0;
 }
 
+// Strip tiles of the narrowest width: the most any width needs.
+size_t max_tiles(const swe_hip_block* k) {
+    return (size_t)((k->ny + kStripRows - 1) / kStripRows) * (size_t)((k->nx + 3) / 4);
+}
+
+int ensure_quiet_record(swe_hip_block* k) {
+    if (k->quiet) return 0;
+    const size_t n = 2 * max_tiles(k);
+    HIP_TRY(hipMalloc(&k->quiet, n * sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&k->tspeed, n * sizeof(float)));
+    const size_t groups = (max_tiles(k) + kPlanGroup - 1) / kPlanGroup;
+    HIP_TRY(hipMalloc(&k->tlist, groups * kPlanGroup * kPlanPieces * sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&k->tcount, groups * sizeof(unsigned)));
+    return 0;
+}
+
 // Workgroups of the deferred-edge launch: one round of waves at the learned
 // solver's three waves per SIMD.
 int defer_groups() {
@@ -215,8 +238,10 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
     if (int r = ensure_dtlog(k, nsteps)) return r;
     if (k->strategy == 0 || k->strategy == 2)
         if (int r = ensure_second_buffer(k)) return r;
-    if (k->strategy == 2)
+    if (k->strategy == 2) {
         if (int r = ensure_second_sums(k)) return r;
+        if (int r = ensure_quiet_record(k)) return r;
+    }
     // the learned middle state's fused loop defers its hard edges to a second launch per step
     const bool defer = k->strategy == 2 && nsteps > 1 && eff_solver(k) == SOLVER_AUGRIE_MLP;
     if (defer)
@@ -241,6 +266,18 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             const int o = k->cur ^ 1;
             FusedIO io{k->h[k->cur], k->hu[k->cur], k->hv[k->cur], k->b, k->h[o], k->hu[o], k->hv[o],
                        set[(s - 1) & 1], set[s & 1]};
+            // quiet tiles: recorded from the first fused launch, skipped from the third
+            io.quiet = k->quiet;
+            io.speed = k->tspeed;
+            io.step = s;
+            // (widths that are not a whole number of record tiles keep no record)
+            if (k->skipQuiet && s >= 3 && k->xseg % kQuietCols == 0) {
+                if (int r = launch_quiet_plan(k->quiet, k->tspeed, k->nx, k->ny, k->xseg, s, k->ctl, s % 3, k->tlist,
+                                              k->tcount, k->stream))
+                    return r;
+                io.list = k->tlist;
+                io.count = k->tcount;
+            }
             if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
             if (defer) {
                 io.deferQ = k->deferQ;
@@ -503,6 +540,10 @@ int swe_hip_destroy(swe_hip_block* k) {
     if (k->ctl) (void)hipFree(k->ctl);
     if (k->dtlog) (void)hipFree(k->dtlog);
     if (k->deferQ) (void)hipFree(k->deferQ);
+    if (k->quiet) (void)hipFree(k->quiet);
+    if (k->tspeed) (void)hipFree(k->tspeed);
+    if (k->tlist) (void)hipFree(k->tlist);
+    if (k->tcount) (void)hipFree(k->tcount);
     if (k->stream) (void)hipStreamDestroy(k->stream);
     delete k;
     return SWE_HIP_OK;
@@ -921,8 +962,8 @@ int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
         set_error("swe_hip_set_tuning: strategy must be 0, 1 or 2");
         return SWE_HIP_EINVAL;
     }
-    if (xseg != -1 && xseg != 8 && xseg != 10 && xseg != 12 && xseg != 16 && xseg != 32) {
-        set_error("swe_hip_set_tuning: xseg must be 8, 10, 12, 16 or 32");
+    if (xseg != -1 && xseg != 4 && xseg != 8 && xseg != 10 && xseg != 12 && xseg != 16 && xseg != 32) {
+        set_error("swe_hip_set_tuning: xseg must be 4, 8, 10, 12, 16 or 32");
         return SWE_HIP_EINVAL;
     }
     if (strategy != -1) k->strategy = strategy;
@@ -930,6 +971,16 @@ int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
         k->xseg = xseg;
         k->xsegSet = true;
     }
+    return SWE_HIP_OK;
+}
+
+int swe_hip_set_quiet_skip(swe_hip_block* k, int enable) {
+    if (int r = check(k)) return r;
+    if (enable != 0 && enable != 1) {
+        set_error("swe_hip_set_quiet_skip: enable must be 0 or 1");
+        return SWE_HIP_EINVAL;
+    }
+    k->skipQuiet = enable != 0;
     return SWE_HIP_OK;
 }
 
@@ -978,8 +1029,12 @@ int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
                                hipMemcpyDeviceToDevice, k->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
     };
-    if (e == hipSuccess) copy_state(true);
-    const int widths[SWE_HIP_TUNE_WIDTHS] = {8, 10, 12, 16, 32};
+    bool saved = false;
+    if (e == hipSuccess) {
+        copy_state(true);
+        saved = e == hipSuccess;
+    }
+    const int widths[SWE_HIP_TUNE_WIDTHS] = {4, 8, 10, 12, 16, 32};
     const int oldXseg = k->xseg;
     int best = oldXseg, rc = 0;
     float bestMs = 0.f;
@@ -998,6 +1053,13 @@ int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
             best = widths[i];
         }
         copy_state(false);
+    }
+    // a failed timed run leaves the block half-advanced: put the snapshot back
+    if (saved && (rc || e != hipSuccess)) {
+        const hipError_t e0 = e;
+        e = hipSuccess;
+        copy_state(false);
+        if (e0 != hipSuccess) e = e0;
     }
     for (float* p : save)
         if (p) (void)hipFree(p);

@@ -75,6 +75,7 @@ def _load() -> C.CDLL:
                                               C.POINTER(C.c_int)]),
         "swe_hip_solve_edges": (C.c_int, [C.c_int, C.c_int, C.c_int64] + [fp] * 11),
         "swe_hip_set_math": (C.c_int, [vp, C.c_int]),
+        "swe_hip_set_quiet_skip": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
         "swe_hip_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "swe_hip_batch_create": (C.c_int, [C.c_int, ip, ip, fp, fp, ip, ip, C.c_int, C.c_int, C.c_float, C.c_float,
@@ -235,7 +236,7 @@ class SweBlock:
     def set_tuning(self, strategy: int = -1, xseg: int = -1) -> None:
         _check(lib.swe_hip_set_tuning(self._h, strategy, xseg), "set_tuning")
 
-    TUNE_WIDTHS = (8, 10, 12, 16, 32)
+    TUNE_WIDTHS = (4, 8, 10, 12, 16, 32)
 
     def autotune(self, steps: int = 20) -> dict:
         """Time `steps` steps at each strip width from the current state,
@@ -247,6 +248,11 @@ class SweBlock:
     def set_math(self, math: int) -> None:
         """0 = IEEE (bit-compatible with the CPU oracle), 1 = fast."""
         _check(lib.swe_hip_set_math(self._h, int(math)), "set_math")
+
+    def set_quiet_skip(self, enable: bool) -> None:
+        """Skip quiet strip tiles in the fused device loop (default on);
+        results are identical either way."""
+        _check(lib.swe_hip_set_quiet_skip(self._h, int(bool(enable))), "set_quiet_skip")
 
     def set_surrogate(self, mlp) -> None:
         """Attach a MiddleStateMLP (None detaches): AugRie then starts its

@@ -146,15 +146,20 @@ def island_bumps(seed: int, length: float = 1000.0, nbumps: int = 16, n_islands:
     raise RuntimeError("could not draw an island configuration")
 
 
-def dam_break_islands(n: int = 2048, seed: int = 2048, length: float = 1000.0):
+def dam_break_islands(n: int = 2048, seed: int = 2048, length: float = 1000.0, surface_noise: float = 0.0):
     """Dam break (15 m within 100 m of the centre, 10 m elsewhere) over 16
     seeded Gaussian bumps, exactly 3 of which emerge as dry islands;
-    h = max(0, eta - b); outflow boundaries."""
+    h = max(0, eta - b); outflow boundaries.  surface_noise > 0 adds seeded
+    uniform noise of that amplitude (m) to eta everywhere, so that no edge
+    starts at rest (the fully disturbed form of the workload)."""
     dx = np.float32(length / n)
     X, Y = cell_centres(n, n, dx, dx)
     centres, amps, sigmas = island_bumps(seed, length)
     b = gaussian_bumps(X, Y, centres, amps, sigmas)
     eta = np.where(_radius(X, Y, length / 2, length / 2) < np.float32(100.0), np.float32(15.0), np.float32(10.0))
+    if surface_noise > 0.0:
+        rng = np.random.default_rng(seed + 1)
+        eta = (eta + rng.uniform(-surface_noise, surface_noise, eta.shape).astype(np.float32)).astype(np.float32)
     h = np.maximum(np.float32(0.0), eta - b).astype(np.float32)
     bnd = [OUTFLOW] * 4
     return (*finish(h, b, bnd), float(dx), float(dx), bnd)

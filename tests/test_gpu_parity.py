@@ -58,7 +58,7 @@ def test_c1_fwave_256_200_steps(swe, oracle):
 
 
 @pytest.mark.parametrize("strategy,xseg", [(1, 16), (0, 16), (1, 8), (0, 32), (2, 8), (2, 16), (2, 32), (2, 10),
-                                           (2, 12), (1, 12)])
+                                           (2, 12), (1, 12), (2, 4), (1, 4)])
 def test_islands_augrie_parity(swe, oracle, strategy, xseg):
     from swe_ppm_amd import scenarios
 
@@ -169,7 +169,7 @@ def test_invalid_arguments_fail_loudly(swe):
         with pytest.raises(swe.SweHipError):
             blk.set_tuning(7, -1)
         with pytest.raises(swe.SweHipError):
-            blk.set_tuning(-1, 11)  # strip widths are 8, 10, 12, 16 or 32
+            blk.set_tuning(-1, 11)  # strip widths are 4, 8, 10, 12, 16 or 32
 
 
 def test_autotune_restores_the_state_and_keeps_the_results(swe):
@@ -189,7 +189,7 @@ def test_autotune_restores_the_state_and_keeps_the_results(swe):
         blk.upload(h, hu, hv, b)
         before = blk.download()
         ms = blk.autotune(4)
-        assert sorted(ms) == [8, 10, 12, 16, 32] and all(v > 0.0 for v in ms.values())
+        assert sorted(ms) == [4, 8, 10, 12, 16, 32] and all(v > 0.0 for v in ms.values())
         assert blk.get_tuning()[1] == min(ms, key=ms.get)
         for a, c in zip(blk.download(), before):
             assert same_bits(a, c)
@@ -273,3 +273,151 @@ def test_grid_scenario_known_answer_and_errors(swe):
                     (np.ones((0, 2), np.float32), 0.0, 0.0, 1.0, 1.0)]:
             with pytest.raises(swe.SweHipError):
                 blk.init_grid_scenario(bad)
+
+
+@pytest.mark.parametrize("solver", [2, 3])
+def test_solve_edges_still_and_almost_still_bitwise(swe, oracle, solver):
+    """Still edges (a lake at rest: zero net updates and the full solver's
+    speed) and edges one condition short of still, per edge against the
+    oracle, bit for bit."""
+    from test_oracle_solvers import still_edges
+
+    hL, hR, huL, huR, bL, bR = still_edges(20000, 51)
+    f = np.float32
+    n = hL.size
+    sets = [(hL, hR, huL, huR, bL, bR),
+            (hL, hR, np.full(n, f(1e-30)), huR, bL, bR),
+            (hL, (np.nextafter(hR + bR, f(np.inf)) - bR).astype(f), huL, huR, bL, bR),
+            (np.full(n, f(0.09)), np.full(n, f(0.09)), huL, huR, np.zeros(n, f), np.zeros(n, f))]
+    ins = [np.concatenate([s[k] for s in sets]).astype(f) for k in range(6)]
+    g = swe.solve_edges(solver, *ins)
+    o = oracle.solve_edges(solver, *ins)
+    for k, (a, b) in enumerate(zip(g, o)):
+        assert same_bits(a, b), f"output {k}"
+    if solver == 2:  # Hybrid sends edges with both depths >= 1 m to the f-wave solver
+        assert np.all(g[0][:n] == 0.0) and np.all(g[3][:n] == 0.0)
+
+
+@pytest.mark.parametrize("solver", [2, 3])
+def test_flavour_s_parity_dx_ne_dy(swe, oracle, solver):
+    """Flavour S on a rectangular block with dx != dy (the reference's split
+    step uses dx for the y-part and only the x-speeds for dt): bit-identical
+    to the oracle, on 2-D data with currents both ways and dry cells."""
+    rng = np.random.default_rng(52)
+    nx, ny, dx, dy = 70, 45, 2.0, 7.0
+    x = np.arange(nx + 2)[:, None]
+    y = np.arange(ny + 2)[None, :]
+    h = (4.0 + 2.0 * np.sin(x / 5.0) + 1.5 * np.cos(y / 4.0) + rng.uniform(0, 0.3, (nx + 2, ny + 2)))
+    h[30:34, 10:14] = 0.0
+    hu = h * rng.uniform(-1.0, 1.0, h.shape)
+    hv = h * rng.uniform(-1.0, 1.0, h.shape)
+    b = rng.uniform(-0.3, 0.3, h.shape)
+    h, hu, hv, b = (a.astype(np.float32) for a in (h, hu, hv, b))
+    bnd = [0, 1, 0, 1]
+    nsteps = 20
+    with swe.SweBlock(nx, ny, dx, dy, solver=solver, flavour=swe.FLAVOUR_S) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        t, gdts = blk.run(nsteps)
+        gh, ghu, ghv = blk.download()
+    oh, ohu, ohv, odts = oracle.run_s(h, hu, hv, b, dx, bnd, solver, nsteps)
+    _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
+
+
+def still_lake(n: int, seed: int, eta: float = 10.0):
+    """A lake at rest over smooth random bathymetry, built so that every wet
+    edge is still in single precision: depths in [3, 5.9] m (neighbours
+    within a factor 2), h + b == eta exactly in float32, plus square dry
+    islands (h = 0, bed above the surface)."""
+    from swe_ppm_amd import scenarios
+
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    k = np.arange(n + 2, dtype=np.float64)
+    ph = rng.uniform(0, 2 * np.pi, 4)
+    field = (np.sin(k[:, None] / 97.0 + ph[0]) * np.cos(k[None, :] / 61.0 + ph[1]) +
+             0.5 * np.sin(k[:, None] / 23.0 + ph[2]) * np.sin(k[None, :] / 41.0 + ph[3]))
+    h = (3.0 + 2.9 * (field - field.min()) / (field.max() - field.min())).astype(f)
+    b = (f(eta) - h).astype(f)
+    for _ in range(4):  # make h + b == eta exact in float32
+        bad = (h + b) != f(eta)
+        h[bad] = (f(eta) - b[bad]).astype(f)
+    bad = (h + b) != f(eta)
+    h[bad], b[bad] = f(0.0), f(eta + 2.0)
+    for cx, cy in rng.integers(n // 8, n - n // 8, size=(3, 2)):
+        h[cx:cx + n // 40, cy:cy + n // 30] = f(0.0)
+        b[cx:cx + n // 40, cy:cy + n // 30] = f(eta + 2.0)
+    bnd = [0, 0, 0, 0]
+    z = np.zeros_like(h)
+    h, hu, hv, b = scenarios.finish(h, b, bnd)
+    return h, hu, hv, b, bnd
+
+
+def test_lake_at_rest_2048_stays_exactly_still(swe, oracle):
+    """A 2048^2 lake at rest over smooth bathymetry with dry islands: AugRie
+    keeps every cell bit for bit at rest over 60 steps of the fused loop
+    (every wet edge is still, every wet/dry edge a wall at rest), with the
+    oracle's dts."""
+    n = 2048
+    h, hu, hv, b, bnd = still_lake(n, 53)
+    dx = 1000.0 / n
+    (gh, ghu, ghv), t, dts = _run_gpu(swe, h, hu, hv, b, dx, dx, bnd, swe.AUGRIE, 60, strategy=2, xseg=-1)
+    assert same_bits(gh[1:-1, 1:-1], h[1:-1, 1:-1])
+    assert np.all(ghu[1:-1, 1:-1] == 0.0) and np.all(ghv[1:-1, 1:-1] == 0.0)
+    odts = oracle.run_u(h, hu, hv, b, dx, dx, bnd, swe.AUGRIE, 2, nthreads=16)[3]
+    assert same_bits(dts[:2], odts) and np.all(dts == dts[0])
+
+
+@pytest.mark.parametrize("solver,n,steps,xseg", [(2, 512, 300, 12), (3, 384, 200, 16), (2, 200, 150, 8), (2, 300, 200, 4),
+                                                 (2, 260, 120, 32), (2, 130, 60, 16),
+                                                 (0, 256, 100, 16), (1, 256, 100, 32)])
+def test_quiet_skip_is_bit_identical(swe, oracle, solver, n, steps, xseg):
+    """The fused loop with quiet tiles skipped against the same loop computing
+    every tile and against the oracle: identical states and dts, on dam breaks
+    whose still far field makes most tiles quiet for most of the run (walls
+    on two sides, dry islands)."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, _ = scenarios.dam_break_islands(n=n, seed=9)
+    bnd = [0, 1, 1, 0]
+    scenarios.apply_bc(h, hu, hv, bnd)
+    scenarios.apply_bc_bathymetry(b, bnd)
+    got = {}
+    for skip in (1, 0):
+        with swe.SweBlock(n, n, dx, dy, solver=solver) as blk:
+            blk.set_boundaries(bnd)
+            blk.set_tuning(2, xseg)
+            blk.set_quiet_skip(bool(skip))
+            blk.upload(h, hu, hv, b)
+            t1, d1 = blk.run(steps // 2)
+            t2, d2 = blk.run(steps - steps // 2)  # a second run on the same block starts a new record
+            got[skip] = (blk.download(), np.concatenate([d1, d2]))
+    for f in range(3):
+        assert same_bits(got[1][0][f], got[0][0][f]), f
+    assert same_bits(got[1][1], got[0][1])
+    if n <= 256:
+        o = oracle.run_u(h, hu, hv, b, dx, dy, bnd, solver, steps, nthreads=16)
+        for f in range(3):
+            assert same_bits(got[1][0][f][1:-1, 1:-1], o[f][1:-1, 1:-1]), f
+
+
+def test_quiet_skip_learned_middle_state_bit_identical(swe):
+    """The learned middle state's deferring fused loop with quiet tiles
+    skipped against the same loop computing every tile."""
+    from swe_ppm_amd import scenarios, surrogate
+
+    net = surrogate.load_default()
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=384, seed=10)
+    got = {}
+    for skip in (1, 0):
+        with swe.SweBlock(384, 384, dx, dy, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            blk.set_surrogate(net)
+            blk.set_quiet_skip(bool(skip))
+            blk.upload(h, hu, hv, b)
+            _, d = blk.run(120)
+            got[skip] = (blk.download(), d)
+    net.close()
+    for f in range(3):
+        assert same_bits(got[1][0][f], got[0][0][f]), f
+    assert same_bits(got[1][1], got[0][1])

@@ -221,3 +221,109 @@ def test_float32_close_to_float64(oracle):
             err = np.abs(a - b) / scale
             # single-precision round-off, except edges next to a branch threshold
             assert np.quantile(err, 0.99) < 1e-4, name
+
+
+def still_edges(n, seed):
+    """Random still edges in single precision: both depths >= 0.1 m and within
+    a factor 2, zero normal momentum (+0 and -0), and the same free surface
+    h + b in float32."""
+    rng = np.random.default_rng(seed)
+    f = np.float32
+    eta = rng.uniform(-20.0, 40.0, 4 * n).astype(f)
+    hL = rng.uniform(0.1, 60.0, 4 * n).astype(f)
+    hR = (hL * rng.uniform(0.5, 2.0, 4 * n)).astype(f)
+    bL = (eta - hL).astype(f)
+    bR = (eta - hR).astype(f)
+    huL = np.where(rng.uniform(size=4 * n) < 0.5, f(0.0), f(-0.0)).astype(f)
+    huR = np.where(rng.uniform(size=4 * n) < 0.5, f(0.0), f(-0.0)).astype(f)
+    keep = ((hL + bL) == (hR + bR)) & (hL >= f(0.1)) & (hR >= f(0.1)) & (hL <= f(2) * hR) & (hR <= f(2) * hL)
+    # the factor-2 bound itself, both ways
+    hL[:4], hR[:4] = f(3.0), f(6.0)
+    hL[4:8], hR[4:8] = f(6.0), f(3.0)
+    bL[:8], bR[:8] = f(1.0) - hL[:8], f(1.0) - hR[:8]
+    keep[:8] = True
+    idx = np.flatnonzero(keep)[:n]
+    return [a[idx] for a in (hL, hR, huL, huR, bL, bR)]
+
+
+def test_augrie_still_edges_zero_updates_and_full_speed(oracle):
+    """A still edge (a lake at rest) gives exactly zero net updates, and the
+    speed the full decomposition reports, bit for bit; the full
+    decomposition's own updates on these edges are round-off of the
+    hydrostatic balance (relative to g h^2), and its speed in FP64 is
+    max(cL, cHat, cR) (u = uHat = 0: the Einfeldt speeds)."""
+    hL, hR, huL, huR, bL, bR = still_edges(20000, 31)
+    assert hL.size == 20000
+    o = oracle.solve_edges(2, hL, hR, huL, huR, bL, bR)
+    full = oracle.solve_edges(12, hL, hR, huL, huR, bL, bR)
+    for a in o[:4]:
+        assert np.all(a == 0.0) and not np.any(np.signbit(a))
+    assert np.array_equal(o[4].view(np.uint32), full[4].view(np.uint32))
+    scale = G * np.maximum(hL, hR).astype(np.float64) ** 2
+    for a in full[:4]:
+        assert np.max(np.abs(a) / scale) < 1e-5
+    d = [a.astype(np.float64) for a in (hL, hR, huL, huR, bL, bR)]
+    f64 = oracle.solve_edges(12, *d, dtype=np.float64)
+    want = np.sqrt(G * np.maximum(np.maximum(d[0], d[1]), 0.5 * (d[0] + d[1])))
+    np.testing.assert_allclose(f64[4], want, rtol=1e-12)
+    np.testing.assert_allclose(o[4], want, rtol=3e-7)
+
+
+def test_augrie_not_still_takes_the_full_solver(oracle):
+    """Edges one condition short of still (a depth below 0.1 m, any normal
+    momentum, a free-surface step of one ulp, depths more than a factor 2
+    apart) go through the full decomposition: the same bits as without the
+    still path, and non-zero updates where the state is not at rest."""
+    hL, hR, huL, huR, bL, bR = still_edges(4000, 32)
+    f = np.float32
+    cases = {
+        "shallow": (np.full_like(hL, f(0.09)), np.full_like(hL, f(0.09)), huL, huR, bL * 0, bL * 0),
+        "momentum": (hL, hR, np.full_like(hL, f(1e-30)), huR, bL, bR),
+        "surface": (hL, (np.nextafter(hR + bR, f(np.inf)) - bR).astype(f), huL, huR, bL, bR),
+        "ratio": (np.full_like(hL, f(2.0)), np.full_like(hL, f(4.5)), huL, huR,
+                  np.full_like(hL, f(2.5)), np.zeros_like(hL)),
+    }
+    for name, args in cases.items():
+        args = [np.asarray(a, np.float32) for a in args]
+        if name == "surface":  # keep the edges whose free surfaces differ in float32
+            m = args[0] + args[4] != args[1] + args[5]
+            assert m.mean() > 0.5
+            args = [a[m] for a in args]
+            cases[name] = args
+        o = oracle.solve_edges(2, *args)
+        full = oracle.solve_edges(12, *args)
+        for a, b in zip(o, full):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+    # the surface step moves water: the decomposition sees it
+    o = oracle.solve_edges(2, *cases["surface"])
+    assert np.count_nonzero(o[0]) > 0.9 * o[0].size
+
+
+def test_augrie_still_lake_stays_exactly_still(oracle):
+    """Single precision, 2-D: a lake at rest over random bathymetry with dry
+    islands stays bit for bit at rest (AugRie); f-wave, which has no still
+    path, moves it only by round-off."""
+    rng = np.random.default_rng(33)
+    n = 40
+    f = np.float32
+    eta = f(5.0)
+    h = rng.uniform(0.2, 6.0, (n + 2, n + 2)).astype(f)
+    # keep depths of neighbours within a factor 2 (smooth bottom) and islands
+    h = (0.5 * (h + np.roll(h, 1, 0))).astype(f)
+    h = np.clip(h, f(2.0), f(3.9)).astype(f)
+    b = (eta - h).astype(f)
+    still = (h + b) == eta
+    h[~still] = f(0.0)  # cells whose surface rounds off eta become dry ...
+    b[~still] = f(7.0)  # ... islands standing out of the lake
+    h[12:16, 20:26] = f(0.0)
+    b[12:16, 20:26] = f(8.0)
+    from swe_ppm_amd import scenarios
+
+    scenarios.apply_bc_bathymetry(b, [1, 1, 1, 1])
+    z = np.zeros_like(h)
+    hh, hu, hv, dts = oracle.run_u(h, z, z, b, 10.0, 10.0, [1, 1, 1, 1], 2, 50)
+    assert np.array_equal(hh[1:-1, 1:-1], h[1:-1, 1:-1])
+    assert np.all(hu[1:-1, 1:-1] == 0.0) and np.all(hv[1:-1, 1:-1] == 0.0)
+    assert np.all(dts > 0)
+    fw = oracle.run_u(h, z, z, b, 10.0, 10.0, [1, 1, 1, 1], 1, 50)
+    assert np.max(np.abs(fw[0][1:-1, 1:-1] - h[1:-1, 1:-1])) < 1e-5
