@@ -218,10 +218,11 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
  * 16 or 32; the sweeps of strategies 0 and 1 run 4 as 16); -1 keeps the
  * current value.  All strategies and widths give
  * identical results.  Until the caller sets xseg, a block uses the widest of
- * 16, 12 and 10 for which nyt * ceil(nx/xseg) / (CUs * 4) reaches the
- * solver's threshold (nyt = ceil(ny/62); AugRie 12, 4, 1; with a learned
- * middle state attached 5, 3, 2; HLLE, f-wave, Hybrid 2, 1, 0.5), else 8
- * (swe_hip_set_surrogate re-picks it). */
+ * 16 and 12 for which nyt * ceil(nx/xseg) / (CUs * 4) reaches the solver's
+ * threshold (nyt = ceil(ny/62); AugRie 12, 4; with a learned middle state
+ * attached 5, 3; HLLE, f-wave, Hybrid 2, 1), else 8 (swe_hip_set_surrogate
+ * re-picks it).  The fused loop skips quiet tiles (swe_hip_set_quiet_skip)
+ * at widths that are multiples of 4. */
 int swe_hip_set_tuning(swe_hip_block* blk, int strategy, int xseg);
 int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
 

@@ -82,15 +82,18 @@ size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pit
 // fewer halo columns per owned column and pays a wave's fixed start fewer
 // times, but gives fewer waves; narrow segments spread the work more evenly
 // over the SIMDs where its cost varies from edge to edge (AugRie's Newton
-// iteration).  The block takes the widest of 16, 12 and 10 whose busiest SIMD
+// iteration).  The block takes the widest of 16 and 12 whose busiest SIMD
 // still holds at least the solver's threshold of waves (waves per SIMD =
 // ceil(ny/62) * ceil(nx/xs) / (CUs x 4)), else 8:
-//                                      16     12     10
-//   AugRie (7 waves per SIMD)          12      4      1
-//   learned middle state (deferring)    5      3      2   lighter, even waves
-//   HLLE, f-wave, Hybrid                2      1    0.5   light edges
-// 32 stays available through swe_hip_set_tuning.  Measured, radial dam break
-// over bathymetry, us per step at 8/10/12/16/32 columns:
+//                                      16     12
+//   AugRie (7 waves per SIMD)          12      4
+//   learned middle state (deferring)    5      3   lighter, even waves
+//   HLLE, f-wave, Hybrid                2      1   light edges
+// Widths are whole numbers of the quiet record's 4-column tiles, so that
+// quiet tiles can be skipped (10 columns, the old third choice, measured
+// within 1% of 8 where it won).  4, 10 and 32 stay available through
+// swe_hip_set_tuning.  Measured without skipping, radial dam break over
+// bathymetry, us per step at 8/10/12/16/32 columns:
 //   AugRie 1024^2  73.5 / 72.7 / 80.2 / 99.9 / 176   (picks 10)
 //          2048^2   156 /  145 /  143 /  161 / 215   (12)
 //          3000^2   299 /  266 /  259 /  268 / 309   (12)
@@ -105,11 +108,12 @@ int default_xseg(int nx, int ny, int solver) {
         return 8;
     const double strips = (double)((ny + kStripRows - 1) / kStripRows);
     const double simds = 4.0 * cus;
-    static const double heavy[3] = {12.0, 4.0, 1.0}, learned[3] = {5.0, 3.0, 2.0}, light[3] = {2.0, 1.0, 0.5};
+    static const double heavy[3] = {12.0, 4.0, 0.0}, learned[3] = {5.0, 3.0, 0.0}, light[3] = {2.0, 1.0, 0.0};
     const double* t = (solver == SOLVER_AUGRIE || solver == SOLVER_AUGRIE_MLP) ? heavy
                       : solver == SOLVER_AUGRIE_DEFER                         ? learned
                                                                               : light;
-    const int widths[3] = {16, 12, 10};
+    // 10 columns is not a whole number of quiet-record tiles: 8 takes its place
+    const int widths[3] = {16, 12, 8};
     for (int i = 0; i < 3; i++)
         if (strips * ((nx + widths[i] - 1) / widths[i]) / simds >= t[i]) return widths[i];
     return 8;
@@ -234,7 +238,11 @@ struct LoopTiming {
 // Global-time-stepping loop of flavour U on the device: dt is reduced and
 // applied on the GPU, the next step's boundary conditions are fused into the
 // update (except after the last step, so ghosts end as in the reference).
-int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log, LoopTiming* tm) {
+// hostIO = false (the entry point that allocates nothing): no host transfer
+// of the simulated time and no synchronisation, so the loop only queues
+// kernels on the stream; dt goes to the device log k->dtlog.
+int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log, LoopTiming* tm,
+                      bool hostIO = true) {
     if (int r = ensure_dtlog(k, nsteps)) return r;
     if (k->strategy == 0 || k->strategy == 2)
         if (int r = ensure_second_buffer(k)) return r;
@@ -248,7 +256,7 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
         if (int r = ensure_defer_queue(k)) return r;
     const int deferGroups = defer ? defer_groups() : 0;
     float t0 = t_inout ? *t_inout : 0.f;
-    HIP_TRY(hipMemcpyAsync(&k->ctl->t, &t0, sizeof(float), hipMemcpyHostToDevice, k->stream));
+    if (hostIO) HIP_TRY(hipMemcpyAsync(&k->ctl->t, &t0, sizeof(float), hipMemcpyHostToDevice, k->stream));
     if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
     if (tm) HIP_TRY(tm->mark(k->stream, 0));
     if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
@@ -323,6 +331,7 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
         }
     }
     if (tm) HIP_TRY(tm->mark(k->stream, 0));
+    if (!hostIO) return SWE_HIP_OK;
     if (dt_log) HIP_TRY(hipMemcpyAsync(dt_log, k->dtlog, sizeof(float) * nsteps, hipMemcpyDeviceToHost, k->stream));
     float t1 = 0.f;
     HIP_TRY(hipMemcpyAsync(&t1, &k->ctl->t, sizeof(float), hipMemcpyDeviceToHost, k->stream));
