@@ -202,11 +202,12 @@ def test_autotune_restores_the_state_and_keeps_the_results(swe):
 
 
 def test_default_strip_width_follows_the_wave_count(swe):
-    """The widest of 16, 12, 10 columns per strip whose busiest SIMD still
-    holds the solver's threshold of waves (256 CUs x 4 SIMDs; AugRie 12, 4, 1
-    waves; learned 5, 3, 2; HLLE, f-wave, Hybrid 2, 1, 0.5), else 8: 4096^2
-    has 67 strips x 256 segments at 16 (16.75 per SIMD), 2048^2 34 x 171 at 12
-    (5.68), 1024^2 17 x 103 at 10 (1.71).  A width the caller set is kept."""
+    """The widest of 16 and 12 columns per strip whose busiest SIMD still
+    holds the solver's threshold of waves (256 CUs x 4 SIMDs; AugRie 12, 4
+    waves; learned 5, 3; HLLE, f-wave, Hybrid 2, 1), else 8 (widths are whole
+    4-column quiet-record tiles): 4096^2 has 67 strips x 256 segments at 16
+    (16.75 per SIMD), 2048^2 34 x 171 at 12 (5.68), 1024^2 17 x 86 at 12
+    (1.43, below 4).  A width the caller set is kept."""
     from swe_ppm_amd import surrogate
 
     with swe.SweBlock(2048, 2048, 1.0, 1.0, solver=swe.HLLE) as blk:
@@ -215,7 +216,7 @@ def test_default_strip_width_follows_the_wave_count(swe):
         assert blk.get_tuning()[1] == 12
     net = surrogate.load_default()
     for nx, ny, exact, learned in ((2048, 2048, 12, 12), (4096, 4096, 16, 16), (4096, 2048, 12, 16),
-                                   (2048, 4096, 12, 16), (1024, 1024, 10, 8), (64, 64, 8, 8)):
+                                   (2048, 4096, 12, 16), (1024, 1024, 8, 8), (64, 64, 8, 8)):
         with swe.SweBlock(nx, ny, 1.0, 1.0) as blk:
             assert blk.get_tuning()[1] == exact, (nx, ny)
             blk.set_surrogate(net)
