@@ -247,6 +247,34 @@ int swe_hip_autotune(swe_hip_block* blk, int steps, float* ms_per_step);
  * edge every step. */
 int swe_hip_set_quiet_skip(swe_hip_block* blk, int enable);
 
+/* nsteps of swe_hip_run (the fused device loop of flavour U with global time
+ * stepping, quiet tiles skipped) on caller-owned device arrays, allocating
+ * nothing.  No counterpart in the reference: this entry point is for callers
+ * that cannot allocate -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  It calls no hipMalloc, hipFree or other
+ * allocating routine and no host synchronisation; it only queues kernels on
+ * `stream` (a hipStream_t; NULL = the null stream).
+ *   h, hu, hv     the state in Float2D layout ((nx+2) x (ny+2), device);
+ *                 it holds the state after nsteps on return of the work
+ *   b             bathymetry (device), ghosts set by the caller
+ *                 (swe_hip_apply_bc_bathymetry's rule)
+ *   h2, hu2, hv2  a second state buffer of the same size (device), written
+ *   boundary      4 BoundaryType values (left, right, bottom, top): OUTFLOW,
+ *                 WALL, CONNECT, CONNECT_WITHIN_RANK or PASSIVE (the last
+ *                 three keep their ghost cells as given)
+ *   dt_log        device array of nsteps floats for the step sizes, or NULL
+ * The working memory (control block, net-update sums, quiet-tile record) is
+ * one static device array of SWE_HIP_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends -- a square block up to about 1100 x 1100 cells; a
+ * block that needs more is rejected with SWE_HIP_EINVAL.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and give wrong results.  Results are bit-identical to
+ * swe_hip_run on a block with the same state, boundaries and solver. */
+#define SWE_HIP_NOALLOC_SCRATCH_BYTES (40u << 20)
+int swe_hip_run_noalloc(int nx, int ny, float dx, float dy, int solver, const int* boundary, float* h, float* hu,
+                        float* hv, const float* b, float* h2, float* hu2, float* hv2, int nsteps, float* dt_log,
+                        void* stream);
+
 /* Arithmetic variant of the flavour-U kernels:
  *   0 = IEEE: no contraction, correctly rounded division and square root;
  *       results are bit-identical to an IEEE single-precision CPU evaluation

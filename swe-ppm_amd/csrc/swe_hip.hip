@@ -2,9 +2,11 @@
 // host<->device transfer and the step entry points (include/swe_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1106,6 +1108,157 @@ int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
     k->mlp = frags;
     if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags ? SOLVER_AUGRIE_DEFER : k->solver);
     return SWE_HIP_OK;
+}
+
+}  // extern "C"
+
+// ---- the entry point that allocates nothing ---------------------------------
+// Scratch of swe_hip_run_noalloc: the control block, both sets of net-update
+// sums, the quiet-tile record and the plan's lists.  One copy per GPU, held
+// until the process ends; calls that overlap on one GPU race on it.
+static __device__ __align__(256) unsigned char scratch[SWE_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 256;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// The device address of the current GPU's scratch array, looked up once per
+// GPU with that GPU current.  In host code the name `scratch` stands for no
+// device address: only this pointer may reach kernels and copies.
+int noalloc_scratch(unsigned char** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("swe_hip_run_noalloc: device %d out of range", dev);
+        return SWE_HIP_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Scratch bytes a block of nx x ny cells needs, and the offsets of its parts:
+// control block, sums A, sums B (4 fields each), quiet flags, tile speeds,
+// plan list, plan counts.
+size_t noalloc_layout(const swe_hip_block& k, size_t off[7]) {
+    const size_t field = align256(field_elems(&k) * sizeof(float));
+    const size_t tiles = max_tiles(&k);
+    const size_t groups = (tiles + kPlanGroup - 1) / kPlanGroup;
+    size_t at = 0;
+    off[0] = at;
+    at += align256(sizeof(DeviceCtl));
+    off[1] = at;
+    at += 4 * field;
+    off[2] = at;
+    at += 4 * field;
+    off[3] = at;
+    at += align256(2 * tiles * sizeof(unsigned));
+    off[4] = at;
+    at += align256(2 * tiles * sizeof(float));
+    off[5] = at;
+    at += align256(groups * kPlanGroup * kPlanPieces * sizeof(unsigned));
+    off[6] = at;
+    at += align256(groups * sizeof(unsigned));
+    return at;
+}
+
+__global__ void copy_state_kernel(const float* __restrict__ h, const float* __restrict__ hu,
+                                  const float* __restrict__ hv, float* __restrict__ oh, float* __restrict__ ohu,
+                                  float* __restrict__ ohv, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        oh[i] = h[i];
+        ohu[i] = hu[i];
+        ohv[i] = hv[i];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_hip_run_noalloc(int nx, int ny, float dx, float dy, int solver, const int* boundary, float* h, float* hu,
+                        float* hv, const float* b, float* h2, float* hu2, float* hv2, int nsteps, float* dt_log,
+                        void* stream) {
+    if (nx < 1 || ny < 1 || !(dx > 0.f) || !(dy > 0.f) || solver < 0 || solver > 3 || !boundary || !h || !hu ||
+        !hv || !b || !h2 || !hu2 || !hv2 || nsteps < 0) {
+        set_error("swe_hip_run_noalloc: invalid argument (nx=%d ny=%d dx=%g dy=%g solver=%d nsteps=%d)", nx, ny, dx,
+                  dy, solver, nsteps);
+        return SWE_HIP_EINVAL;
+    }
+    for (int i = 0; i < 4; ++i)
+        if (boundary[i] < OUTFLOW || boundary[i] > PASSIVE || boundary[i] == INFLOW) {
+            set_error("swe_hip_run_noalloc: boundary type %d of side %d is not supported", boundary[i], i);
+            return SWE_HIP_EINVAL;
+        }
+    if ((size_t)(nx + 2) * (size_t)(ny + 2) > ((size_t)1 << 31)) {
+        set_error("swe_hip_run_noalloc: block too large");
+        return SWE_HIP_EINVAL;
+    }
+    if (nsteps == 0) return SWE_HIP_OK;
+    // a block description on the stack whose device parts all point into the
+    // caller's arrays and the scratch array, so the loop allocates nothing
+    swe_hip_block k;
+    k.nx = nx;
+    k.ny = ny;
+    k.pitch = ny + 2;
+    k.dx = dx;
+    k.dy = dy;
+    k.solver = solver;
+    k.flavour = FLAVOUR_U;
+    k.strategy = 2;
+    size_t off[7];
+    const size_t need = noalloc_layout(k, off);
+    if (need > (size_t)SWE_HIP_NOALLOC_SCRATCH_BYTES) {
+        set_error("swe_hip_run_noalloc: a %d x %d block needs %zu bytes of scratch, more than the %zu this entry "
+                  "point holds (SWE_HIP_NOALLOC_SCRATCH_BYTES); use swe_hip_run",
+                  nx, ny, need, (size_t)SWE_HIP_NOALLOC_SCRATCH_BYTES);
+        return SWE_HIP_EINVAL;
+    }
+    k.xseg = default_xseg(nx, ny, solver);
+    unsigned char* base = nullptr;
+    if (int r = noalloc_scratch(&base)) return r;
+    const size_t field = align256(field_elems(&k) * sizeof(float));
+    for (int i = 0; i < 4; ++i) {
+        k.sums[i] = reinterpret_cast<float*>(base + off[1] + i * field);
+        k.sums2[i] = reinterpret_cast<float*>(base + off[2] + i * field);
+    }
+    k.ctl = reinterpret_cast<DeviceCtl*>(base + off[0]);
+    k.quiet = reinterpret_cast<unsigned*>(base + off[3]);
+    k.tspeed = reinterpret_cast<float*>(base + off[4]);
+    k.tlist = reinterpret_cast<unsigned*>(base + off[5]);
+    k.tcount = reinterpret_cast<unsigned*>(base + off[6]);
+    k.h[0] = h;
+    k.hu[0] = hu;
+    k.hv[0] = hv;
+    k.h[1] = h2;
+    k.hu[1] = hu2;
+    k.hv[1] = hv2;
+    k.b = const_cast<float*>(b);
+    k.cur = 0;
+    k.dtlog = dt_log;
+    k.dtlogCap = 1 << 30;
+    k.stream = static_cast<hipStream_t>(stream);
+    for (int i = 0; i < 4; ++i) k.bi.type[i] = boundary[i];
+    int rc = run_u_device_loop(&k, nsteps, nullptr, nullptr, nullptr, false);
+    if (rc == 0 && k.cur != 0) {  // the state ended in the second buffer: bring it home
+        const size_t n = field_elems(&k);
+        hipLaunchKernelGGL(copy_state_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                           k.stream, h2, hu2, hv2, h, hu, hv, n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = hip_fail(e, "copy_state_kernel launch");
+    }
+    // k is a view: nothing of it is freed (its destructor-free struct owns nothing)
+    return rc;
 }
 
 }  // extern "C"
