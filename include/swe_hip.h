@@ -237,11 +237,13 @@ int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
 #define SWE_HIP_TUNE_WIDTHS 6
 int swe_hip_autotune(swe_hip_block* blk, int steps, float* ms_per_step);
 
-/* Quiet-tile skipping of the fused device loop (strategy 2), on by default.
- * A strip tile whose net-update sums were all +0 on the last two steps, with
- * its four neighbours quiet on the last step, would compute the same zero
- * sums and the same CFL maximum again from an unchanged state; the loop
- * skips it and carries its maximum forward.  Results are bit-identical with
+/* Quiet-tile skipping of the fused device loop (strategy 2), on by default,
+ * for the augmented solver (AugRie, with or without a learned middle state)
+ * at strip widths that are multiples of 4.  A tile of 62 rows x 4 columns
+ * whose net-update sums were all +0 on the last two steps, with the adjacent
+ * rows and columns of its four neighbours quiet on the last step, would
+ * compute the same zero sums and the same CFL maximum again from an
+ * unchanged state; the loop skips it and carries its maximum forward.  Results are bit-identical with
  * and without skipping (enable = 0 computes every tile, e.g. for timing the
  * full sweep).  No reference counterpart: the reference's loops visit every
  * edge every step. */

@@ -46,6 +46,10 @@ typedef struct swe_hip_mlp swe_hip_mlp;
 /* A network with seeded He-initialised weights and an empty device sample
  * store of `capacity` samples. */
 int swe_hip_mlp_create(int64_t capacity, uint64_t seed, swe_hip_mlp** out);
+/* Frees the network; refused with SWE_HIP_ESTATE while a block or batch has
+ * it attached (the blocks read its weights on the device).  Setting new
+ * weights (swe_hip_mlp_set_params, swe_hip_mlp_train) first waits for the
+ * work queued on the attached blocks' and batches' streams. */
 int swe_hip_mlp_destroy(swe_hip_mlp* m);
 
 /* FP32 master parameters (SWE_MLP_PARAMS floats, layout above). */

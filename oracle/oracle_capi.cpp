@@ -142,9 +142,90 @@ int multiblock_run(int nblocks, const int* nx, const int* ny, T dx, T dy, const 
     return rc;
 }
 
+// Driver iterations of the overdecomposed LTS loop one at a time (the inner
+// do-loop of swe_mpi_overdecomp.cpp:266-336, with resetStepSizeCounter when
+// every block has its macro step), for tests that look inside a macro step.
+// info receives per block, after the last iteration, kTraceInfo floats:
+// total, stepSize, stepSizeCounter, timestepCounter, notified, gl[4],
+// borderTs[4], updates.
+constexpr int kTraceInfo = 14;
+
+template <typename T>
+int lts_trace(int nblocks, const int* nx, const int* ny, T dx, T dy, const int* bnd, const int* nbr, int solver,
+              T maxTimestepLocal, T duration, int niter, T* h, T* hu, T* hv, T* b, T* info, int32_t* macroEnds) {
+    MultiBlock<T> mb;
+    mb.solver = solver;
+    mb.blocks.resize(nblocks);
+    size_t off = 0;
+    std::vector<size_t> offs(nblocks);
+    for (int k = 0; k < nblocks; k++) {
+        LtsBlock<T>& bl = mb.blocks[k];
+        bl.nx = nx[k];
+        bl.ny = ny[k];
+        bl.dx = dx;
+        bl.dy = dy;
+        const size_t n = (size_t)(nx[k] + 2) * (ny[k] + 2);
+        offs[k] = off;
+        bl.h.assign(h + off, h + off + n);
+        bl.hu.assign(hu + off, hu + off + n);
+        bl.hv.assign(hv + off, hv + off + n);
+        bl.b.assign(b + off, b + off + n);
+        bl.lts = true;
+        bl.bufH.assign(n, T(0));
+        bl.bufHu.assign(n, T(0));
+        bl.bufHv.assign(n, T(0));
+        for (int s = 0; s < 4; s++) {
+            bl.bnd[s] = bnd[4 * k + s];
+            bl.nb[s] = nbr[4 * k + s];
+        }
+        bl.maxTimestepLocal = maxTimestepLocal;
+        bl.duration = duration;
+        off += n;
+    }
+    mb.prepare();
+    mb.sendBathymetry();
+    int macro = 0;
+    for (int it = 0; it < niter; it++) {
+        mb.iterate(true);
+        bool synced = true;
+        for (auto& bl : mb.blocks)
+            if (!bl.hasMaxLocal()) synced = false;
+        if (synced) {
+            for (auto& bl : mb.blocks) bl.resetStepSizeCounter();
+            if (macroEnds) macroEnds[macro] = it + 1;
+            macro++;
+        }
+    }
+    for (int k = 0; k < nblocks; k++) {
+        LtsBlock<T>& bl = mb.blocks[k];
+        std::copy(bl.h.begin(), bl.h.end(), h + offs[k]);
+        std::copy(bl.hu.begin(), bl.hu.end(), hu + offs[k]);
+        std::copy(bl.hv.begin(), bl.hv.end(), hv + offs[k]);
+        T* o = info + (size_t)k * kTraceInfo;
+        o[0] = bl.total();
+        o[1] = (T)bl.stepSize;
+        o[2] = (T)bl.stepSizeCounter;
+        o[3] = (T)bl.timestepCounter;
+        o[4] = bl.notified ? T(1) : T(0);
+        for (int s = 0; s < 4; s++) {
+            o[5 + s] = (T)bl.gl[s];
+            o[9 + s] = bl.borderTs[s];
+        }
+        o[13] = (T)bl.updates;
+    }
+    return macro;
+}
+
 }  // namespace
 
 extern "C" {
+
+int oracle_lts_trace_f32(int nblocks, const int* nx, const int* ny, float dx, float dy, const int* bnd,
+                         const int* nbr, int solver, float maxTimestepLocal, float duration, int niter, float* h,
+                         float* hu, float* hv, float* b, float* info, int32_t* macroEnds) {
+    return lts_trace<float>(nblocks, nx, ny, dx, dy, bnd, nbr, solver, maxTimestepLocal, duration, niter, h, hu, hv,
+                            b, info, macroEnds);
+}
 
 int oracle_solver_f32(int solver, int64_t n, const float* hL, const float* hR, const float* huL,
                       const float* huR, const float* bL, const float* bR, float* hNetL, float* hNetR,

@@ -21,10 +21,10 @@ def build() -> None:
     subprocess.run(["make", "-C", str(HERE), "-j4"], check=True, capture_output=True)
 
 
-def _load() -> C.CDLL:
-    if not LIB_PATH.exists():
+def _load(path: Path = LIB_PATH) -> C.CDLL:
+    if not path.exists():
         build()
-    lib = C.CDLL(str(LIB_PATH))
+    lib = C.CDLL(str(path))
     i64 = C.c_int64
     for t, ct in (("f32", C.c_float), ("f64", C.c_double)):
         p = C.POINTER(ct)
@@ -48,10 +48,21 @@ def _load() -> C.CDLL:
                                               C.c_float, C.c_float, C.c_int, C.c_int, fp, fp, fp, fp,
                                               C.POINTER(C.c_int64), C.POINTER(C.c_int32), fp]
     lib.oracle_round_timestep_f32.argtypes = [C.c_float, C.c_float, fp]
+    lib.oracle_lts_trace_f32.argtypes = [C.c_int, ip, ip, C.c_float, C.c_float, ip, ip, C.c_int, C.c_float,
+                                         C.c_float, C.c_int, fp, fp, fp, fp, fp, C.POINTER(C.c_int32)]
     return lib
 
 
 lib = _load()
+_canonical = None
+
+
+def canonical_lib() -> C.CDLL:
+    """The oracle built with the textbook per-cell forms (hu / h, sqrt(g h))."""
+    global _canonical
+    if _canonical is None:
+        _canonical = _load(HERE / "build" / "liboracle_canonical.so")
+    return _canonical
 
 
 def _p(a: np.ndarray):
@@ -99,14 +110,16 @@ def apply_bc_bathymetry(bnd, b) -> None:
     lib.oracle_apply_bc_bathymetry_f32(nx, ny, bp, _p(b))
 
 
-def run_u(h, hu, hv, b, dx, dy, bnd, solver, nsteps, nthreads=1, dt_in=None, dtype=np.float32):
-    """nsteps of flavour U (in place on copies); returns (h, hu, hv, dts)."""
+def run_u(h, hu, hv, b, dx, dy, bnd, solver, nsteps, nthreads=1, dt_in=None, dtype=np.float32, canonical=False):
+    """nsteps of flavour U (in place on copies); returns (h, hu, hv, dts).
+    canonical: the restatement with the textbook per-cell forms."""
     h, hu, hv, b = [np.array(a, dtype=dtype, order="C", copy=True) for a in (h, hu, hv, b)]
     nx, ny = h.shape[0] - 2, h.shape[1] - 2
     dts = np.empty(max(nsteps, 1), dtype)
     dti = None if dt_in is None else np.ascontiguousarray(dt_in, dtype=dtype)
     bb, bp = _ints(bnd)
-    f = lib.oracle_run_u_f32 if dtype == np.float32 else lib.oracle_run_u_f64
+    L = canonical_lib() if canonical else lib
+    f = L.oracle_run_u_f32 if dtype == np.float32 else L.oracle_run_u_f64
     rc = f(nx, ny, dx, dy, bp, solver, nsteps, nthreads, _p(h), _p(hu), _p(hv), _p(b), _p(dti), _p(dts))
     assert rc == 0
     return h, hu, hv, dts[:nsteps]
@@ -174,3 +187,36 @@ def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nth
         out.append(tuple(a[off:off + sz].reshape(shp).copy() for a in cat))
         off += sz
     return out, upd, its[:nmacro], dts[:nmacro]
+
+
+TRACE_FIELDS = ("total", "stepSize", "stepSizeCounter", "timestepCounter", "notified", "gl", "borderTs", "updates")
+
+
+def lts_trace(blocks, dx, dy, solver, max_local, duration, niter):
+    """niter driver iterations of the overdecomposed LTS loop (macro steps end
+    when every block has its maximum local time step).  Returns (states,
+    info, macro_ends): per block (h, hu, hv) incl. ghosts, a dict of the LTS
+    bookkeeping after the last iteration, and the iteration count at which
+    each completed macro step ended."""
+    n = len(blocks)
+    nx, nxp = _ints([bl["nx"] for bl in blocks])
+    ny, nyp = _ints([bl["ny"] for bl in blocks])
+    bnd, bndp = _ints(np.concatenate([bl["bnd"] for bl in blocks]))
+    nbr, nbrp = _ints(np.concatenate([bl["nbr"] for bl in blocks]))
+    cat = [np.concatenate([np.asarray(bl[k], np.float32).ravel() for bl in blocks]) for k in ("h", "hu", "hv", "b")]
+    info = np.zeros((n, 14), np.float32)
+    ends = np.zeros(max(niter, 1), np.int32)
+    macros = lib.oracle_lts_trace_f32(n, nxp, nyp, dx, dy, bndp, nbrp, solver, max_local, duration, niter,
+                                      *[_p(a) for a in cat], _p(info), ends.ctypes.data_as(C.POINTER(C.c_int32)))
+    out, off = [], 0
+    for bl in blocks:
+        sz = (bl["nx"] + 2) * (bl["ny"] + 2)
+        shp = (bl["nx"] + 2, bl["ny"] + 2)
+        out.append(tuple(a[off:off + sz].reshape(shp).copy() for a in cat[:3]))
+        off += sz
+    rows = []
+    for r in info:
+        rows.append(dict(total=float(r[0]), stepSize=int(r[1]), stepSizeCounter=int(r[2]),
+                         timestepCounter=int(r[3]), notified=bool(r[4]), gl=[int(v) for v in r[5:9]],
+                         borderTs=[float(v) for v in r[9:13]], updates=int(r[13])))
+    return out, rows, [int(e) for e in ends[:macros]]

@@ -64,6 +64,23 @@ struct SolverConsts {
     static constexpr T stillRatio = T(2);
 };
 
+// A cell's velocity and celerity.  The kernels compute them once per loaded
+// cell as hu * (1/h) and sqrt(g) * sqrt(h) (the reciprocal and the root are
+// shared by the cell's edges); the oracle follows that rounding.  Built with
+// ORACLE_CANONICAL=1 it uses the textbook forms hu / h and sqrt(g h) instead,
+// so that a test can bound what the kernels' rounding choice changes.
+#ifndef ORACLE_CANONICAL
+#define ORACLE_CANONICAL 0
+#endif
+template <typename T>
+inline T cell_velocity(T hu, T h) {
+    return ORACLE_CANONICAL ? hu / h : hu * (T(1) / h);
+}
+template <typename T>
+inline T cell_celerity(T h) {
+    return ORACLE_CANONICAL ? std::sqrt(SolverConsts<T>::g * h) : SolverConsts<T>::sqrtG * std::sqrt(h);
+}
+
 // ---------------------------------------------------------------------------
 // Still edge of the augmented solver: both cells wet with at least stillTol
 // of water (cells shallower than 0.1 m have their momentum zeroed by the dry
@@ -100,8 +117,8 @@ inline bool augrie_still(T hL, T hR, T huL, T huR, T bL, T bR) {
 template <typename T>
 inline T augrie_still_speed(T hL, T hR) {
     typedef SolverConsts<T> C;
-    const T cL = C::sqrtG * std::sqrt(hL);
-    const T cR = C::sqrtG * std::sqrt(hR);
+    const T cL = cell_celerity(hL);
+    const T cR = cell_celerity(hR);
     const T cHat = std::sqrt(C::g * T(0.5) * (hR + hL));
     const T sE1 = std::min(-cL, -cHat);
     const T sE2 = std::max(cR, cHat);
@@ -150,12 +167,12 @@ void fwave(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, 
         bR = bL;
     }
     // a cell's velocity and celerity from 1/h and sqrt(h)
-    const T uL = huL * (T(1) / hL);
-    const T uR = huR * (T(1) / hR);
+    const T uL = cell_velocity(huL, hL);
+    const T uR = cell_velocity(huR, hR);
     const T sqhL = std::sqrt(hL);
     const T sqhR = std::sqrt(hR);
-    const T cL = C::sqrtG * sqhL;
-    const T cR = C::sqrtG * sqhR;
+    const T cL = cell_celerity(hL);
+    const T cR = cell_celerity(hR);
     // Roe averages
     const T hRoe = T(0.5) * (hL + hR);
     const T uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
@@ -212,8 +229,8 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
     const T hMin = std::min(hL, hR);
     const T hMax = std::max(hL, hR);
     const T du = uR - uL;
-    const T cL = C::sqrtG * std::sqrt(hL);
-    const T cR = C::sqrtG * std::sqrt(hR);
+    const T cL = cell_celerity(hL);
+    const T cR = cell_celerity(hR);
     int it = 0;
     int kind;
     if (hMin < C::dryTol) {
@@ -222,8 +239,8 @@ int middle_state(T hL, T hR, T uL, T uR, T& hm, T& s1m, T& s2m, int* iters = nul
         s2m = s1m;
         kind = 0;
     } else {
-        const T cMin = C::sqrtG * std::sqrt(hMin);
-        const T cMax = C::sqrtG * std::sqrt(hMax);
+        const T cMin = cell_celerity(hMin);
+        const T cMax = cell_celerity(hMax);
         const T invMin = T(1) / hMin;
         const T invMax = T(1) / hMax;
         const T halfG = T(0.5) * C::g;
@@ -322,14 +339,14 @@ void augrie_full(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& hu
         huL = T(0);
         uL = T(0);
     } else {
-        uL = huL * (T(1) / hL);
+        uL = cell_velocity(huL, hL);
     }
     if (hR < C::dryTol) {
         hR = T(0);
         huR = T(0);
         uR = T(0);
     } else {
-        uR = huR * (T(1) / hR);
+        uR = cell_velocity(huR, hR);
     }
 
     // wet/dry interface: wall (the dry cell's bathymetry is above anything the
@@ -364,8 +381,8 @@ void augrie_full(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& hu
     }
 
     // Einfeldt speeds from the Roe average weighted by the celerities
-    const T cL = C::sqrtG * std::sqrt(hL);
-    const T cR = C::sqrtG * std::sqrt(hR);
+    const T cL = cell_celerity(hL);
+    const T cR = cell_celerity(hR);
     const T uHat = (cL * uL + cR * uR) / (cL + cR);
     const T cHat = std::sqrt(C::g * T(0.5) * (hR + hL));
     const T sE1 = std::min(uL - cL, uHat - cHat);
@@ -495,12 +512,12 @@ void hlle(T hL, T hR, T huL, T huR, T bL, T bR, T& hNetL, T& hNetR, T& huNetL, T
         huR = -huL;
         bR = bL;
     }
-    const T uL = huL * (T(1) / hL);
-    const T uR = huR * (T(1) / hR);
+    const T uL = cell_velocity(huL, hL);
+    const T uR = cell_velocity(huR, hR);
     const T sqhL = std::sqrt(hL);
     const T sqhR = std::sqrt(hR);
-    const T cL = C::sqrtG * sqhL;
-    const T cR = C::sqrtG * sqhR;
+    const T cL = cell_celerity(hL);
+    const T cR = cell_celerity(hR);
     const T uRoe = (uL * sqhL + uR * sqhR) / (sqhL + sqhR);
     const T cRoe = std::sqrt(C::g * T(0.5) * (hL + hR));
     // HLLE speeds bounded by zero so that each side gets one wave

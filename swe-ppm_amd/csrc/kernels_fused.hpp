@@ -69,6 +69,25 @@ constexpr int kQuietCols = 4;
 // lengthens the pieces it lists.
 constexpr unsigned kPlanWaves = 2048;
 
+// Lanes the quiet-tile plan gives a wave segment of `fine` record tiles: the
+// smallest power of two >= fine, so that a segment's lanes share one wave.
+__host__ __device__ constexpr int plan_lanes(int fine) { return fine <= 1 ? 1 : fine <= 2 ? 2 : fine <= 4 ? 4 : 8; }
+// Workgroups of the plan (and entries of its count array) for nyt x nxs
+// segments of `fine` record tiles.
+__host__ __device__ constexpr int plan_groups(int nyt, int nxs, int fine) {
+    return (nyt * nxs * plan_lanes(fine) + kPlanGroup - 1) / kPlanGroup;
+}
+
+// Which fused kernels keep the quiet record: the augmented solver's, exact
+// and deferring, whose still edges give +0 sums (HLLE and f-wave leave the
+// round-off of a lake at rest, so their tiles are never quiet; the in-sweep
+// network variant is at its three-wave register limit and the device loop
+// runs the deferring variant instead), at widths that are whole record tiles.
+template <int S>
+constexpr bool quiet_record(int xs) {
+    return (S == SOLVER_AUGRIE || S == SOLVER_AUGRIE_DEFER) && xs % kQuietCols == 0;
+}
+
 struct FusedIO {
     const float* h;   // Q(k-1)
     const float* hu;
@@ -85,8 +104,8 @@ struct FusedIO {
     unsigned* quiet = nullptr;
     float* speed = nullptr;
     int step = 0;  // launch index within the run (1, 2, ...)
-    // list mode: group g of kPlanGroup tiles lists count[g] tiles to compute
-    // at list[g * kPlanGroup ...]
+    // list mode: workgroup g of the plan lists count[g] runs of record tiles
+    // to compute at list[g * kPlanGroup * kPlanPieces ...]
     const unsigned* list = nullptr;
     const unsigned* count = nullptr;
 };
@@ -142,14 +161,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
         ctl->steps += 1;
         if (dtLog) dtLog[logIdx] = dt;
     }
-    // the in-sweep network variant keeps no record (its registers are at the
-    // three-wave limit; the device loop runs the deferring variant instead),
-    // nor do widths that are not a whole number of record tiles
-    constexpr bool kRecord = S != SOLVER_AUGRIE_MLP && XS % kQuietCols == 0;
+    constexpr bool kRecord = quiet_record<S>(XS);
     const int items = nyt * nxs;
     int item = wave, first = 0, ncols = XS;
     if (kRecord && io.list) {
-        const int e = listed_item(io, (items + kPlanGroup - 1) / kPlanGroup, wave);
+        const int e = listed_item(io, plan_groups(nyt, nxs, XS / kQuietCols), wave);
         if (e < 0) return;  // uniform per wave
         item = e >> 6;
         first = (e >> 3) & 7;

@@ -74,85 +74,86 @@ __global__ void apply_bc_bathymetry_kernel(float* b, int nx, int ny, int pitch, 
 }
 
 // Skip decisions for fused launch `step` (>= 3) over the wave segments of
-// XS = fine * kQuietCols columns, one thread per segment in groups of
-// kPlanGroup consecutive segments.  A skipped record tile carries its speed
-// into slot step & 1 and into the CFL maximum of slot slotOut.  The record
-// tiles of a segment from its first to its last busy one are listed in
-// pieces of at most `piece` tiles, (segment << 6) | (first << 3) |
-// (length - 1); group g keeps its entries in segment order at
-// list[g * kPlanGroup * kPlanPieces ...] and their number in count[g].
-// piece adapts to the load: busy tiles found by the previous launch's plan
-// over kPlanWaves, between 1 and `fine` (with few busy tiles short pieces
-// spread them over more waves, the busiest wave sets the launch time; with
-// many, long pieces solve fewer halo edges).  The number of busy tiles goes
-// to ctl->busy[slotOut].
+// XS = fine * kQuietCols columns.  One lane per record tile: the `lanes`
+// lanes of a segment (plan_lanes(fine)) sit side by side in a wave,
+// and a ballot gives each segment its busy tiles.  A skipped record tile
+// carries its speed into slot step & 1 and into the CFL maximum of slot
+// slotOut.  The record tiles of a segment from its first to its last busy
+// one are listed in pieces of at most `piece` tiles,
+// (segment << 6) | (first << 3) | (length - 1); workgroup g keeps its
+// entries in segment order at list[g * kPlanGroup * kPlanPieces ...] and
+// their number in count[g].  piece adapts to the load: busy tiles found by
+// the previous launch's plan over kPlanWaves, between 1 and `fine` (with few
+// busy tiles short pieces spread them over more waves, the busiest wave sets
+// the launch time; with many, long pieces solve fewer halo edges).  The
+// number of busy tiles goes to ctl->busy[slotOut].
 __global__ __launch_bounds__(kPlanGroup) void quiet_plan_kernel(const unsigned* __restrict__ quiet, float* speed,
-                                                                int nyt, int nxs, int nxf, int fine, int step,
-                                                                DeviceCtl* ctl, int slotPrev3, int slotOut,
+                                                                int nyt, int nxs, int nxf, int fine, int lanes,
+                                                                int step, DeviceCtl* ctl, int slotPrev3, int slotOut,
                                                                 unsigned* list, unsigned* count) {
     __shared__ unsigned wavePieces[kPlanGroup / 64];
     __shared__ unsigned waveBusy[kPlanGroup / 64];
     __shared__ unsigned waveMax[kPlanGroup / 64];
     const int tiles = nyt * nxf;
-    const int seg = blockIdx.x * kPlanGroup + threadIdx.x;
+    const int g = blockIdx.x * kPlanGroup + threadIdx.x;
+    const int seg = g / lanes, i = g % lanes;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int slotNow = (step & 1) * tiles, slotPrev = ((step - 1) & 1) * tiles;
     const unsigned prevBusy = ctl->busy[slotPrev3];
     const int piece = prevBusy == 0u ? fine : (int)min((unsigned)fine, max(1u, (prevBusy + kPlanWaves - 1) / kPlanWaves));
-    int firstBusy = -1, lastBusy = -1;
-    unsigned nBusy = 0;
+    const int yt = seg % nyt, xs = seg / nyt;
+    const int f = xs * fine + i;
+    bool busy = false;
     float sp = 0.f;
-    if (seg < nyt * nxs) {
-        const int yt = seg % nyt, xs = seg / nyt;
+    if (seg < nyt * nxs && i < fine && f < nxf) {
+        const int t = yt + nyt * f;
         const unsigned* q = quiet;
-        for (int i = 0; i < fine; ++i) {
-            const int f = xs * fine + i;
-            if (f >= nxf) break;
-            const int t = yt + nyt * f;
-            const bool skip = (q[slotPrev + t] & kQuietAll) && (q[slotNow + t] & kQuietAll) &&
-                              (f == 0 || (q[slotPrev + t - nyt] & kQuietRight)) &&
-                              (f == nxf - 1 || (q[slotPrev + t + nyt] & kQuietLeft)) &&
-                              (yt == 0 || (q[slotPrev + t - 1] & kQuietTop)) &&
-                              (yt == nyt - 1 || (q[slotPrev + t + 1] & kQuietBottom));
-            if (skip) {
-                const float s = speed[slotPrev + t];
-                speed[slotNow + t] = s;
-                sp = fmaxf(sp, s);
-            } else {
-                if (firstBusy < 0) firstBusy = i;
-                lastBusy = i;
-                ++nBusy;
-            }
+        // every load of the tile at once: the flags and the speed to carry
+        const unsigned self = q[slotPrev + t], selfOld = q[slotNow + t];
+        const unsigned left = f == 0 ? kQuietRight : q[slotPrev + t - nyt];
+        const unsigned right = f == nxf - 1 ? kQuietLeft : q[slotPrev + t + nyt];
+        const unsigned below = yt == 0 ? kQuietTop : q[slotPrev + t - 1];
+        const unsigned above = yt == nyt - 1 ? kQuietBottom : q[slotPrev + t + 1];
+        const float spPrev = speed[slotPrev + t];
+        const bool skip = (self & kQuietAll) && (selfOld & kQuietAll) && (left & kQuietRight) &&
+                          (right & kQuietLeft) && (below & kQuietTop) && (above & kQuietBottom);
+        if (skip) {
+            sp = spPrev;
+            speed[slotNow + t] = sp;
+        } else {
+            busy = true;
         }
     }
-    const int runLen = firstBusy >= 0 ? lastBusy - firstBusy + 1 : 0;
+    // the segment's busy tiles, from the ballot bits of its lanes
+    const unsigned long long m = __ballot(busy);
+    const unsigned segBits = (unsigned)(m >> (lane & ~(lanes - 1))) & ((1u << lanes) - 1u);
+    const bool leader = i == 0 && segBits != 0u;
+    const int firstBusy = leader ? __builtin_ctz(segBits) : 0;
+    const int runLen = leader ? 32 - __builtin_clz(segBits) - firstBusy : 0;
     const unsigned pieces = (unsigned)((runLen + piece - 1) / piece);
-    // exclusive prefix of the pieces over the wave, then over the group
+    // exclusive prefix of the pieces over the wave, then over the workgroup
     unsigned incl = pieces;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const unsigned o = __shfl_up(incl, off, 64);
         incl += (lane >= off) ? o : 0u;
     }
-    unsigned busyW = nBusy;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) busyW += __shfl_xor(busyW, off, 64);
     sp = wave_max(sp);
     if (lane == 63) wavePieces[w] = incl;
     if (lane == 0) {
-        waveBusy[w] = busyW;
+        waveBusy[w] = (unsigned)__popcll(m);
         waveMax[w] = __float_as_uint(sp);
     }
     __syncthreads();
-    unsigned before = 0, total = 0, busy = 0, mx = 0;
-    for (int i = 0; i < kPlanGroup / 64; ++i) {
-        const unsigned c = wavePieces[i];
-        before += (i < w) ? c : 0u;
+    unsigned before = 0, total = 0, nbusy = 0, mx = 0;
+    for (int k = 0; k < kPlanGroup / 64; ++k) {
+        const unsigned c = wavePieces[k];
+        before += (k < w) ? c : 0u;
         total += c;
-        busy += waveBusy[i];
-        mx = max(mx, waveMax[i]);
+        nbusy += waveBusy[k];
+        mx = max(mx, waveMax[k]);
     }
-    unsigned pos = (unsigned)blockIdx.x * kPlanGroup * kPlanPieces + before + incl - pieces;
+    const unsigned pos = (unsigned)blockIdx.x * kPlanGroup * kPlanPieces + before + incl - pieces;
     for (unsigned p = 0; p < pieces; ++p) {
         const int f0 = firstBusy + (int)p * piece;
         const int len = min(piece, runLen - (int)p * piece);
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(kPlanGroup) void quiet_plan_kernel(const unsigned* 
     }
     if (threadIdx.x == 0) {
         count[blockIdx.x] = total;
-        if (busy) atomicAdd(&ctl->busy[slotOut], busy);
+        if (nbusy) atomicAdd(&ctl->busy[slotOut], nbusy);
         if (mx > 0u) atomicMax(&ctl->smax[slotOut], mx);
     }
 }
@@ -323,9 +324,9 @@ int launch_quiet_plan(const unsigned* quiet, float* speed, int nx, int ny, int x
                       int slotOut, unsigned* list, unsigned* count, hipStream_t stream) {
     const int nyt = (ny + kStripRows - 1) / kStripRows, nxs = (nx + xseg - 1) / xseg;
     const int nxf = (nx + kQuietCols - 1) / kQuietCols;
-    const int segs = nyt * nxs;
-    hipLaunchKernelGGL(quiet_plan_kernel, dim3((segs + kPlanGroup - 1) / kPlanGroup), dim3(kPlanGroup), 0, stream,
-                       quiet, speed, nyt, nxs, nxf, xseg / kQuietCols, step, ctl, (step + 2) % 3, slotOut, list,
+    const int fine = xseg / kQuietCols;
+    hipLaunchKernelGGL(quiet_plan_kernel, dim3((unsigned)plan_groups(nyt, nxs, fine)), dim3(kPlanGroup), 0, stream,
+                       quiet, speed, nyt, nxs, nxf, fine, plan_lanes(fine), step, ctl, (step + 2) % 3, slotOut, list,
                        count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "quiet_plan_kernel launch");

@@ -614,6 +614,7 @@ int launch_sweep(int solver, int xs, int groups, BatchArena A, const BatchBlockD
 }  // namespace
 
 struct swe_hip_batch {
+    swe_hip_mlp* net = nullptr;  // the attached network's handle
     int nblocks = 0, solver = SOLVER_AUGRIE, mode = BATCH_ENSEMBLE;
     float T = 1.f, duration = 0.f;
     std::vector<BatchBlockDesc> desc;
@@ -981,6 +982,7 @@ int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float*
 int swe_hip_batch_destroy(swe_hip_batch* k) {
     if (!k) return SWE_HIP_OK;
     if (k->stream) (void)hipStreamSynchronize(k->stream);
+    if (k->net) swe::mlp_detach(k->net, k->stream);
     free_graph(k);
     float* fields[16] = {k->A.h[0],    k->A.hu[0],   k->A.hv[0],   k->A.h[1],    k->A.hu[1],   k->A.hv[1],
                          k->A.b,       k->A.s[0][0], k->A.s[0][1], k->A.s[0][2], k->A.s[0][3], k->A.s[1][0],
@@ -1141,6 +1143,11 @@ int swe_hip_batch_set_surrogate(swe_hip_batch* k, swe_hip_mlp* m) {
     const unsigned* frags = m ? swe::mlp_frags(m) : nullptr;
     HIP_TRY(hipStreamSynchronize(k->stream));
     HIP_TRY(hipMemcpy(&k->dG->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
+    if (k->net != m) {
+        if (k->net) swe::mlp_detach(k->net, k->stream);
+        if (m) swe::mlp_attach(m, k->stream);
+        k->net = m;
+    }
     k->mlp = frags;
     free_graph(k);  // the captured sweep launches the other instantiation
     if (m && !k->deferQ) {

@@ -65,6 +65,7 @@ struct swe_hip_block {
     hipStream_t stream = nullptr;
     float fluxDt = -1.f;  // dt of the last flavour-S flux pass (its Q* depends on it)
     const unsigned* mlp = nullptr;  // attached learned middle state (weight fragments), or null
+    swe_hip_mlp* net = nullptr;     // the attached network's handle
     uint2* deferQ = nullptr;        // deferred-edge queue of the learned middle state's fused loop
     unsigned deferCap = 0;          // its capacity: every edge of the block
     // quiet-tile record of the fused loop (kernels_fused.hpp): two slots of
@@ -190,12 +191,20 @@ size_t max_tiles(const swe_hip_block* k) {
     return (size_t)((k->ny + kStripRows - 1) / kStripRows) * (size_t)((k->nx + 3) / 4);
 }
 
+// Workgroups the quiet-tile plan can use at any width of whole record tiles.
+size_t plan_groups_max(const swe_hip_block* k) {
+    const int nyt = (k->ny + kStripRows - 1) / kStripRows;
+    size_t g = 0;
+    for (int xs : {4, 8, 12, 16, 32}) g = std::max(g, (size_t)plan_groups(nyt, (k->nx + xs - 1) / xs, xs / kQuietCols));
+    return g;
+}
+
 int ensure_quiet_record(swe_hip_block* k) {
     if (k->quiet) return 0;
     const size_t n = 2 * max_tiles(k);
     HIP_TRY(hipMalloc(&k->quiet, n * sizeof(unsigned)));
     HIP_TRY(hipMalloc(&k->tspeed, n * sizeof(float)));
-    const size_t groups = (max_tiles(k) + kPlanGroup - 1) / kPlanGroup;
+    const size_t groups = plan_groups_max(k);
     HIP_TRY(hipMalloc(&k->tlist, groups * kPlanGroup * kPlanPieces * sizeof(unsigned)));
     HIP_TRY(hipMalloc(&k->tcount, groups * sizeof(unsigned)));
     return 0;
@@ -280,8 +289,10 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             io.quiet = k->quiet;
             io.speed = k->tspeed;
             io.step = s;
-            // (widths that are not a whole number of record tiles keep no record)
-            if (k->skipQuiet && s >= 3 && k->xseg % kQuietCols == 0) {
+            // (only the augmented solver keeps a record, at widths of whole record tiles)
+            const int es = defer ? SOLVER_AUGRIE_DEFER : eff_solver(k);
+            const bool record = (es == SOLVER_AUGRIE || es == SOLVER_AUGRIE_DEFER) && k->xseg % kQuietCols == 0;
+            if (k->skipQuiet && s >= 3 && record) {
                 if (int r = launch_quiet_plan(k->quiet, k->tspeed, k->nx, k->ny, k->xseg, s, k->ctl, s % 3, k->tlist,
                                               k->tcount, k->stream))
                     return r;
@@ -538,6 +549,7 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
 int swe_hip_destroy(swe_hip_block* k) {
     if (!k) return SWE_HIP_OK;
     if (k->stream) (void)hipStreamSynchronize(k->stream);
+    if (k->net) swe::mlp_detach(k->net, k->stream);
     for (int i = 0; i < 2; ++i) {
         if (k->h[i]) (void)hipFree(k->h[i]);
         if (k->hu[i]) (void)hipFree(k->hu[i]);
@@ -1105,6 +1117,11 @@ int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
     const unsigned* frags = m ? swe::mlp_frags(m) : nullptr;
     HIP_TRY(hipStreamSynchronize(k->stream));
     HIP_TRY(hipMemcpy(&k->ctl->mlp, &frags, sizeof(frags), hipMemcpyHostToDevice));
+    if (k->net != m) {
+        if (k->net) swe::mlp_detach(k->net, k->stream);
+        if (m) swe::mlp_attach(m, k->stream);
+        k->net = m;
+    }
     k->mlp = frags;
     if (!k->xsegSet) k->xseg = default_xseg(k->nx, k->ny, frags ? SOLVER_AUGRIE_DEFER : k->solver);
     return SWE_HIP_OK;
@@ -1153,7 +1170,7 @@ size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 size_t noalloc_layout(const swe_hip_block& k, size_t off[7]) {
     const size_t field = align256(field_elems(&k) * sizeof(float));
     const size_t tiles = max_tiles(&k);
-    const size_t groups = (tiles + kPlanGroup - 1) / kPlanGroup;
+    const size_t groups = plan_groups_max(&k);
     size_t at = 0;
     off[0] = at;
     at += align256(sizeof(DeviceCtl));

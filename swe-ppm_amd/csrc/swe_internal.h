@@ -22,6 +22,10 @@ void set_error(const char* fmt, ...);
 // device weight fragments of a network.
 int block_view(swe_hip_block* k, StateView* v, hipStream_t* stream);
 const unsigned* mlp_frags(swe_hip_mlp* m);
+// Attachment bookkeeping of a network: `user` is the attached block's or
+// batch's stream (swe_mlp.hip).
+void mlp_attach(swe_hip_mlp* m, hipStream_t user);
+void mlp_detach(swe_hip_mlp* m, hipStream_t user);
 int hip_fail(hipError_t e, const char* what);
 
 // Host copy of the flavour-U dt rule (SWE_DimensionalSplittingMpi.cpp:448-456);

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <mutex>
 #include <vector>
 
 #include "../../include/swe_hip_mlp.h"
@@ -370,15 +371,36 @@ struct swe_hip_mlp {
     int lossCap = 0;
     int adamStep = 0;
     hipStream_t stream = nullptr;
+    // streams of the blocks and batches the network is attached to (one
+    // entry per attachment): destroy is refused while any remain, and a
+    // repack waits for their queued work before rewriting the weights
+    std::mutex usersMu;
+    std::vector<hipStream_t> users;
 };
 
 namespace swe {
 const unsigned* mlp_frags(swe_hip_mlp* m) { return m->frags; }
+
+void mlp_attach(swe_hip_mlp* m, hipStream_t user) {
+    std::lock_guard<std::mutex> lock(m->usersMu);
+    m->users.push_back(user);
+}
+
+void mlp_detach(swe_hip_mlp* m, hipStream_t user) {
+    std::lock_guard<std::mutex> lock(m->usersMu);
+    auto it = std::find(m->users.begin(), m->users.end(), user);
+    if (it != m->users.end()) m->users.erase(it);
+}
 }  // namespace swe
 
 namespace {
 
 int repack(swe_hip_mlp* m) {
+    {
+        // no kernel of an attached block may read the fragments while they change
+        std::lock_guard<std::mutex> lock(m->usersMu);
+        for (hipStream_t s : m->users) HIP_TRY(hipStreamSynchronize(s));
+    }
     std::vector<float> P(kParams);
     HIP_TRY(hipMemcpyAsync(P.data(), m->P, sizeof(float) * kParams, hipMemcpyDeviceToHost, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
@@ -498,6 +520,15 @@ int swe_hip_mlp_create(int64_t capacity, uint64_t seed, swe_hip_mlp** out) {
 
 int swe_hip_mlp_destroy(swe_hip_mlp* m) {
     if (!m) return SWE_HIP_OK;
+    {
+        std::lock_guard<std::mutex> lock(m->usersMu);
+        if (!m->users.empty()) {
+            set_error("swe_hip_mlp_destroy: the network is attached to %zu block(s) or batch(es); detach it "
+                      "(swe_hip_set_surrogate / swe_hip_batch_set_surrogate with NULL) or destroy them first",
+                      m->users.size());
+            return SWE_HIP_ESTATE;
+        }
+    }
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (void* p : {(void*)m->P, (void*)m->M, (void*)m->V, (void*)m->frags, (void*)m->samples, (void*)m->dCount,
                     (void*)m->act, (void*)m->partials, (void*)m->lossLog})

@@ -389,8 +389,10 @@ class MiddleStateMLP:
         self._h = h
 
     def close(self) -> None:
+        """Free the network.  Refused (SweHipError, the handle kept) while a
+        block or batch still has it attached."""
         if self._h:
-            lib.swe_hip_mlp_destroy(self._h)
+            _check(lib.swe_hip_mlp_destroy(self._h), "mlp_destroy")
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -301,3 +301,29 @@ def test_surrogate_block_matches_exact_solver_to_its_fp32_error(swe, oracle):
     d = np.max(np.abs(out["exact"][0] - out["surrogate"][0])) / np.max(np.abs(out["exact"][0]))
     print(f"rel Linf vs FP64: exact {err['exact']:.3e}, surrogate {err['surrogate']:.3e}; between {d:.3e}")
     assert err["surrogate"] <= 1.5 * err["exact"] + 1e-5
+
+
+def test_destroying_an_attached_network_is_refused(swe):
+    """A block or batch with a network attached reads its weights on the
+    device: destroying the network is refused (SWE_HIP_ESTATE) until every
+    attachment is gone, by detaching or by destroying the holder.  Creates,
+    attaches and frees only; no step runs."""
+    from swe_ppm_amd import scenarios, surrogate
+
+    net = surrogate.load_default()
+    blk = swe.SweBlock(64, 64, 1.0, 1.0, solver=swe.AUGRIE)
+    blk.set_surrogate(net)
+    with pytest.raises(swe.SweHipError, match="attached"):
+        net.close()
+    blk.set_surrogate(None)
+    inst = scenarios.ensemble_instances(count=2, seed=3, nmin=16, nmax=20)
+    bt = swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE)
+    bt.set_surrogate(net)
+    blk.set_surrogate(net)
+    with pytest.raises(swe.SweHipError, match="2 block"):
+        net.close()
+    bt.close()
+    with pytest.raises(swe.SweHipError, match="1 block"):
+        net.close()
+    blk.close()
+    net.close()  # now allowed
