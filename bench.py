@@ -300,6 +300,49 @@ def every_tile_line(swe, state, steps: int, xseg: int):
             "fused_kernel_avg_us": round(kms[1] / max(kl[1], 1) * 1e3, 3), "xseg": xseg}
 
 
+def per_call_line(state, steps: int, warmup: int):
+    """The headline workload driven through the drop-in block one call at a
+    time, as the reference's single-block driver does (swe_simple.cpp:225-239:
+    setGhostLayer, computeNumericalFluxes, getMaxTimestep, updateUnknowns per
+    step): swe_simple_hip from the same initial state, its own wall clock
+    around the loop (host round trip for dt included), with and without the
+    look-ahead launches."""
+    import re
+    import subprocess
+    import tempfile
+
+    h, hu, hv, b, dx, dy, bnd = state
+    n = h.shape[0] - 2
+    exe = ROOT / "swe-ppm_amd" / "host" / "build" / "swe_simple_hip"
+    if not exe.exists():
+        return {"skipped": f"{exe} not built"}
+    out = {"workload": f"the headline workload ({n}x{n} islands, AugRie, {steps} steps from t=0) through "
+                       "SWE_DimensionalSplittingHip's setGhostLayer / computeNumericalFluxes / getMaxTimestep / "
+                       "updateUnknowns, one call each per step (swe_simple_hip --input)",
+           "unit": "MCell-updates/s"}
+    with tempfile.TemporaryDirectory() as tmp:
+        snap = Path(tmp) / "islands.bin"
+        with open(snap, "wb") as f:
+            np.array([1, n, n], np.int32).tofile(f)
+            for a in (h, hu, hv, b):
+                np.ascontiguousarray(a, np.float32).tofile(f)
+        for key, extra in (("lookahead", []), ("no_lookahead", ["--lookahead", "0"])):
+            cmd = [str(exe), "--input", str(snap), "-t", "1e9", "-n", "1", "--solver", "augrie", "--max-steps",
+                   str(steps), "--warmup", str(warmup)] + extra
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+            m = re.search(r"iterations (\d+) simulated ([\d.]+) s wall ([\d.]+) s\s+([\d.]+) MCell", p.stdout)
+            if p.returncode != 0 or not m:
+                out[key] = {"error": (p.stderr or p.stdout)[-300:]}
+                continue
+            it, wall = int(m.group(1)), float(m.group(3))
+            out[key] = {"value": round(float(m.group(4)), 3), "ms_per_step": round(wall / it * 1e3, 6),
+                        "simulated_s": float(m.group(2))}
+    if "value" in out.get("lookahead", {}):
+        out["value"] = out["lookahead"]["value"]
+        out["ms_per_step"] = out["lookahead"]["ms_per_step"]
+    return out
+
+
 def disturbed_line(swe, scenarios, n: int, seed: int, steps: int, noise: float = 0.01):
     """The headline workload with seeded surface noise of +-noise m on every
     cell: no edge starts still, so every wet edge runs the full augmented
@@ -540,6 +583,9 @@ def main() -> int:
     }
     line["every_tile"] = every_tile_line(swe, (h, hu, hv, b, dx, dy, bnd), args.steps, xseg)
     line["every_tile"]["vs_headline"] = round(line["every_tile"]["value"] / mcells, 4)
+    line["per_call"] = per_call_line((h, hu, hv, b, dx, dy, bnd), args.steps, args.warmup)
+    if "value" in line["per_call"]:
+        line["per_call"]["vs_headline"] = round(line["per_call"]["value"] / mcells, 4)
     line["disturbed"] = disturbed_line(swe, scenarios, n, args.seed, args.steps)
     line["disturbed"]["vs_headline"] = round(line["disturbed"]["value"] / mcells, 4)
     if not args.no_solvers:

@@ -20,10 +20,12 @@
  *  - Host arrays use the reference's Float2D layout: (nx+2) x (ny+2) floats,
  *    a[x][y] = raw[x*(ny+2) + y], ghost cells at 0 and n+1
  *    (src/tools/Float2D.hh:74-80).
- *  - A handle is used by one host thread at a time; calls on one handle are
- *    ordered on the handle's HIP stream.  Different handles may be driven
- *    from different threads (the overdecomposed drivers do this,
- *    swe_mpi_overdecomp.cpp:269-323).
+ *  - Calls on one handle are ordered on the handle's HIP stream and
+ *    serialised by the library (a layer copy or transfer takes the lock of
+ *    every block it touches, since reading a block's layer applies its
+ *    pending update).  Different handles may be driven from different
+ *    threads (the overdecomposed drivers do this,
+ *    swe_mpi_overdecomp.cpp:269-323), neighbour exchanges included.
  *  - Enumerations match the reference: BoundaryType and Boundary
  *    (src/types/Boundary.hh:5-28), solver ids of WAVE_PROPAGATION_SOLVER
  *    (SWE_DimensionalSplittingMpi.hh:42-50) plus 3 = Hybrid.
@@ -150,7 +152,21 @@ int swe_hip_fluxes(swe_hip_block* blk, float* max_timestep);
 /* updateUnknowns (flavour U: SWE_DimensionalSplittingMpi.cpp:484-518, dry
  * clamp included; flavour S: SWE_DimensionalSplitting.cpp:187-210) with time
  * step dt (the reference's U blocks use their maxTimestep member, which the
- * driver may have lowered to the global minimum). */
+ * driver may have lowered to the global minimum).  It needs a flux pass of
+ * the current state (SWE_HIP_ESTATE otherwise, e.g. a second update after
+ * one swe_hip_fluxes).
+ *
+ * Flavour U defers the work: the update is applied by the next call that
+ * needs the state; when that is swe_hip_fluxes after swe_hip_apply_bc (the
+ * drivers' order), one fused launch applies the update and the boundary
+ * conditions as it loads the cells and solves the next edges.  Once a caller
+ * has called update(dt returned) -> apply_bc -> fluxes, each swe_hip_fluxes
+ * also queues that next launch at once, with the dt it returns, so the GPU
+ * works while the host goes round its loop; a different next call (another
+ * dt, another order, any other function) drops it, at the cost of the wasted
+ * launch (swe_hip_set_lookahead turns this off).  Results are bit-identical
+ * to applying every update at once; quiet tiles are skipped as in the device
+ * loop (swe_hip_set_quiet_skip). */
 int swe_hip_update(swe_hip_block* blk, float dt);
 
 /* nsteps of the single-block global-time-stepping loop of swe_simple.cpp:
@@ -248,6 +264,12 @@ int swe_hip_autotune(swe_hip_block* blk, int steps, float* ms_per_step);
  * full sweep).  No reference counterpart: the reference's loops visit every
  * edge every step. */
 int swe_hip_set_quiet_skip(swe_hip_block* blk, int enable);
+
+/* Look-ahead of the per-call API (see swe_hip_update), on by default;
+ * enable = 0 queues each step only when its calls arrive.  Results are the
+ * same either way.  No reference counterpart (the reference's calls run
+ * synchronously on the host). */
+int swe_hip_set_lookahead(swe_hip_block* blk, int enable);
 
 /* nsteps of swe_hip_run (the fused device loop of flavour U with global time
  * stepping, quiet tiles skipped) on caller-owned device arrays, allocating

@@ -108,6 +108,9 @@ struct FusedIO {
     // to compute at list[g * kPlanGroup * kPlanPieces ...]
     const unsigned* list = nullptr;
     const unsigned* count = nullptr;
+    // >= 0: the previous update's dt, given by the caller (the per-call
+    // drop-in API: updateUnknowns' dt applied on the next flux pass's load)
+    float dtFixed = -1.f;
 };
 
 // The list entry of wave w in list mode, or -1 past the last listed entry: a
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
     const int nxs = (v.nx + XS - 1) / XS;
     const int nx = v.nx, ny = v.ny;
 
-    const float dt = timestep_from_max(__uint_as_float(ctl->smax[slotIn]), v.dx, v.dy);
+    const float dt = io.dtFixed >= 0.f ? io.dtFixed : timestep_from_max(__uint_as_float(ctl->smax[slotIn]), v.dx, v.dy);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->smax[slotClear] = 0u;
         ctl->defer[slotClear] = 0u;

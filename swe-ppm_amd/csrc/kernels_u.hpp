@@ -148,6 +148,9 @@ struct DeviceCtl {
     const unsigned* mlp;  // weight fragments of the learned middle state (solver SOLVER_AUGRIE_MLP)
     unsigned int defer[3];  // edges queued by the deferring sweep, per CFL slot (SOLVER_AUGRIE_DEFER)
     unsigned int busy[3];   // busy record tiles found by the quiet-tile plan of each CFL slot's launch
+    // coherent host ring the readback kernel writes (kept here rather than
+    // passed as a kernel argument, so the launch is an ordinary one)
+    unsigned long long* ring;
 };
 
 // Which CFL-maximum slot a launch reads or accumulates into, and which slot

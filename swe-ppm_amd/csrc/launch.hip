@@ -86,11 +86,13 @@ __global__ void apply_bc_bathymetry_kernel(float* b, int nx, int ny, int pitch, 
 // the previous launch's plan over kPlanWaves, between 1 and `fine` (with few
 // busy tiles short pieces spread them over more waves, the busiest wave sets
 // the launch time; with many, long pieces solve fewer halo edges).  The
-// number of busy tiles goes to ctl->busy[slotOut].
+// number of busy tiles goes to ctl->busy[slotOut].  rbSeq != 0: the plan also
+// reports the previous launch's CFL maximum to the host, as readback_kernel
+// does (one kernel less between a flux pass and the next).
 __global__ __launch_bounds__(kPlanGroup) void quiet_plan_kernel(const unsigned* __restrict__ quiet, float* speed,
                                                                 int nyt, int nxs, int nxf, int fine, int lanes,
                                                                 int step, DeviceCtl* ctl, int slotPrev3, int slotOut,
-                                                                unsigned* list, unsigned* count) {
+                                                                unsigned* list, unsigned* count, unsigned rbSeq) {
     __shared__ unsigned wavePieces[kPlanGroup / 64];
     __shared__ unsigned waveBusy[kPlanGroup / 64];
     __shared__ unsigned waveMax[kPlanGroup / 64];
@@ -100,6 +102,9 @@ __global__ __launch_bounds__(kPlanGroup) void quiet_plan_kernel(const unsigned* 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int slotNow = (step & 1) * tiles, slotPrev = ((step - 1) & 1) * tiles;
     const unsigned prevBusy = ctl->busy[slotPrev3];
+    if (rbSeq != 0u && blockIdx.x == 0 && threadIdx.x == 0)  // the previous launch's maximum to the host
+        __hip_atomic_store(&ctl->ring[rbSeq % kReadbackRing], ((unsigned long long)rbSeq << 32) | ctl->smax[slotPrev3],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const int piece = prevBusy == 0u ? fine : (int)min((unsigned)fine, max(1u, (prevBusy + kPlanWaves - 1) / kPlanWaves));
     const int yt = seg % nyt, xs = seg / nyt;
     const int f = xs * fine + i;
@@ -163,6 +168,19 @@ __global__ __launch_bounds__(kPlanGroup) void quiet_plan_kernel(const unsigned* 
         count[blockIdx.x] = total;
         if (nbusy) atomicAdd(&ctl->busy[slotOut], nbusy);
         if (mx > 0u) atomicMax(&ctl->smax[slotOut], mx);
+    }
+}
+
+// The CFL maximum of a slot to the host: one 64-bit word (sequence number
+// above, float bits below) stored at system scope into coherent host memory,
+// ctl->ring[seq % kReadbackRing], which the host polls for that number.
+// A kernel of its own after the flux pass: it needs no event in the stream
+// (an event there costs the stream a pause before its next kernel), and
+// the host learns the maximum while the stream runs on.
+__global__ void readback_kernel(const DeviceCtl* ctl, int slot, unsigned seq) {
+    if (threadIdx.x == 0) {
+        const unsigned long long w = ((unsigned long long)seq << 32) | ctl->smax[slot];
+        __hip_atomic_store(&ctl->ring[seq % kReadbackRing], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -321,15 +339,23 @@ int launch_apply_bc_bathymetry(float* b, int nx, int ny, int pitch, BoundaryInfo
 }
 
 int launch_quiet_plan(const unsigned* quiet, float* speed, int nx, int ny, int xseg, int step, DeviceCtl* ctl,
-                      int slotOut, unsigned* list, unsigned* count, hipStream_t stream) {
+                      int slotPrev, int slotOut, unsigned* list, unsigned* count, unsigned rbSeq,
+                      hipStream_t stream) {
     const int nyt = (ny + kStripRows - 1) / kStripRows, nxs = (nx + xseg - 1) / xseg;
     const int nxf = (nx + kQuietCols - 1) / kQuietCols;
     const int fine = xseg / kQuietCols;
     hipLaunchKernelGGL(quiet_plan_kernel, dim3((unsigned)plan_groups(nyt, nxs, fine)), dim3(kPlanGroup), 0, stream,
-                       quiet, speed, nyt, nxs, nxf, fine, plan_lanes(fine), step, ctl, (step + 2) % 3, slotOut, list,
-                       count);
+                       quiet, speed, nyt, nxs, nxf, fine, plan_lanes(fine), step, ctl, slotPrev, slotOut, list,
+                       count, rbSeq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "quiet_plan_kernel launch");
+    return 0;
+}
+
+int launch_readback(const DeviceCtl* ctl, int slot, unsigned seq, hipStream_t stream) {
+    hipLaunchKernelGGL(readback_kernel, dim3(1), dim3(64), 0, stream, ctl, slot, seq);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "readback_kernel launch");
     return 0;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -75,6 +76,34 @@ struct swe_hip_block {
     unsigned* tlist = nullptr;   // tiles listed for the next fused launch, per plan group
     unsigned* tcount = nullptr;  // their number per plan group
     bool skipQuiet = true;  // skip quiet tiles (swe_hip_set_quiet_skip)
+    // per-call API (swe_hip_fluxes / swe_hip_update) of flavour U: the update
+    // of swe_hip_update is kept pending and applied by the next flux pass on
+    // load (the fused kernel), unless another call needs the state first
+    bool pending = false;    // an update of dt pendingDt is not yet applied
+    bool pendingBc = false;  // swe_hip_apply_bc was called after it
+    float pendingDt = 0.f;
+    int pcSums = 0;  // sums set holding the last flux pass's sums (0: sums, 1: sums2)
+    bool fluxValid = false;  // those sums belong to the current state (swe_hip_update may use them)
+    int pcSlot = 0;  // CFL slot of the next flux pass (0, 1, 2 in turn)
+    int pcStep = 0;  // fused flux passes since the last plain one: the quiet record's step
+    // look-ahead (swe_hip_set_lookahead): once a caller has gone through
+    // update(the dt returned) -> apply_bc -> fluxes, each flux pass queues
+    // the next one at once, with the dt it returns, and that launch is kept
+    // if the caller goes on in the same order
+    bool lookahead = true;
+    bool spec = false;     // a look-ahead launch is queued
+    int specOut = 0;       // its CFL slot
+    float lastDt = 0.f;    // the dt the last flux pass returned (and the look-ahead assumed)
+    bool pcClean = false;  // no call but swe_hip_update / swe_hip_apply_bc since that pass
+    // each flux pass's CFL maximum is written by a small kernel into a ring
+    // of coherent host words, tagged with the pass's sequence number
+    unsigned long long* rbRing = nullptr;
+    unsigned rbSeq = 0;  // sequence number of the last report asked for (never 0)
+    // Calls on one block are serialised: besides its own calls, a block is
+    // read by its neighbours' layer copies, which apply its pending update
+    // (drivers run blocks on several host threads).  Recursive: some entry
+    // points call others.
+    std::recursive_mutex mu;
 };
 
 namespace {
@@ -143,6 +172,56 @@ StateView view(swe_hip_block* k, int buf) {
 
 SumsView sums_view(swe_hip_block* k) { return SumsView{k->sums[0], k->sums[1], k->sums[2], k->sums[3]}; }
 SumsView sums2_view(swe_hip_block* k) { return SumsView{k->sums2[0], k->sums2[1], k->sums2[2], k->sums2[3]}; }
+SumsView pc_sums(swe_hip_block* k) { return k->pcSums ? sums2_view(k) : sums_view(k); }
+
+// Apply a pending per-call update now, as swe_hip_update did it eagerly
+// (interior update with the dry clamp), then the boundary conditions if they
+// were asked for since.
+int flush_pending(swe_hip_block* k) {
+    if (!k->pending) return 0;
+    k->pending = false;
+    k->pcStep = 0;  // updated in place: the state buffers no longer alternate
+    if (int r = launch_apply_u(k->math, view(k, k->cur), pc_sums(k), k->bi, k->ctl, Slots{0, 1}, k->pendingDt, 0, nullptr,
+                               0, k->stream))
+        return r;
+    if (k->pendingBc) {
+        k->pendingBc = false;
+        if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
+    }
+    return 0;
+}
+
+// Give up a queued look-ahead launch: what it writes (the other state
+// buffer and sums set, a CFL slot, a quiet record slot) is never used, and
+// the quiet record starts again.
+void drop_lookahead(swe_hip_block* k) {
+    if (!k->spec) return;
+    k->spec = false;
+    k->pcStep = 0;
+}
+
+// Entry of every call other than swe_hip_update, swe_hip_apply_bc and
+// swe_hip_fluxes: the state is brought to what the caller has asked for.
+int settle(swe_hip_block* k) {
+    drop_lookahead(k);
+    k->pcClean = false;
+    return flush_pending(k);
+}
+
+// ... and of calls that write the state, ghosts or boundary types (or may,
+// through device pointers): the quiet record starts again.
+int touch(swe_hip_block* k) {
+    const int r = settle(k);
+    k->pcStep = 0;
+    return r;
+}
+
+// The state is replaced: nothing pending, no flux pass to update with.
+void discard_per_call(swe_hip_block* k) {
+    drop_lookahead(k);
+    k->pending = k->pendingBc = k->fluxValid = k->pcClean = false;
+    k->pcStep = 0;
+}
 
 int check(swe_hip_block* k) {
     if (!k) {
@@ -210,6 +289,96 @@ int ensure_quiet_record(swe_hip_block* k) {
     return 0;
 }
 
+// A fused flux pass of the per-call API (kernels_fused.hpp): it applies the
+// pending update -- of dt dtFixed, or with dtFixed < 0 of the dt that CFL
+// slot slotIn gives, as a look-ahead launch does -- and the boundary
+// conditions as it loads the cells, writes that state to the other buffer
+// and its sums to the other set, its CFL maximum to slot `out`, and clears
+// the slot after it.  The caller switches buffers and sums sets when it
+// keeps the pass.  The augmented solver records quiet tiles from the first
+// fused pass after a plain one and skips them from the third, as the device
+// loop does: the passes between alternate the buffers with nothing else
+// writing the state (every call that could resets pcStep).  rbSeq != 0:
+// slot slotIn (the pass before, complete by then) is reported to the host
+// under that sequence number first, by the quiet-tile plan where there is
+// one, else by a readback kernel.
+int pc_fused(swe_hip_block* k, int slotIn, int out, float dtFixed, unsigned rbSeq) {
+    if (int r = ensure_second_buffer(k)) return r;
+    if (int r = ensure_second_sums(k)) return r;
+    const int o = k->cur ^ 1;
+    FusedIO io{k->h[k->cur], k->hu[k->cur], k->hv[k->cur], k->b, k->h[o], k->hu[o], k->hv[o], pc_sums(k),
+               k->pcSums ? sums_view(k) : sums2_view(k)};
+    io.dtFixed = dtFixed;
+    const int step = k->pcStep + 1;
+    const int es = eff_solver(k);
+    if (k->skipQuiet && es == SOLVER_AUGRIE && k->xseg % kQuietCols == 0) {
+        if (int r = ensure_quiet_record(k)) return r;
+        io.quiet = k->quiet;
+        io.speed = k->tspeed;
+        io.step = step;
+        if (step >= 3) {
+            if (int r = launch_quiet_plan(k->quiet, k->tspeed, k->nx, k->ny, k->xseg, step, k->ctl, slotIn, out,
+                                          k->tlist, k->tcount, rbSeq, k->stream))
+                return r;
+            rbSeq = 0u;
+            io.list = k->tlist;
+            io.count = k->tcount;
+        }
+    }
+    if (rbSeq != 0u)
+        if (int r = launch_readback(k->ctl, slotIn, rbSeq, k->stream)) return r;
+    if (int r = launch_fused_u(es, k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl, slotIn, out, (out + 1) % 3,
+                               nullptr, 0, k->stream))
+        return r;
+    k->pcStep = step;
+    return 0;
+}
+
+// Wait for the report of flux pass `seq` (launch_readback) and return its
+// CFL maximum's bits.  The host polls the coherent ring word.  Only after
+// kStreamCheckMs without it does it ask whether the stream still has work
+// (so that a failed stream, or one that has run everything without the word
+// arriving, ends the wait with an error): a stream query queues a marker,
+// which costs the stream a pause before its next kernel.
+constexpr double kStreamCheckMs = 20.0;
+
+int wait_readback(swe_hip_block* k, unsigned seq, unsigned* bits) {
+    unsigned long long* word = &k->rbRing[seq % kReadbackRing];
+    auto last = std::chrono::steady_clock::now();
+    for (unsigned polls = 1;; ++polls) {
+        const unsigned long long v = __atomic_load_n(word, __ATOMIC_ACQUIRE);
+        if ((unsigned)(v >> 32) == seq) {
+            *bits = (unsigned)v;
+            return 0;
+        }
+        if ((polls & 4095u) == 0u) {
+            const auto now = std::chrono::steady_clock::now();
+            if (std::chrono::duration<double, std::milli>(now - last).count() >= kStreamCheckMs) {
+                last = now;
+                const hipError_t q = hipStreamQuery(k->stream);
+                if (q == hipSuccess) {
+                    const unsigned long long w = __atomic_load_n(word, __ATOMIC_ACQUIRE);
+                    if ((unsigned)(w >> 32) == seq) {
+                        *bits = (unsigned)w;
+                        return 0;
+                    }
+                    set_error("swe_hip_fluxes: the flux pass ended without its CFL maximum reaching the host");
+                    return SWE_HIP_EDEVICE;
+                }
+                if (q != hipErrorNotReady) return hip_fail(q, "swe_hip_fluxes");
+            }
+        }
+        __builtin_ia32_pause();
+    }
+}
+
+bool same_bits(float a, float b) {
+    unsigned x, y;
+    std::memcpy(&x, &a, sizeof(x));
+    std::memcpy(&y, &b, sizeof(y));
+    return x == y;
+}
+
 // Workgroups of the deferred-edge launch: one round of waves at the learned
 // solver's three waves per SIMD.
 int defer_groups() {
@@ -254,6 +423,7 @@ struct LoopTiming {
 // kernels on the stream; dt goes to the device log k->dtlog.
 int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log, LoopTiming* tm,
                       bool hostIO = true) {
+    k->fluxValid = false;  // the loop's sums are not a flux pass of the per-call API
     if (int r = ensure_dtlog(k, nsteps)) return r;
     if (k->strategy == 0 || k->strategy == 2)
         if (int r = ensure_second_buffer(k)) return r;
@@ -293,8 +463,8 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             const int es = defer ? SOLVER_AUGRIE_DEFER : eff_solver(k);
             const bool record = (es == SOLVER_AUGRIE || es == SOLVER_AUGRIE_DEFER) && k->xseg % kQuietCols == 0;
             if (k->skipQuiet && s >= 3 && record) {
-                if (int r = launch_quiet_plan(k->quiet, k->tspeed, k->nx, k->ny, k->xseg, s, k->ctl, s % 3, k->tlist,
-                                              k->tcount, k->stream))
+                if (int r = launch_quiet_plan(k->quiet, k->tspeed, k->nx, k->ny, k->xseg, s, k->ctl, (s + 2) % 3, s % 3,
+                                              k->tlist, k->tcount, 0u, k->stream))
                     return r;
                 io.list = k->tlist;
                 io.count = k->tcount;
@@ -482,6 +652,8 @@ int block_view(swe_hip_block* k, StateView* v, hipStream_t* stream) {
         set_error("block_view: a flavour-U block is needed");
         return SWE_HIP_EINVAL;
     }
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     *v = view(k, k->cur);
     *stream = k->stream;
     return 0;
@@ -524,6 +696,9 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
     const size_t bytes = field_elems(k) * sizeof(float);
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&k->rbRing, kReadbackRing * sizeof(unsigned long long), hipHostMallocCoherent);
+    if (e == hipSuccess) std::memset(k->rbRing, 0, kReadbackRing * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&k->h[0], bytes);
     if (e == hipSuccess) e = hipMalloc(&k->hu[0], bytes);
     if (e == hipSuccess) e = hipMalloc(&k->hv[0], bytes);
@@ -531,6 +706,8 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipMalloc(&k->sums[i], bytes);
     if (e == hipSuccess) e = hipMalloc(&k->ctl, sizeof(DeviceCtl));
     if (e == hipSuccess) e = hipMemsetAsync(k->ctl, 0, sizeof(DeviceCtl), k->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&k->ctl->ring, &k->rbRing, sizeof(k->rbRing), hipMemcpyHostToDevice, k->stream);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipMemsetAsync(k->sums[i], 0, bytes, k->stream);
     if (e == hipSuccess) e = hipMemsetAsync(k->h[0], 0, bytes, k->stream);
     if (e == hipSuccess) e = hipMemsetAsync(k->hu[0], 0, bytes, k->stream);
@@ -567,6 +744,7 @@ int swe_hip_destroy(swe_hip_block* k) {
     if (k->tspeed) (void)hipFree(k->tspeed);
     if (k->tlist) (void)hipFree(k->tlist);
     if (k->tcount) (void)hipFree(k->tcount);
+    if (k->rbRing) (void)hipHostFree(k->rbRing);
     if (k->stream) (void)hipStreamDestroy(k->stream);
     delete k;
     return SWE_HIP_OK;
@@ -574,6 +752,8 @@ int swe_hip_destroy(swe_hip_block* k) {
 
 int swe_hip_upload(swe_hip_block* k, const float* h, const float* hu, const float* hv, const float* b) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    discard_per_call(k);  // the state is replaced
     if (!h || !hu || !hv || !b) {
         set_error("swe_hip_upload: null array");
         return SWE_HIP_EINVAL;
@@ -589,6 +769,8 @@ int swe_hip_upload(swe_hip_block* k, const float* h, const float* hu, const floa
 
 int swe_hip_download(swe_hip_block* k, float* h, float* hu, float* hv, float* b) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = settle(k)) return r;
     const size_t bytes = field_elems(k) * sizeof(float);
     if (h) HIP_TRY(hipMemcpyAsync(h, k->h[k->cur], bytes, hipMemcpyDeviceToHost, k->stream));
     if (hu) HIP_TRY(hipMemcpyAsync(hu, k->hu[k->cur], bytes, hipMemcpyDeviceToHost, k->stream));
@@ -600,6 +782,8 @@ int swe_hip_download(swe_hip_block* k, float* h, float* hu, float* hv, float* b)
 
 int swe_hip_set_boundary(swe_hip_block* k, int side, int type) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (side < BND_LEFT || side > BND_TOP || type < OUTFLOW || type > PASSIVE) {
         set_error("swe_hip_set_boundary: invalid side %d or type %d", side, type);
         return SWE_HIP_EINVAL;
@@ -615,11 +799,22 @@ int swe_hip_set_boundary(swe_hip_block* k, int side, int type) {
 
 int swe_hip_apply_bc(swe_hip_block* k) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (k->pending) {  // the next flux pass writes these ghosts as it applies the update
+        k->pendingBc = true;
+        return SWE_HIP_OK;
+    }
+    // ghosts of the current state rewritten: not the order a look-ahead assumes
+    drop_lookahead(k);
+    k->pcClean = false;
+    k->pcStep = 0;
     return launch_apply_bc(view(k, k->cur), k->bi, k->stream);
 }
 
 int swe_hip_apply_bc_bathymetry(swe_hip_block* k) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;
     HIP_TRY(hipStreamSynchronize(k->stream));
     return SWE_HIP_OK;
@@ -627,6 +822,8 @@ int swe_hip_apply_bc_bathymetry(swe_hip_block* k) {
 
 int swe_hip_init_grid_scenario(swe_hip_block* k, const swe_hip_grid* bathymetry, const swe_hip_grid* displacement) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    discard_per_call(k);  // the state is replaced
     if (!grid_ok(bathymetry) || (displacement && !grid_ok(displacement))) {
         set_error("swe_hip_init_grid_scenario: a grid needs values, nx, ny >= 1 and positive spacing");
         return SWE_HIP_EINVAL;
@@ -658,6 +855,8 @@ int swe_hip_init_grid_scenario(swe_hip_block* k, const swe_hip_grid* bathymetry,
 
 int swe_hip_init_scenario(swe_hip_block* k, int scenario) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    discard_per_call(k);  // the state is replaced
     if (scenario < SWE_SCENARIO_HALF_DOMAIN_DRY || scenario > SWE_SCENARIO_CENTRED_DAM_BREAK) {
         set_error("swe_hip_init_scenario: unknown scenario %d", scenario);
         return SWE_HIP_EINVAL;
@@ -675,21 +874,70 @@ int swe_hip_init_scenario(swe_hip_block* k, int scenario) {
 
 int swe_hip_fluxes(swe_hip_block* k, float* max_timestep) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
     if (!max_timestep) {
         set_error("swe_hip_fluxes: null output");
         return SWE_HIP_EINVAL;
     }
     unsigned int bits = 0;
     if (k->flavour == FLAVOUR_U) {
-        if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
-        if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_STORE, k->xseg, view(k, k->cur), sums_view(k), nullptr, nullptr,
-                                   nullptr, k->bi, k->ctl, Slots{0, 1}, -1.f, 0, nullptr, 0, k->stream))
+        // an update left pending by swe_hip_update, with the boundary
+        // conditions asked for since (the drivers' order: updateUnknowns,
+        // setGhostLayer, computeNumericalFluxes), is applied by the fused
+        // kernel as it loads the cells -- one launch for the whole step.
+        // Without the boundary conditions it is applied on its own first.
+        const bool inOrder = k->pcClean && k->pending && k->pendingBc && same_bits(k->pendingDt, k->lastDt);
+        int out;
+        if (k->spec && inOrder) {
+            // the look-ahead launch made exactly this pass: keep it
+            k->spec = false;
+            out = k->specOut;
+            k->pending = k->pendingBc = false;
+            k->cur ^= 1;
+            k->pcSums ^= 1;
+        } else {
+            drop_lookahead(k);
+            if (k->pending && !k->pendingBc)
+                if (int r = flush_pending(k)) return r;
+            out = k->pcSlot;
+            k->pcSlot = (out + 1) % 3;
+            if (int r = launch_ctl_reset(k->ctl, out, k->stream)) return r;
+            if (k->pending) {
+                if (int r = pc_fused(k, (out + 2) % 3, out, k->pendingDt, 0u)) return r;
+                k->pending = k->pendingBc = false;
+                k->cur ^= 1;
+                k->pcSums ^= 1;
+            } else {
+                if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_STORE, k->xseg, view(k, k->cur), pc_sums(k),
+                                           nullptr, nullptr, nullptr, k->bi, k->ctl, Slots{out, (out + 1) % 3}, -1.f,
+                                           0, nullptr, 0, k->stream))
+                    return r;
+                k->pcStep = 0;
+            }
+        }
+        k->fluxValid = true;
+        if (++k->rbSeq == 0u) k->rbSeq = 1u;  // 0 means "no report"
+        const unsigned want = k->rbSeq;
+        if (inOrder && k->lookahead && !k->mlp) {
+            // queue the next pass with the dt being returned while the host
+            // waits, if the caller keeps to the order just seen (the device
+            // forms that dt from the same maximum by the same IEEE
+            // operations); it reports this pass's maximum first
+            const int next = k->pcSlot;
+            k->pcSlot = (next + 1) % 3;
+            if (int r = pc_fused(k, out, next, -1.f, want)) return r;
+            k->spec = true;
+            k->specOut = next;
+        } else if (int r = launch_readback(k->ctl, out, want, k->stream)) {
             return r;
-        HIP_TRY(hipMemcpyAsync(&bits, &k->ctl->smax[0], sizeof(bits), hipMemcpyDeviceToHost, k->stream));
-        HIP_TRY(hipStreamSynchronize(k->stream));
+        }
+        if (int r = wait_readback(k, want, &bits)) return r;
         float s;
         std::memcpy(&s, &bits, sizeof(s));
-        *max_timestep = host_timestep_u(s, k->dx, k->dy);
+        const float dt = host_timestep_u(s, k->dx, k->dy);
+        k->lastDt = dt;
+        k->pcClean = true;
+        *max_timestep = dt;
     } else {
         if (int r = launch_ctl_reset(k->ctl, 0, k->stream)) return r;
         if (int r = launch_sweep_s_x(k->solver, view(k, k->cur), sums_view(k), k->ctl, 0, k->stream)) return r;
@@ -707,13 +955,22 @@ int swe_hip_fluxes(swe_hip_block* k, float* max_timestep) {
 
 int swe_hip_update(swe_hip_block* k, float dt) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
     if (k->flavour == FLAVOUR_U) {
         if (!(dt >= 0.f)) {
             set_error("swe_hip_update: dt must be >= 0");
             return SWE_HIP_EINVAL;
         }
-        return launch_apply_u(k->math, view(k, k->cur), sums_view(k), k->bi, k->ctl, Slots{0, 1}, dt, 0, nullptr, 0,
-                              k->stream);
+        if (!k->fluxValid) {
+            set_error("swe_hip_update: no flux pass of the current state (call swe_hip_fluxes first; device loops, "
+                      "uploads and scenario initialisation discard the last one)");
+            return SWE_HIP_ESTATE;
+        }
+        k->fluxValid = false;  // the sums are used up
+        k->pending = true;
+        k->pendingBc = false;
+        k->pendingDt = dt;
+        return SWE_HIP_OK;
     }
     // flavour S: Q* was formed with the flux pass's own dt (the reference
     // asserts that dt equals it, SWE_DimensionalSplitting.cpp:278)
@@ -726,6 +983,8 @@ int swe_hip_update(swe_hip_block* k, float dt) {
 
 int swe_hip_run(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (nsteps < 0) {
         set_error("swe_hip_run: nsteps < 0");
         return SWE_HIP_EINVAL;
@@ -752,6 +1011,8 @@ int swe_hip_run(swe_hip_block* k, int nsteps, float* t_inout, float* dt_log) {
 int swe_hip_run_timed(swe_hip_block* k, int nsteps, float* t_inout, float* total_ms, float* kernel_ms,
                       int* kernel_launches) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (nsteps < 1 || k->flavour != FLAVOUR_U) {
         set_error("swe_hip_run_timed: needs nsteps >= 1 and flavour U");
         return SWE_HIP_EINVAL;
@@ -792,12 +1053,16 @@ int swe_hip_run_timed(swe_hip_block* k, int nsteps, float* t_inout, float* total
 
 int swe_hip_synchronize(swe_hip_block* k) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = settle(k)) return r;
     HIP_TRY(hipStreamSynchronize(k->stream));
     return SWE_HIP_OK;
 }
 
 int swe_hip_device_pointers(swe_hip_block* k, float** h, float** hu, float** hv, float** b, int* pitch) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (h) *h = k->h[k->cur];
     if (hu) *hu = k->hu[k->cur];
     if (hv) *hv = k->hv[k->cur];
@@ -871,6 +1136,22 @@ int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fie
         set_error("swe_hip_copy_layer: invalid side %d or fields %d", side, fields);
         return SWE_HIP_EINVAL;
     }
+    std::unique_lock<std::recursive_mutex> lockDst(dst->mu, std::defer_lock), lockSrc(src->mu, std::defer_lock);
+    if (dst == src)
+        lockDst.lock();
+    else
+        std::lock(lockDst, lockSrc);
+    // the source's copy layer must hold its latest update; the destination's
+    // pending update keeps the ghost lines of CONNECT and PASSIVE sides as
+    // they are (the next flux pass carries them over), but rewrites those of
+    // OUTFLOW and WALL sides, so a copy into such a line waits for it.  A
+    // look-ahead launch of the destination read the old ghost line.
+    if (int r = settle(src)) return r;
+    drop_lookahead(dst);
+    dst->pcClean = false;
+    dst->pcStep = 0;
+    if (dst->bi.type[side] == OUTFLOW || dst->bi.type[side] == WALL)
+        if (int r = flush_pending(dst)) return r;
     const LineGeom gd = line_geom(dst, side, 1);
     const LineGeom gs = line_geom(src, opposite(side), 0);
     if (gd.n != gs.n) {
@@ -899,10 +1180,12 @@ int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fie
 
 static int layer_host_transfer(swe_hip_block* k, int side, int ghost, int fields, float* out, const float* in) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
     if (side < BND_LEFT || side > BND_TOP || fields <= 0 || fields > 15 || (!out && !in)) {
         set_error("layer transfer: invalid side %d, fields %d or buffer", side, fields);
         return SWE_HIP_EINVAL;
     }
+    if (int r = out ? settle(k) : touch(k)) return r;
     const LineGeom g = line_geom(k, side, ghost ? 1 : 0);
     size_t pos = 0;
     for (int bit = 1; bit <= 8; bit <<= 1) {
@@ -981,6 +1264,8 @@ int swe_hip_solve_edges(int solver, int math, int64_t n, const float* hL, const 
 
 int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (strategy != -1 && strategy != 0 && strategy != 1 && strategy != 2) {
         set_error("swe_hip_set_tuning: strategy must be 0, 1 or 2");
         return SWE_HIP_EINVAL;
@@ -999,6 +1284,8 @@ int swe_hip_set_tuning(swe_hip_block* k, int strategy, int xseg) {
 
 int swe_hip_set_quiet_skip(swe_hip_block* k, int enable) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (enable != 0 && enable != 1) {
         set_error("swe_hip_set_quiet_skip: enable must be 0 or 1");
         return SWE_HIP_EINVAL;
@@ -1007,8 +1294,22 @@ int swe_hip_set_quiet_skip(swe_hip_block* k, int enable) {
     return SWE_HIP_OK;
 }
 
+int swe_hip_set_lookahead(swe_hip_block* k, int enable) {
+    if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = settle(k)) return r;
+    if (enable != 0 && enable != 1) {
+        set_error("swe_hip_set_lookahead: enable must be 0 or 1");
+        return SWE_HIP_EINVAL;
+    }
+    k->lookahead = enable != 0;
+    return SWE_HIP_OK;
+}
+
 int swe_hip_set_math(swe_hip_block* k, int math) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (math != 0 && math != 1) {
         set_error("swe_hip_set_math: math must be 0 (IEEE) or 1 (fast)");
         return SWE_HIP_EINVAL;
@@ -1027,6 +1328,8 @@ int swe_hip_set_math(swe_hip_block* k, int math) {
 
 int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (steps < 2 || steps > 10000 || k->flavour != FLAVOUR_U) {
         set_error("swe_hip_autotune: needs 2 <= steps <= 10000 and flavour U");
         return SWE_HIP_EINVAL;
@@ -1099,6 +1402,7 @@ int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
 
 int swe_hip_get_tuning(swe_hip_block* k, int* strategy, int* xseg) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
     if (strategy) *strategy = k->strategy;
     if (xseg) *xseg = k->xseg;
     return SWE_HIP_OK;
@@ -1106,6 +1410,8 @@ int swe_hip_get_tuning(swe_hip_block* k, int* strategy, int* xseg) {
 
 int swe_hip_set_surrogate(swe_hip_block* k, swe_hip_mlp* m) {
     if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = touch(k)) return r;
     if (m && (k->solver != SOLVER_AUGRIE || k->flavour != FLAVOUR_U)) {
         set_error("swe_hip_set_surrogate: the learned middle state serves flavour-U AugRie blocks only");
         return SWE_HIP_EINVAL;

@@ -49,11 +49,16 @@ int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, Dev
 int launch_apply_bc(StateView v, BoundaryInfo bi, hipStream_t stream);
 int launch_apply_bc_bathymetry(float* b, int nx, int ny, int pitch, BoundaryInfo bi, hipStream_t stream);
 int launch_ctl_reset(DeviceCtl* ctl, int par, hipStream_t stream);
+// Words of the host ring the readback kernel writes (slot seq % kReadbackRing).
+constexpr unsigned kReadbackRing = 4;
+int launch_readback(const DeviceCtl* ctl, int slot, unsigned seq, hipStream_t stream);
 // Quiet-tile plan of fused launch `step` at strip width xseg (a multiple of
 // kQuietCols; kernels_fused.hpp): skipped tiles carry their speeds, the busy
-// runs are listed for the launch.
+// runs are listed for the launch.  slotPrev is the CFL slot of the previous
+// launch (its busy-tile count sets the piece length), slotOut the launch's;
+// rbSeq != 0 also reports slot slotPrev to the host (launch_readback).
 int launch_quiet_plan(const unsigned* quiet, float* speed, int nx, int ny, int xseg, int step, DeviceCtl* ctl,
-                      int slotOut, unsigned* list, unsigned* count, hipStream_t stream);
+                      int slotPrev, int slotOut, unsigned* list, unsigned* count, unsigned rbSeq, hipStream_t stream);
 
 // Flavour S.
 int launch_sweep_s_x(int solver, StateView v, SumsView sums, DeviceCtl* ctl, int par, hipStream_t stream);

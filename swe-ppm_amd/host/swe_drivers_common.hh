@@ -7,6 +7,7 @@
 
 #include <getopt.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "../../include/swe_hip.h"
+#include "types/Boundary.hh"
 
 struct DriverArgs {
     float duration = 1.f;    // -t
@@ -36,6 +38,8 @@ struct DriverArgs {
     std::string bathymetry;    // --bathymetry-file: file-driven scenario
     std::string displacement;  // --displacement-file
     bool hostInit = false;     // --host-init: sample the scenario on the host (initScenario), not the device
+    long warmup = 0;           // --warmup: untimed steps, after which the initial state is loaded again
+    int lookahead = 1;         // --lookahead 0: no look-ahead launches (swe_hip_set_lookahead)
 };
 
 inline int solver_from_name(const std::string& s) {
@@ -68,6 +72,8 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
                             {"bathymetry-file", required_argument, nullptr, 9},
                             {"displacement-file", required_argument, nullptr, 10},
                             {"host-init", no_argument, nullptr, 11},
+                            {"warmup", required_argument, nullptr, 12},
+                            {"lookahead", required_argument, nullptr, 13},
                             {nullptr, 0, nullptr, 0}};
     int c;
     while ((c = getopt_long(argc, argv, "t:n:x:y:o:b:l:w:", opts, nullptr)) != -1) {
@@ -130,6 +136,12 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
             case 11:
                 a.hostInit = true;
                 break;
+            case 12:
+                a.warmup = std::atol(optarg);
+                break;
+            case 13:
+                a.lookahead = std::atoi(optarg) != 0;
+                break;
             default:
                 throw std::invalid_argument("bad command line");
         }
@@ -187,6 +199,19 @@ inline std::vector<BlockState> read_snapshot(const std::string& path) {
     }
     std::fclose(f);
     return out;
+}
+
+// A snapshot's state into a block's host arrays, the boundary types set,
+// then uploaded (the block's initScenario without a scenario object).
+template <typename Block>
+void load_snapshot(Block& blk, const BlockState& s, BoundaryType bnd[4]) {
+    if (s.nx != blk.nx || s.ny != blk.ny) throw std::runtime_error("snapshot size differs from the block's");
+    std::copy(s.h.begin(), s.h.end(), blk.h.getRawPointer());
+    std::copy(s.hu.begin(), s.hu.end(), blk.hu.getRawPointer());
+    std::copy(s.hv.begin(), s.hv.end(), blk.hv.getRawPointer());
+    std::copy(s.b.begin(), s.b.end(), blk.b.getRawPointer());
+    for (int i = 0; i < 4; i++) blk.setBoundaryType(static_cast<Boundary>(i), bnd[i]);
+    blk.synchAfterWrite();
 }
 
 template <typename Block>

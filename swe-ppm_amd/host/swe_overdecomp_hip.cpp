@@ -51,19 +51,6 @@ std::array<int, 4> get_neighbours(int px, int py, int cols, int rows, int k) {
 }
 
 // initial state copied from a snapshot instead of a scenario
-struct SnapshotScenarioLoader {
-    static void load(SWE_DimensionalSplittingHip& blk, const BlockState& s, BoundaryType bnd[4]) {
-        const size_t m = (size_t)(s.nx + 2) * (s.ny + 2);
-        std::copy(s.h.begin(), s.h.end(), blk.h.getRawPointer());
-        std::copy(s.hu.begin(), s.hu.end(), blk.hu.getRawPointer());
-        std::copy(s.hv.begin(), s.hv.end(), blk.hv.getRawPointer());
-        std::copy(s.b.begin(), s.b.end(), blk.b.getRawPointer());
-        (void)m;
-        for (int i = 0; i < 4; i++) blk.setBoundaryType(static_cast<Boundary>(i), bnd[i]);
-        blk.synchAfterWrite();
-    }
-};
-
 void batch_check(int rc, const char* what) {
     if (rc != SWE_HIP_OK) throw std::runtime_error(std::string(what) + ": " + swe_hip_last_error());
 }
@@ -246,7 +233,7 @@ int main(int argc, char** argv) {
             else if (init.empty())
                 blocks[k]->initScenario(*scenario, bnd);
             else
-                SnapshotScenarioLoader::load(*blocks[k], init[k], bnd);
+                load_snapshot(*blocks[k], init[k], bnd);
             blocks[k]->connectLocalNeighbours(nbb);
             blocks[k]->setRank(k);
             blocks[k]->setDuration(a.duration);

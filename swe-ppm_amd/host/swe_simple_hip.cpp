@@ -5,11 +5,15 @@
 //
 //   swe_simple_hip -t 25 -n 5 -x 256 -y 256 -o out [-w 1] [--writer vtk|netcdf]
 //                  [--solver fwave] [--flavour u|s] [--scenario radial|centred|...]
-//                  [-b bathymetry.nc [-d displacement.nc] [--host-init]] [--dump f]
+//                  [-b bathymetry.nc [-d displacement.nc] [--host-init]] [--input f] [--dump f]
 //
 // -b / -d (--bathymetry-file / --displacement-file) select the file-driven
 // scenario (the reference's ASAGI build, swe_simple.cpp:68-71, 109-113),
 // sampled on the device unless --host-init asks for the host loop.
+// --input starts from a state snapshot of one block instead (sizes from the
+// snapshot, domain and boundary types from the scenario).  --warmup K runs K
+// untimed steps first and starts again from the initial state; --lookahead 0
+// turns off the look-ahead launches of the per-call API.
 //
 // With -w 1 the state is written at t = 0 and at every checkpoint: VTK files
 // <out>.<k>.vts (the default) or one netCDF file <out>.nc, the two writers
@@ -56,6 +60,15 @@ int main(int argc, char** argv) {
             args.push_back(&w[0]);
         }
         DriverArgs a = parse_args((int)args.size(), args.data(), "radial");
+        // --input: the initial state from a snapshot (one block) in the
+        // scenario's domain and with its boundary types
+        std::vector<BlockState> init;
+        if (!a.input.empty()) {
+            init = read_snapshot(a.input);
+            if (init.size() != 1) throw std::invalid_argument("--input needs a snapshot of one block");
+            a.nx = init[0].nx;
+            a.ny = init[0].ny;
+        }
         std::unique_ptr<SWE_Scenario> scenario;
         SWE_NetCdfScenario* fileScenario = nullptr;
         int scenarioId = -1;
@@ -77,13 +90,28 @@ int main(int argc, char** argv) {
         for (int s = 0; s < 4; s++) boundaries[s] = scenario->getBoundaryType(static_cast<Boundary>(s));
 
         SWE_DimensionalSplittingHip sim(a.nx, a.ny, dx, dy, ox, oy, false, a.solver, a.flavour);
+        if (!a.lookahead && swe_hip_set_lookahead(sim.handle(), 0) != SWE_HIP_OK)
+            throw std::runtime_error(swe_hip_last_error());
         // sampled on the device unless --host-init asks for the host loop
-        if (fileScenario && !a.hostInit)
-            sim.initScenarioOnDevice(*fileScenario, boundaries);
-        else if (scenarioId >= 0 && !a.hostInit)
-            sim.initScenarioOnDevice(scenarioId, boundaries);
-        else
-            sim.initScenario(*scenario, boundaries);
+        auto initialise = [&]() {
+            if (!init.empty())
+                load_snapshot(sim, init[0], boundaries);
+            else if (fileScenario && !a.hostInit)
+                sim.initScenarioOnDevice(*fileScenario, boundaries);
+            else if (scenarioId >= 0 && !a.hostInit)
+                sim.initScenarioOnDevice(scenarioId, boundaries);
+            else
+                sim.initScenario(*scenario, boundaries);
+        };
+        initialise();
+        if (a.warmup > 0) {  // untimed steps (first launches load the kernels), then the initial state again
+            for (long s = 0; s < a.warmup; s++) {
+                sim.setGhostLayer();
+                sim.computeNumericalFluxes();
+                sim.updateUnknowns(sim.getMaxTimestep());
+            }
+            initialise();
+        }
         if (!a.dump0.empty()) write_snapshot(a.dump0, {snapshot_of(sim)});
 
         std::unique_ptr<Writer> writer;
@@ -100,14 +128,12 @@ int main(int argc, char** argv) {
         double wall = 0.0;
         bool stop = false;
         for (int i = 0; i < a.checkpoints && !stop; i++) {
+            auto t0 = std::chrono::steady_clock::now();
             while (t < cp[i]) {
-                auto t0 = std::chrono::steady_clock::now();
                 sim.setGhostLayer();
                 sim.computeNumericalFluxes();
                 const float timestep = sim.getMaxTimestep();
                 sim.updateUnknowns(timestep);
-                swe_hip_synchronize(sim.handle());
-                wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 t += timestep;
                 iterations++;
                 if (a.max_steps > 0 && iterations >= a.max_steps) {
@@ -115,6 +141,9 @@ int main(int argc, char** argv) {
                     break;
                 }
             }
+            // the last update is still queued: the clock stops once it is done
+            swe_hip_synchronize(sim.handle());
+            wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             std::printf("Write timestep (%fs)\n", t);
             if (writer)
                 writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(), t);

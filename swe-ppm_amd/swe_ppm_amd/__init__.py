@@ -76,6 +76,7 @@ def _load() -> C.CDLL:
         "swe_hip_solve_edges": (C.c_int, [C.c_int, C.c_int, C.c_int64] + [fp] * 11),
         "swe_hip_set_math": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_quiet_skip": (C.c_int, [vp, C.c_int]),
+        "swe_hip_set_lookahead": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
         "swe_hip_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "swe_hip_batch_create": (C.c_int, [C.c_int, ip, ip, fp, fp, ip, ip, C.c_int, C.c_int, C.c_float, C.c_float,
@@ -253,6 +254,11 @@ class SweBlock:
         """Skip quiet strip tiles in the fused device loop (default on);
         results are identical either way."""
         _check(lib.swe_hip_set_quiet_skip(self._h, int(bool(enable))), "set_quiet_skip")
+
+    def set_lookahead(self, enable: bool) -> None:
+        """Queue the next per-call step while the host runs its loop (default
+        on); results are identical either way."""
+        _check(lib.swe_hip_set_lookahead(self._h, int(bool(enable))), "set_lookahead")
 
     def set_surrogate(self, mlp) -> None:
         """Attach a MiddleStateMLP (None detaches): AugRie then starts its

@@ -110,6 +110,97 @@ def test_per_call_api_matches_oracle_step_by_step(swe, oracle):
                 assert same_bits(a, c), f"step {step}"
 
 
+@pytest.mark.parametrize("solver,xseg", [(2, 16), (2, 4), (1, 12), (0, 8)])
+def test_per_call_steps_without_downloads_match_oracle(swe, oracle, solver, xseg):
+    """The per-call loop with no host transfer between steps: the update is
+    left pending and folded into the next flux pass, ghosts included."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=150, seed=4)
+    bnd = [1, 0, 0, 1]
+    scenarios.apply_bc(h, hu, hv, bnd)
+    nsteps = 23
+    with swe.SweBlock(150, 150, dx, dy, solver=solver) as blk:
+        blk.set_boundaries(bnd)
+        blk.set_tuning(-1, xseg)
+        blk.upload(h, hu, hv, b)
+        dts = []
+        for _ in range(nsteps):
+            blk.apply_bc()
+            dt = blk.fluxes()
+            blk.update(dt)
+            dts.append(dt)
+        gh, ghu, ghv = blk.download()
+    oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, dx, dy, bnd, solver, nsteps)
+    assert same_bits(np.array(dts, np.float32), odts)
+    for a, c in ((gh, oh), (ghu, ohu), (ghv, ohv)):
+        assert same_bits(a, c)
+
+
+def test_per_call_orders_equal_eager_updates(swe):
+    """Runs of the drivers' order (setGhostLayer -> fluxes -> update) broken by
+    other orders (fluxes without a boundary refresh, repeated setGhostLayer,
+    bathymetry ghosts, a boundary type changed mid-run, a run() between
+    per-call steps, an update with a smaller dt than returned, a download)
+    give the same bits whether each update is forced out at once, left
+    pending, or left pending with the next step queued ahead."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=100, seed=8)
+    in_order = ["bc", "f", "u"] * 8
+    script = (in_order + ["f", "u", "f", "u", "bc", "bc", "f", "u", "bcb", "bc", "f", "u", "wall", "bc", "f", "u"]
+              + in_order + ["run", "f", "u", "bc", "f", "half", "bc", "f", "u", "bc", "f", "u", "bc", "f", "u",
+                            "bc", "f", "u", "get"] + in_order + ["bc", "f", "u", "f", "bc", "u", "bc", "f", "u"])
+    outs = []
+    for mode in ("lookahead", "lazy", "eager"):
+        with swe.SweBlock(100, 100, dx, dy, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            blk.set_tuning(-1, 8)
+            blk.set_lookahead(mode == "lookahead")
+            blk.upload(h, hu, hv, b)
+            dts, mid = [], None
+            for op in script:
+                if op == "bc":
+                    blk.apply_bc()
+                elif op == "bcb":
+                    blk.apply_bc_bathymetry()
+                elif op == "f":
+                    dts.append(blk.fluxes())
+                elif op in ("u", "half"):
+                    blk.update(dts[-1] if op == "u" else dts[-1] * 0.5)
+                    if mode == "eager":
+                        blk.download()
+                elif op == "wall":
+                    blk.set_boundaries([1, 1, 0, 0])
+                elif op == "run":
+                    _, rd = blk.run(3)
+                    dts.extend(rd.tolist())
+                elif op == "get":
+                    mid = blk.download()
+            outs.append((blk.download(with_b=True), mid, np.array(dts, np.float32)))
+    a, ma, da = outs[2]
+    for c, mc, dc in outs[:2]:
+        assert same_bits(da, dc)
+        for x, y in zip(a + ma, c + mc):
+            assert same_bits(x, y)
+
+
+def test_update_needs_a_fresh_flux_pass(swe):
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_c1(32)
+    with swe.SweBlock(32, 32, dx, dy, solver=swe.FWAVE) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        with pytest.raises(RuntimeError):
+            blk.update(0.1)
+        blk.apply_bc()
+        dt = blk.fluxes()
+        blk.update(dt)
+        with pytest.raises(RuntimeError):
+            blk.update(dt)
+
+
 def test_run_ghosts_match_reference_sequence(swe, oracle):
     """After run(), ghost cells hold the boundary values set at the start of
     the last step, as after the reference's loop."""
