@@ -92,32 +92,92 @@ def timed_run_from(swe, blk, nsteps, t0):
     return tt.value, total.value, None, None
 
 
-def cpu_baseline(h, hu, hv, b, dx, dy, bnd, solver, budget_s: float):
-    """Oracle steps on all host threads until the time budget is used."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def _tiles(state, bnd, cols, rows):
+    """The state cut into cols x rows connected blocks (ghosts from the
+    neighbours' interiors or the domain's ghosts) for the overdecomposed
+    driver: block k = c * rows + r."""
+    h, hu, hv, b = state
+    nx, ny = h.shape[0] - 2, h.shape[1] - 2
+    bw, bh = nx // cols, ny // rows
+    out = []
+    for c in range(cols):
+        for r in range(rows):
+            x0, x1 = c * bw, (c + 1) * bw if c < cols - 1 else nx
+            y0, y1 = r * bh, (r + 1) * bh if r < rows - 1 else ny
+            k = c * rows + r
+            nbr = [k - rows if c > 0 else -1, k + rows if c < cols - 1 else -1, k - 1 if r > 0 else -1,
+                   k + 1 if r < rows - 1 else -1]
+            out.append(dict(nx=x1 - x0, ny=y1 - y0, nbr=nbr, bnd=[4 if m >= 0 else t for m, t in zip(nbr, bnd)],
+                            **{key: np.ascontiguousarray(a[x0:x1 + 2, y0:y1 + 2]) for key, a in
+                               zip(("h", "hu", "hv", "b"), (h, hu, hv, b))}))
+    return out
+
+
+def cpu_baseline(h, hu, hv, b, dx, dy, bnd, solver, budget_s: float, reps: int = 5):
+    """The CPU oracle on the host's cores, as the baseline protocol asks: 5
+    warm-up steps from t=0, then `reps` timed repetitions of the same K steps
+    (monotonic clock), median; two modes -- one block with OpenMP over rows
+    (swe_simple's structure) and overdecomposed, one connected block per
+    thread with global dt (swe_mpi_overdecomp's) -- and two builds of the same
+    source: the IEEE parity build (-O2, no contraction) and the reference's
+    Release flags (-O3 -ffast-math, timing only).  K is sized so that the
+    whole baseline takes about budget_s.  value: the fastest mode of the
+    Release build."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_py
 
     threads = min(cpu_threads(), 64)
     nx, ny = h.shape[0] - 2, h.shape[1] - 2
-    steps, elapsed = 0, 0.0
-    state = (h, hu, hv)
-    # first step alone to size the sample
-    t0 = time.perf_counter()
-    state = oracle_py.run_u(*state, b, dx, dy, bnd, solver, 1, nthreads=threads)[:3]
-    elapsed += time.perf_counter() - t0
-    steps += 1
-    per = elapsed
-    more = max(0, int(budget_s / max(per, 1e-6)) - 1)
-    more = min(more, 200)
-    if more:
+    rows = max(d for d in range(1, int(threads ** 0.5) + 1) if threads % d == 0)
+    cols = threads // rows
+    # 5 warm-up steps from t=0 (the state every repetition starts from)
+    w = oracle_py.run_u(h, hu, hv, b, dx, dy, bnd, solver, 5, nthreads=threads)
+    state = (w[0], w[1], w[2], b)
+    tiles = _tiles(state, bnd, cols, rows)
+
+    def single(release, k):
         t0 = time.perf_counter()
-        oracle_py.run_u(*state, b, dx, dy, bnd, solver, more, nthreads=threads)
-        elapsed += time.perf_counter() - t0
-        steps += more
-    mcells = nx * ny * steps / elapsed / 1e6
-    return {"value": mcells, "unit": "MCell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} flavour-U AugRie steps of the same {nx}x{ny} workload from t=0 "
-                      f"({elapsed:.1f} s, OpenMP {threads} threads, C++ oracle -O2)"}
+        oracle_py.run_u(*state, dx, dy, bnd, solver, k, nthreads=threads, release=release)
+        return time.perf_counter() - t0
+
+    def overdecomposed(release, k):
+        t0 = time.perf_counter()
+        oracle_py.multiblock_run(tiles, dx, dy, solver, False, 1.0, 1e30, k, nthreads=threads, release=release)
+        return time.perf_counter() - t0
+
+    modes = {}
+    per_sample = budget_s / (4 * reps + 4)
+    for mode, fn in (("single_block", single), ("overdecomposed", overdecomposed)):
+        modes[mode] = {}
+        for build, release in (("release", True), ("ieee", False)):
+            one = fn(release, 1)
+            k = int(min(200, max(2, per_sample / max(one, 1e-6))))
+            times = sorted(fn(release, k) for _ in range(reps))
+            med = times[len(times) // 2]
+            modes[mode][build] = {"value": round(nx * ny * k / med / 1e6, 2), "steps": k,
+                                  "median_s": round(med, 4), "spread": round(times[-1] / times[0] - 1.0, 4)}
+    best_mode = max(modes, key=lambda m: modes[m]["release"]["value"])
+    best = modes[best_mode]["release"]
+    return {"value": best["value"], "unit": "MCell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"median of {reps} repetitions of {best['steps']} flavour-U AugRie steps of the same {nx}x{ny} "
+                      f"workload after 5 warm-up steps from t=0; C++ oracle built with the reference's Release flags, "
+                      f"OpenMP {threads} threads, {best_mode.replace('_', ' ')} mode (fastest of the two)",
+            "cpu_model": cpu_model(),
+            "modes": modes,
+            "overdecomposed_blocks": f"{cols} x {rows}"}
 
 
 def batched_lts(swe, scenarios, autotune: bool, seed: int = 28, n: int = 2048, T: float = 1.0,
@@ -252,16 +312,17 @@ def surrogate_compare(swe, state, weights, steps):
             "weights": str(Path(weights).name)}
 
 
-def solver_lines(swe, state, steps: int = 200, autotune: bool = True):
-    """The headline workload with each solver of the plain path (fused
-    strategy, IEEE): MCell-updates/s and the algorithmic HBM fraction, to show
-    where the same kernel stands when the Riemann solver costs less.  Each
-    block's strip width is autotuned on the workload's first 20 steps (the
-    state is restored before the timed run)."""
+def solver_lines(swe, state, steps: int = 200, autotune: bool = True, names=("hlle", "fwave", "hybrid", "augrie")):
+    """The headline workload with each solver of the plain path (IEEE):
+    MCell-updates/s and the algorithmic HBM fraction, to show where the same
+    step stands when the Riemann solver costs less.  Each block's strip width
+    (and, for HLLE and f-wave, the strategy) is autotuned on the workload's
+    first 20 steps (the state is restored before the timed run)."""
     h, hu, hv, b, dx, dy, bnd = state
     n = h.shape[0] - 2
     out = {}
-    for name, sid in (("hlle", swe.HLLE), ("fwave", swe.FWAVE), ("hybrid", swe.HYBRID), ("augrie", swe.AUGRIE)):
+    for name in names:
+        sid = swe.SOLVER_NAMES[name]
         with swe.SweBlock(n, n, dx, dy, solver=sid) as blk:
             blk.set_boundaries(bnd)
             blk.upload(h, hu, hv, b)
@@ -270,14 +331,14 @@ def solver_lines(swe, state, steps: int = 200, autotune: bool = True):
             blk.run(10, want_dts=False)  # warm-up
             blk.upload(h, hu, hv, b)
             _, total_ms, _, _ = timed_run(swe, blk, steps, per_kernel=False)
-            xseg = blk.get_tuning()[1]
+            strategy, xseg = blk.get_tuning()
         v = n * n * steps / (total_ms * 1e-3) / 1e6
         out[name] = {"value": round(v, 1), "ms_per_step": round(total_ms / steps, 5),
                      "hbm_frac_algorithmic": round(BYTES_PER_CELL * v * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
-                     "xseg": xseg, "default_xseg": default,
+                     "strategy": ["recompute", "store", "fused"][strategy], "xseg": xseg, "default_xseg": default,
                      "autotune_ms_per_step": None if tune is None else {k: round(t, 5) for k, t in tune.items()}}
-    return {"workload": f"{n}x{n} islands dam break (the headline workload), {steps} steps per solver, strip "
-                        f"width {'autotuned' if autotune else 'default'}",
+    return {"workload": f"{n}x{n} islands dam break, {steps} steps per solver, strip width "
+                        f"{'and strategy autotuned' if autotune else 'default'}",
             "unit": "MCell-updates/s", "solvers": out}
 
 
@@ -364,36 +425,40 @@ def disturbed_line(swe, scenarios, n: int, seed: int, steps: int, noise: float =
             "hbm_frac_algorithmic": round(BYTES_PER_CELL * n * n / (fused_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 5)}
 
 
-def batched_surrogate_ensemble(swe, scenarios, weights, plain_value: float, count: int = 512, seed: int = 512,
-                               steps: int = 100, autotune: bool = True):
-    """The 512-instance ensemble through the batched engine with the learned
-    middle state attached (same instances as the exact ensemble line, tuned
-    the same way), against the plain 2048^2 line."""
+def batched_surrogate_ensemble(swe, scenarios, weights, count: int = 512, seed: int = 512, steps: int = 100,
+                               autotune: bool = True):
+    """The 512-instance ensemble through the batched engine, exact and with
+    the learned middle state attached: the same instances, steps and tuning
+    procedure, so the ratio compares like with like."""
     from swe_ppm_amd import surrogate
 
     inst = scenarios.ensemble_instances(count=count, seed=seed)
     fields = [[i[f] for i in inst] for f in ("h", "hu", "hv", "b")]
-    mlp = surrogate.load_default(path=weights)
-    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
-        bt.set_surrogate(mlp)
-        bt.upload(*fields)
-        if autotune:
-            bt.autotune(iterations=16)
-        bt.run(5)  # warm-up
-        bt.upload(*fields)
-        _, _, ms = bt.run(steps)
-        h = bt.download(packed=True)[0]
-        tuning = bt.get_tuning()
-    mlp.close()
-    if not np.all(np.isfinite(h)):
-        raise RuntimeError("non-finite ensemble state (learned middle state)")
     updates = int(sum(i["nx"] * i["ny"] for i in inst)) * steps
-    value = updates / (ms * 1e-3) / 1e6
-    return {"workload": f"the {count}-instance ensemble (seed {seed}), AugRie FP32 with the learned middle state, "
-                        f"{steps} steps each",
-            "value": round(value, 3), "unit": "MCell-updates/s", "elapsed_ms": round(ms, 3),
-            "vs_plain": round(value / plain_value, 4), "weights": str(Path(weights).name),
-            "tuning": dict(zip(("xs", "groups", "chunk", "graph", "fused"), tuning))}
+    out = {"workload": f"the {count}-instance ensemble (seed {seed}), AugRie FP32, {steps} steps each, exact and "
+                       "with the learned middle state", "unit": "MCell-updates/s", "weights": str(Path(weights).name)}
+    for key in ("exact", "learned"):
+        mlp = surrogate.load_default(path=weights) if key == "learned" else None
+        with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+            if mlp is not None:
+                bt.set_surrogate(mlp)
+            bt.upload(*fields)
+            if autotune:
+                bt.autotune(iterations=16)
+            bt.upload(*fields)
+            bt.run(5)  # warm-up
+            bt.upload(*fields)
+            _, _, ms = bt.run(steps)
+            h = bt.download(packed=True)[0]
+            tuning = bt.get_tuning()
+        if mlp is not None:
+            mlp.close()
+        if not np.all(np.isfinite(h)):
+            raise RuntimeError(f"non-finite ensemble state ({key})")
+        out[key] = {"value": round(updates / (ms * 1e-3) / 1e6, 3), "elapsed_ms": round(ms, 3),
+                    "tuning": dict(zip(("xs", "groups", "chunk", "graph", "fused"), tuning))}
+    out["learned_vs_exact"] = round(out["learned"]["value"] / out["exact"]["value"], 4)
+    return out
 
 
 def surrogate_trench(swe, scenarios, weights, n: int = 4096, steps: int = 1000, seed: int = 4096):
@@ -420,7 +485,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2048)
     ap.add_argument("--strategy", type=int, default=-1)
     ap.add_argument("--xseg", type=int, default=-1)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--math", type=int, default=0, help="0 IEEE (bit-compatible), 1 fast")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched LTS / ensemble lines")
@@ -435,6 +500,10 @@ def main() -> int:
                     help="trained network (.npy); default: the packaged weights")
     ap.add_argument("--trench-n", type=int, default=4096)
     ap.add_argument("--trench-steps", type=int, default=1000)
+    ap.add_argument("--only-headline", action="store_true",
+                    help="print the main line only (no sub-lines): the command the profiles are taken with")
+    ap.add_argument("--solver", choices=["augrie", "hlle", "fwave", "hybrid"], default="augrie",
+                    help="solver of the main line (profiling of the light solvers; the metric is AugRie's)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -466,8 +535,7 @@ def main() -> int:
     if args.batched_only == "surrogate":
         trench = surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps)
         islands = surrogate_islands(swe, scenarios, weights, steps=args.steps)
-        ens = batched_surrogate_ensemble(swe, scenarios, weights, islands["plain"]["value"],
-                                         autotune=not args.no_autotune)
+        ens = batched_surrogate_ensemble(swe, scenarios, weights, autotune=not args.no_autotune)
         print(json.dumps({"trench": trench, "islands": islands, "batched_ensemble_512": ens}))
         return 0
     if args.batched_only:
@@ -480,7 +548,8 @@ def main() -> int:
 
     n = args.n
     h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=n, seed=args.seed)
-    blk = swe.SweBlock(n, n, dx, dy, solver=swe.AUGRIE)
+    solver = {"augrie": swe.AUGRIE, "hlle": swe.HLLE, "fwave": swe.FWAVE, "hybrid": swe.HYBRID}[args.solver]
+    blk = swe.SweBlock(n, n, dx, dy, solver=solver)
     blk.set_boundaries(bnd)
     blk.set_tuning(args.strategy, args.xseg)
     blk.set_math(args.math)
@@ -499,12 +568,15 @@ def main() -> int:
     cells = n * n
     mcells = cells * args.steps / (total_ms * 1e-3) / 1e6
 
-    # per-kernel attribution (separate run with events between launches)
+    # per-kernel attribution (separate run with events between launches):
+    # class 0 boundary + quiet-tile plans, 1 the fused (or first) pass, 2 the
+    # final apply (or second pass)
     blk.upload(h, hu, hv, b)
     _, total2, kms, kl = timed_run(swe, blk, args.steps, per_kernel=True)
     pass1_ms = kms[1] / max(kl[1], 1)
     pass2_ms = kms[2] / max(kl[2], 1)
-    # kernel time per step: every launch of the loop (boundary, passes) / steps
+    plans = max(kl[0] - 1, 0)  # one boundary launch, the rest are plans
+    # kernel time per step: every launch of the loop (boundary, plans, passes) / steps
     step_ms_k = (kms[0] + kms[1] + kms[2]) / args.steps
     # the dominant kernel's own average launch time (the fused launch for the fused strategy)
     achieved = BYTES_PER_CELL * cells / (pass1_ms * 1e-3) / 1e9
@@ -518,8 +590,9 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, args.cpu_budget)
 
-    traffic = measured_traffic(xseg) if (strategy == 2 and n == 2048 and args.math == 0) else None
-    valu = measured_valu(xseg) if (strategy == 2 and n == 2048 and args.math == 0) else None
+    profiled = strategy == 2 and n == 2048 and args.math == 0 and args.solver == "augrie"
+    traffic = measured_traffic(xseg) if profiled else None
+    valu = measured_valu(xseg) if profiled else None
     cus = 256
     try:
         import torch
@@ -537,10 +610,12 @@ def main() -> int:
                      "source": "rocprofv3 SQ_INSTS_VALU of fused_u_kernel on this workload "
                                "(profiles/fused_u_augrie_2048_islands.md); peak = CUs x 4 SIMDs x 1/4 "
                                "wave64 instruction per cycle x 2.4 GHz"}
-    names = {0: ("sweep_u_kernel<AUGRIE,MODE_MAX>", "sweep_u_kernel<AUGRIE,MODE_UPDATE>"),
-             1: ("sweep_u_kernel<AUGRIE,MODE_STORE>", "apply_u_kernel"),
-             2: ("fused_u_kernel<AUGRIE> (first launch: sweep_u_kernel<AUGRIE,MODE_STORE>)",
+    sname = args.solver.upper() if args.solver != "augrie" else "AUGRIE"
+    names = {0: (f"sweep_u_kernel<{sname},MODE_MAX>", f"sweep_u_kernel<{sname},MODE_UPDATE>"),
+             1: (f"sweep_u_kernel<{sname},MODE_STORE>", "apply_u_kernel"),
+             2: (f"fused_u_kernel<{sname}> (first launch: sweep_u_kernel<{sname},MODE_STORE>)",
                  "apply_u_kernel (once, after the last step)")}[strategy]
+    solver_label = {"augrie": "AugRie", "hlle": "HLLE", "fwave": "f-wave", "hybrid": "Hybrid"}[args.solver]
     line = {
         "metric": METRIC,
         "value": round(mcells, 3),
@@ -555,7 +630,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{n}x{n} flavour-U AugRie FP32 global-time-stepping step: radial dam break "
+            "workload": f"{n}x{n} flavour-U {solver_label} FP32 global-time-stepping step: radial dam break "
                         f"(15/10 m) + 16 seeded Gaussian bumps with 3 dry islands (seed {args.seed}), "
                         f"outflow boundaries, {args.steps} steps from t=0",
             "cells": cells,
@@ -565,6 +640,9 @@ def main() -> int:
             "wall_s_incl_sync": round(wall_s, 4),
         },
         "roofline": {
+            # the contract's roofline: algorithmic bytes over the dominant
+            # kernel's time, against HBM peak; what binds that kernel is
+            # vector-ALU issue ("binding", "valu")
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -574,13 +652,18 @@ def main() -> int:
             "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of fused_u_kernel on this workload "
                               "(profiles/fused_u_augrie_2048_islands.md)" if traffic else None,
             "kernel": f"{names[0]}: {kl[1]} launches, avg {pass1_ms * 1e3:.2f} us; {names[1]}: {kl[2]} launches, "
-                      f"avg {pass2_ms * 1e3:.2f} us; boundary {kms[0] * 1e3:.1f} us total; "
-                      f"{step_ms_k * 1e3:.2f} us of kernel time per step",
+                      f"avg {pass2_ms * 1e3:.2f} us; boundary launch, {plans} quiet_plan_kernel launches and the gaps "
+                      f"between launches: {kms[0] * 1e3:.1f} us; {step_ms_k * 1e3:.2f} us per step in this "
+                      f"attribution run (fused launches timed on their own dispatches)",
             "algorithmic_bytes_per_cell_update": BYTES_PER_CELL,
+            "binding": "valu" if args.solver == "augrie" else None,
             "valu": valu_line,
         },
         "cpu_baseline": cpu,
     }
+    if args.only_headline:
+        print(json.dumps(line))
+        return 0
     line["every_tile"] = every_tile_line(swe, (h, hu, hv, b, dx, dy, bnd), args.steps, xseg)
     line["every_tile"]["vs_headline"] = round(line["every_tile"]["value"] / mcells, 4)
     line["per_call"] = per_call_line((h, hu, hv, b, dx, dy, bnd), args.steps, args.warmup)
@@ -590,6 +673,10 @@ def main() -> int:
     line["disturbed"]["vs_headline"] = round(line["disturbed"]["value"] / mcells, 4)
     if not args.no_solvers:
         line["solvers"] = solver_lines(swe, (h, hu, hv, b, dx, dy, bnd), autotune=not args.no_autotune)
+        # the light solvers where the state does not fit the GPU's caches
+        big = scenarios.dam_break_islands(n=2 * n, seed=args.seed)
+        line["solvers_hbm_resident"] = solver_lines(swe, big, autotune=not args.no_autotune, names=("hlle", "fwave"))
+        del big
     if not args.no_batched:
         lts = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
         ens = batched_ensemble(swe, scenarios, not args.no_autotune, steps=args.ens_steps)
@@ -601,7 +688,7 @@ def main() -> int:
             line["surrogate"] = {
                 "trench": surrogate_trench(swe, scenarios, weights, n=args.trench_n, steps=args.trench_steps),
                 "islands": surrogate_islands(swe, scenarios, weights, n=n, steps=args.steps, seed=args.seed),
-                "batched_ensemble_512": batched_surrogate_ensemble(swe, scenarios, weights, mcells,
+                "batched_ensemble_512": batched_surrogate_ensemble(swe, scenarios, weights,
                                                                    autotune=not args.no_autotune)}
         else:
             line["surrogate"] = {"skipped": f"no trained network at {weights}"}

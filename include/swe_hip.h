@@ -179,8 +179,9 @@ int swe_hip_run(swe_hip_block* blk, int nsteps, float* t_inout, float* dt_log);
 /* swe_hip_run with HIP-event timing on the block's stream.  *total_ms is
  * the elapsed time of the whole loop.  If kernel_ms is not NULL, events are
  * also recorded between launches and kernel_ms[0..2] receive the summed
- * durations of the boundary, first-pass (flux / max) and second-pass
- * (update / apply) launches; kernel_launches[0..2] their launch counts.
+ * durations of the boundary and quiet-tile plan, first-pass (flux / max or
+ * fused) and second-pass (update / apply) launches; kernel_launches[0..2]
+ * their launch counts.  The events themselves slow the loop a little.
  * No reference counterpart: measurement hook for the bench. */
 int swe_hip_run_timed(swe_hip_block* blk, int nsteps, float* t_inout, float* total_ms, float* kernel_ms,
                       int* kernel_launches);
@@ -248,8 +249,12 @@ int swe_hip_get_tuning(swe_hip_block* blk, int* strategy, int* xseg);
  * state at each of the SWE_HIP_TUNE_WIDTHS widths 4, 8, 10, 12, 16, 32, restores
  * the state (h, hu, hv, the simulated time and step count), and keeps the
  * fastest width as if set with swe_hip_set_tuning.  ms_per_step (may be
- * null) receives the per-step times in that order.  Results of later runs do
- * not depend on the width chosen.  Allocates and frees 3 fields of scratch. */
+ * null) receives the per-step times in that order.  With the fused strategy
+ * and the HLLE or f-wave solver it also times the recompute strategy (0) at
+ * 16 and 32 columns and keeps it if faster: on blocks that do not fit the
+ * GPU's caches recompute moves fewer bytes per cell (swe_hip_get_tuning
+ * reports the strategy kept).  Results of later runs do not depend on the
+ * choice.  Allocates and frees 3 fields of scratch. */
 #define SWE_HIP_TUNE_WIDTHS 6
 int swe_hip_autotune(swe_hip_block* blk, int steps, float* ms_per_step);
 

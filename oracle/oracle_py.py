@@ -65,6 +65,18 @@ def canonical_lib() -> C.CDLL:
     return _canonical
 
 
+_release = None
+
+
+def release_lib() -> C.CDLL:
+    """The same source built with the reference's Release flags (fast math):
+    for the CPU baseline's timing only, never as a parity reference."""
+    global _release
+    if _release is None:
+        _release = _load(HERE / "build" / "liboracle_release.so")
+    return _release
+
+
 def _p(a: np.ndarray):
     if a is None:
         return None
@@ -110,15 +122,17 @@ def apply_bc_bathymetry(bnd, b) -> None:
     lib.oracle_apply_bc_bathymetry_f32(nx, ny, bp, _p(b))
 
 
-def run_u(h, hu, hv, b, dx, dy, bnd, solver, nsteps, nthreads=1, dt_in=None, dtype=np.float32, canonical=False):
+def run_u(h, hu, hv, b, dx, dy, bnd, solver, nsteps, nthreads=1, dt_in=None, dtype=np.float32, canonical=False,
+          release=False):
     """nsteps of flavour U (in place on copies); returns (h, hu, hv, dts).
-    canonical: the restatement with the textbook per-cell forms."""
+    canonical: the restatement with the textbook per-cell forms; release: the
+    Release-flags build (timing only)."""
     h, hu, hv, b = [np.array(a, dtype=dtype, order="C", copy=True) for a in (h, hu, hv, b)]
     nx, ny = h.shape[0] - 2, h.shape[1] - 2
     dts = np.empty(max(nsteps, 1), dtype)
     dti = None if dt_in is None else np.ascontiguousarray(dt_in, dtype=dtype)
     bb, bp = _ints(bnd)
-    L = canonical_lib() if canonical else lib
+    L = canonical_lib() if canonical else release_lib() if release else lib
     f = L.oracle_run_u_f32 if dtype == np.float32 else L.oracle_run_u_f64
     rc = f(nx, ny, dx, dy, bp, solver, nsteps, nthreads, _p(h), _p(hu), _p(hv), _p(b), _p(dti), _p(dts))
     assert rc == 0
@@ -163,10 +177,11 @@ def round_timestep(max_local: float, ts: float):
     return k, r.value
 
 
-def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nthreads=1):
+def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nthreads=1, release=False):
     """Overdecomposed driver.  blocks: list of dicts with nx, ny, bnd[4], nbr[4]
     (neighbour block index or -1) and h, hu, hv, b arrays.  Returns
-    (states, updates, iterations, dts)."""
+    (states, updates, iterations, dts).  release: the Release-flags build
+    (timing only)."""
     n = len(blocks)
     nx, nxp = _ints([bl["nx"] for bl in blocks])
     ny, nyp = _ints([bl["ny"] for bl in blocks])
@@ -176,7 +191,8 @@ def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nth
     upd = np.zeros(n, np.int64)
     its = np.zeros(max(nmacro, 1), np.int32)
     dts = np.zeros(max(nmacro, 1), np.float32)
-    rc = lib.oracle_multiblock_run_f32(n, nxp, nyp, dx, dy, bndp, nbrp, solver, int(lts), max_local, duration,
+    L = release_lib() if release else lib
+    rc = L.oracle_multiblock_run_f32(n, nxp, nyp, dx, dy, bndp, nbrp, solver, int(lts), max_local, duration,
                                        nmacro, nthreads, *[_p(a) for a in cat], _p(upd), _p(its), _p(dts))
     if rc != 0:
         raise RuntimeError("oracle multiblock run hit its iteration cap")

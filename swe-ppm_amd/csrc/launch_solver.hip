@@ -8,6 +8,7 @@
 //   variant 1 (fast): FMA contraction, hardware reciprocal / square root.
 // The variant is also a template parameter of every kernel, so the two
 // instantiations never share a symbol.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels_fused.hpp"
@@ -99,8 +100,15 @@ int launch_fused_u_solver(int xseg, StateView v, FusedIO io, BoundaryInfo bi, De
         // list mode may list up to one piece per record tile
         const int waves = io.list ? nyt * ((v.nx + kQuietCols - 1) / kQuietCols) : nyt * nxs;
         const int groups = (waves + kWavesPerGroup - 1) / kWavesPerGroup;
-        hipLaunchKernelGGL((fused_u_kernel<S, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v, io, bi,
-                           ctl, slotIn, slotOut, slotClear, dtLog, logIdx);
+        if (g_fusedTimer.start) {
+            hipExtLaunchKernelGGL((fused_u_kernel<S, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream,
+                                  g_fusedTimer.start, g_fusedTimer.stop, 0, v, io, bi, ctl, slotIn, slotOut, slotClear,
+                                  dtLog, logIdx);
+            g_fusedTimer = LaunchTimer{};
+        } else {
+            hipLaunchKernelGGL((fused_u_kernel<S, XS, V>), dim3(groups), dim3(64 * kWavesPerGroup), 0, stream, v, io,
+                               bi, ctl, slotIn, slotOut, slotClear, dtLog, logIdx);
+        }
     };
     if (xseg == 4)
         go(std::integral_constant<int, 4>());

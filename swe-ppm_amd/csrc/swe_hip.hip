@@ -106,6 +106,10 @@ struct swe_hip_block {
     std::recursive_mutex mu;
 };
 
+namespace swe {
+thread_local LaunchTimer g_fusedTimer;
+}
+
 namespace {
 
 size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pitch; }
@@ -400,12 +404,18 @@ int ensure_dtlog(swe_hip_block* k, int n) {
 
 // Optional HIP-event instrumentation of the device loop.  Event i is
 // recorded before launch i; cls[i] is the class of the work between event i
-// and i+1 (0 boundary, 1 first pass, 2 second pass).
+// and i+1 (0 boundary or quiet-tile plan, 1 first pass, 2 second pass).
+// With the fused strategy (exact solvers) the fused launches are timed on
+// their own dispatches instead (g_fusedTimer), so that no marker sits
+// between the steps; their time then moves from class 0 to class 1.
 struct LoopTiming {
     bool perKernel = false;
     std::vector<hipEvent_t> events;
     std::vector<int> cls;
     int used = 0;
+    std::vector<hipEvent_t> fusedStart, fusedStop;  // per fused launch
+    int fusedUsed = 0;
+    int plans = 0;  // quiet-tile plan launches
     hipError_t mark(hipStream_t s, int nextClass) {
         if (used >= (int)events.size()) return hipSuccess;
         hipError_t e = hipEventRecord(events[used], s);
@@ -451,6 +461,8 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
         if (int r = launch_sweep_u(eff_solver(k), k->math, MODE_STORE, k->xseg, view(k, k->cur), set[0], nullptr, nullptr,
                                    nullptr, k->bi, k->ctl, Slots{0, 1}, -1.f, 0, nullptr, 0, k->stream))
             return r;
+        const bool extTimed = tm && tm->perKernel && !defer;
+        if (extTimed) HIP_TRY(tm->mark(k->stream, 0));  // the plans from here; fused launches timed on their own
         for (int s = 1; s < nsteps; ++s) {
             const int o = k->cur ^ 1;
             FusedIO io{k->h[k->cur], k->hu[k->cur], k->hv[k->cur], k->b, k->h[o], k->hu[o], k->hv[o],
@@ -463,13 +475,19 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
             const int es = defer ? SOLVER_AUGRIE_DEFER : eff_solver(k);
             const bool record = (es == SOLVER_AUGRIE || es == SOLVER_AUGRIE_DEFER) && k->xseg % kQuietCols == 0;
             if (k->skipQuiet && s >= 3 && record) {
+                if (tm && tm->perKernel && !extTimed) HIP_TRY(tm->mark(k->stream, 0));
+                if (tm) ++tm->plans;
                 if (int r = launch_quiet_plan(k->quiet, k->tspeed, k->nx, k->ny, k->xseg, s, k->ctl, (s + 2) % 3, s % 3,
                                               k->tlist, k->tcount, 0u, k->stream))
                     return r;
                 io.list = k->tlist;
                 io.count = k->tcount;
             }
-            if (tm && tm->perKernel) HIP_TRY(tm->mark(k->stream, 1));
+            if (tm && tm->perKernel && !extTimed) HIP_TRY(tm->mark(k->stream, 1));
+            if (extTimed && tm->fusedUsed < (int)tm->fusedStart.size()) {
+                g_fusedTimer = LaunchTimer{tm->fusedStart[tm->fusedUsed], tm->fusedStop[tm->fusedUsed]};
+                ++tm->fusedUsed;
+            }
             if (defer) {
                 io.deferQ = k->deferQ;
                 if (int r = launch_fused_defer(k->xseg, view(k, k->cur), io, k->bi, k->ctl, (s - 1) % 3, s % 3,
@@ -478,9 +496,11 @@ int run_u_device_loop(swe_hip_block* k, int nsteps, float* t_inout, float* dt_lo
                 if (int r = launch_defer_solve(view(k, o), set[s & 1], k->ctl, s % 3, k->deferQ, k->deferCap,
                                                deferGroups, k->stream))
                     return r;
-            } else if (int r = launch_fused_u(eff_solver(k), k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl,
-                                              (s - 1) % 3, s % 3, (s + 1) % 3, k->dtlog, s - 1, k->stream)) {
-                return r;
+            } else {
+                const int r = launch_fused_u(eff_solver(k), k->math, k->xseg, view(k, k->cur), io, k->bi, k->ctl,
+                                             (s - 1) % 3, s % 3, (s + 1) % 3, k->dtlog, s - 1, k->stream);
+                g_fusedTimer = LaunchTimer{};
+                if (r) return r;
             }
             k->cur = o;
         }
@@ -1019,9 +1039,15 @@ int swe_hip_run_timed(swe_hip_block* k, int nsteps, float* t_inout, float* total
     }
     LoopTiming tm;
     tm.perKernel = kernel_ms != nullptr;
-    const int nev = tm.perKernel ? 2 * nsteps + 3 : 2;
+    const int nev = tm.perKernel ? 3 * nsteps + 3 : 2;
     tm.events.resize(nev);
     for (auto& e : tm.events) HIP_TRY(hipEventCreate(&e));
+    if (tm.perKernel && k->strategy == 2) {
+        tm.fusedStart.resize(nsteps);
+        tm.fusedStop.resize(nsteps);
+        for (auto& e : tm.fusedStart) HIP_TRY(hipEventCreate(&e));
+        for (auto& e : tm.fusedStop) HIP_TRY(hipEventCreate(&e));
+    }
     int rc = run_u_device_loop(k, nsteps, t_inout, nullptr, &tm);
     if (rc == 0) {
         float ms = 0.f;
@@ -1041,6 +1067,19 @@ int swe_hip_run_timed(swe_hip_block* k, int nsteps, float* t_inout, float* total
                 acc[tm.cls[i]] += d;
                 cnt[tm.cls[i]] += 1;
             }
+            if (tm.fusedUsed > 0) {  // the fused launches ran inside the class-0 stretch
+                float fused = 0.f;
+                for (int i = 0; i < tm.fusedUsed && rc == 0; ++i) {
+                    float d = 0.f;
+                    e = hipEventElapsedTime(&d, tm.fusedStart[i], tm.fusedStop[i]);
+                    if (e != hipSuccess) rc = hip_fail(e, "hipEventElapsedTime");
+                    fused += d;
+                }
+                acc[0] -= fused;
+                acc[1] += fused;
+                cnt[0] += tm.plans - 1;
+                cnt[1] += tm.fusedUsed;
+            }
             for (int c = 0; c < 3; ++c) {
                 kernel_ms[c] = acc[c];
                 if (kernel_launches) kernel_launches[c] = cnt[c];
@@ -1048,6 +1087,8 @@ int swe_hip_run_timed(swe_hip_block* k, int nsteps, float* t_inout, float* total
         }
     }
     for (auto& e : tm.events) (void)hipEventDestroy(e);
+    for (auto& e : tm.fusedStart) (void)hipEventDestroy(e);
+    for (auto& e : tm.fusedStop) (void)hipEventDestroy(e);
     return rc;
 }
 
@@ -1380,6 +1421,28 @@ int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
         }
         copy_state(false);
     }
+    // the light solvers on HBM-resident blocks: the recompute strategy moves
+    // 44 B per cell against the fused step's ~70 and can win at wide strips
+    const int oldStrategy = k->strategy;
+    int bestStrategy = oldStrategy;
+    const int es = eff_solver(k);
+    if (oldStrategy == 2 && (es == SOLVER_HLLE || es == SOLVER_FWAVE)) {
+        k->strategy = 0;
+        for (int xs : {16, 32}) {
+            if (e != hipSuccess || rc != 0) break;
+            k->xseg = xs;
+            float total = 0.f;
+            rc = swe_hip_run_timed(k, steps, nullptr, &total, nullptr, nullptr);
+            if (rc) break;
+            if (total / steps < bestMs) {
+                bestMs = total / steps;
+                best = xs;
+                bestStrategy = 0;
+            }
+            copy_state(false);
+        }
+        k->strategy = oldStrategy;
+    }
     // a failed timed run leaves the block half-advanced: put the snapshot back
     if (saved && (rc || e != hipSuccess)) {
         const hipError_t e0 = e;
@@ -1393,9 +1456,11 @@ int swe_hip_autotune(swe_hip_block* k, int steps, float* ms_per_step) {
     k->cur = cur0;
     if (rc || e != hipSuccess) {
         k->xseg = oldXseg;
+        k->strategy = oldStrategy;
         return rc ? rc : hip_fail(e, "swe_hip_autotune");
     }
     k->xseg = best;
+    k->strategy = bestStrategy;
     k->xsegSet = true;
     return SWE_HIP_OK;
 }

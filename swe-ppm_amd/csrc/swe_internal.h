@@ -49,6 +49,14 @@ int launch_apply_u(int variant, StateView v, SumsView sums, BoundaryInfo bi, Dev
 int launch_apply_bc(StateView v, BoundaryInfo bi, hipStream_t stream);
 int launch_apply_bc_bathymetry(float* b, int nx, int ny, int pitch, BoundaryInfo bi, hipStream_t stream);
 int launch_ctl_reset(DeviceCtl* ctl, int par, hipStream_t stream);
+// One-shot timing of the next fused launch made on this thread:
+// launch_fused_u records these events on the kernel's own dispatch
+// (hipExtLaunchKernelGGL) and clears them.  Event markers between launches
+// would make the stream pause; these do not.
+struct LaunchTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTimer g_fusedTimer;
 // Words of the host ring the readback kernel writes (slot seq % kReadbackRing).
 constexpr unsigned kReadbackRing = 4;
 int launch_readback(const DeviceCtl* ctl, int slot, unsigned seq, hipStream_t stream);

@@ -292,6 +292,31 @@ def test_autotune_restores_the_state_and_keeps_the_results(swe):
             blk.autotune(1)
 
 
+@pytest.mark.parametrize("solver", [0, 1])
+def test_autotune_may_keep_recompute_for_light_solvers(swe, solver):
+    """For HLLE and f-wave the autotuner also times the recompute strategy at
+    16 and 32 columns; whatever it keeps, a run afterwards gives the bits of
+    an untuned block."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=700, seed=3)
+    with swe.SweBlock(700, 700, dx, dy, solver=solver) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        t_ref, d_ref = blk.run(25)
+        ref = blk.download()
+    with swe.SweBlock(700, 700, dx, dy, solver=solver) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        blk.autotune(5)
+        strategy, xseg = blk.get_tuning()
+        assert strategy == 2 or (strategy == 0 and xseg in (16, 32))
+        t, d = blk.run(25)
+        assert t == t_ref and same_bits(d, d_ref)
+        for a, c in zip(blk.download(), ref):
+            assert same_bits(a, c)
+
+
 def test_default_strip_width_follows_the_wave_count(swe):
     """The widest of 16 and 12 columns per strip whose busiest SIMD still
     holds the solver's threshold of waves (256 CUs x 4 SIMDs; AugRie 12, 4
