@@ -610,6 +610,12 @@ def main() -> int:
                      "source": "rocprofv3 SQ_INSTS_VALU of fused_u_kernel on this workload "
                                "(profiles/fused_u_augrie_2048_islands.md); peak = CUs x 4 SIMDs x 1/4 "
                                "wave64 instruction per cycle x 2.4 GHz"}
+    # what binds the dominant kernel, from the committed counters: VALU issue
+    # or HBM traffic near their peaks, else latency (neither saturated)
+    binding = None
+    if valu_line and traffic:
+        hbm_frac = traffic / (pass1_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+        binding = "valu" if valu_line["frac"] >= 0.8 else "hbm" if hbm_frac >= 0.7 else "latency"
     sname = args.solver.upper() if args.solver != "augrie" else "AUGRIE"
     names = {0: (f"sweep_u_kernel<{sname},MODE_MAX>", f"sweep_u_kernel<{sname},MODE_UPDATE>"),
              1: (f"sweep_u_kernel<{sname},MODE_STORE>", "apply_u_kernel"),
@@ -640,9 +646,9 @@ def main() -> int:
             "wall_s_incl_sync": round(wall_s, 4),
         },
         "roofline": {
-            # the contract's roofline: algorithmic bytes over the dominant
-            # kernel's time, against HBM peak; what binds that kernel is
-            # vector-ALU issue ("binding", "valu")
+            # algorithmic bytes over the dominant kernel's time, against HBM
+            # peak; "binding" names what limits that kernel (valu, hbm or
+            # latency, from the committed counters)
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -656,7 +662,7 @@ def main() -> int:
                       f"between launches: {kms[0] * 1e3:.1f} us; {step_ms_k * 1e3:.2f} us per step in this "
                       f"attribution run (fused launches timed on their own dispatches)",
             "algorithmic_bytes_per_cell_update": BYTES_PER_CELL,
-            "binding": "valu" if args.solver == "augrie" else None,
+            "binding": binding,
             "valu": valu_line,
         },
         "cpu_baseline": cpu,
