@@ -193,6 +193,24 @@ int swe_hip_synchronize(swe_hip_block* blk);
  * (floats between consecutive x columns; equal to ny+2). */
 int swe_hip_device_pointers(swe_hip_block* blk, float** h, float** hu, float** hv, float** b, int* pitch);
 
+/* Asynchronous snapshots of the state for output writers, double-buffered
+ * (the reference writes synchronously from host mirrors at each checkpoint,
+ * swe_simple.cpp:187, 257; no counterpart entry).  swe_hip_snapshot queues a
+ * copy of the current h, hu, hv (ghosts included, Float2D layout with pitch
+ * ny+2) into the next of SWE_HIP_SNAPSHOT_SLOTS slots and returns the slot at
+ * once: a device-side copy in the block's stream order, so that later steps
+ * may overwrite the state, then the transfer to pinned host arrays on a
+ * stream of its own while the steps go on.  If that slot's previous snapshot
+ * has not been released yet, it waits for the release first.
+ * swe_hip_snapshot_wait waits for the slot's transfer and returns its host
+ * arrays (valid until the slot is released); swe_hip_snapshot_release hands
+ * the slot back.  Wait and release may come from another thread than the one
+ * stepping the block (a writer thread); they do not take the block's lock. */
+#define SWE_HIP_SNAPSHOT_SLOTS 2
+int swe_hip_snapshot(swe_hip_block* blk, int* slot);
+int swe_hip_snapshot_wait(swe_hip_block* blk, int slot, const float** h, const float** hu, const float** hv);
+int swe_hip_snapshot_release(swe_hip_block* blk, int slot);
+
 /* Field masks for the layer functions below. */
 #define SWE_FIELD_H 1
 #define SWE_FIELD_HU 2

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -99,6 +100,19 @@ struct swe_hip_block {
     // of coherent host words, tagged with the pass's sequence number
     unsigned long long* rbRing = nullptr;
     unsigned rbSeq = 0;  // sequence number of the last report asked for (never 0)
+    // asynchronous snapshots (swe_hip_snapshot): per slot a device copy of
+    // h, hu, hv taken on the block's stream and pinned host arrays filled
+    // from it on snapStream while the steps go on
+    struct Snap {
+        float* dev[3] = {nullptr, nullptr, nullptr};
+        float* host[3] = {nullptr, nullptr, nullptr};
+        hipEvent_t staged = nullptr, done = nullptr;
+        bool busy = false;  // queued and not yet released
+    } snap[SWE_HIP_SNAPSHOT_SLOTS];
+    int snapNext = 0;
+    hipStream_t snapStream = nullptr;
+    std::mutex snapMu;  // guards busy; never held while waiting on the GPU
+    std::condition_variable snapCv;
     // Calls on one block are serialised: besides its own calls, a block is
     // read by its neighbours' layer copies, which apply its pending update
     // (drivers run blocks on several host threads).  Recursive: some entry
@@ -746,6 +760,16 @@ int swe_hip_create(int nx, int ny, float dx, float dy, float ox, float oy, int s
 int swe_hip_destroy(swe_hip_block* k) {
     if (!k) return SWE_HIP_OK;
     if (k->stream) (void)hipStreamSynchronize(k->stream);
+    if (k->snapStream) (void)hipStreamSynchronize(k->snapStream);
+    for (auto& sn : k->snap) {
+        for (int i = 0; i < 3; ++i) {
+            if (sn.dev[i]) (void)hipFree(sn.dev[i]);
+            if (sn.host[i]) (void)hipHostFree(sn.host[i]);
+        }
+        if (sn.staged) (void)hipEventDestroy(sn.staged);
+        if (sn.done) (void)hipEventDestroy(sn.done);
+    }
+    if (k->snapStream) (void)hipStreamDestroy(k->snapStream);
     if (k->net) swe::mlp_detach(k->net, k->stream);
     for (int i = 0; i < 2; ++i) {
         if (k->h[i]) (void)hipFree(k->h[i]);
@@ -1169,6 +1193,94 @@ int opposite(int side) { return side ^ 1; }
 }  // namespace
 
 extern "C" {
+
+int swe_hip_snapshot(swe_hip_block* k, int* slot) {
+    if (int r = check(k)) return r;
+    if (!slot) {
+        set_error("swe_hip_snapshot: null slot");
+        return SWE_HIP_EINVAL;
+    }
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (int r = settle(k)) return r;
+    const int s = k->snapNext;
+    auto& sn = k->snap[s];
+    {  // double buffering: the slot's last snapshot must have been released
+        std::unique_lock<std::mutex> g(k->snapMu);
+        k->snapCv.wait(g, [&] { return !sn.busy; });
+    }
+    const size_t bytes = field_elems(k) * sizeof(float);
+    if (!k->snapStream) HIP_TRY(hipStreamCreateWithFlags(&k->snapStream, hipStreamNonBlocking));
+    for (int i = 0; i < 3; ++i) {
+        if (!sn.dev[i]) HIP_TRY(hipMalloc(&sn.dev[i], bytes));
+        if (!sn.host[i]) HIP_TRY(hipHostMalloc((void**)&sn.host[i], bytes, hipHostMallocDefault));
+    }
+    if (!sn.staged) HIP_TRY(hipEventCreateWithFlags(&sn.staged, hipEventDisableTiming));
+    if (!sn.done) HIP_TRY(hipEventCreateWithFlags(&sn.done, hipEventDisableTiming));
+    // a device copy in the block's stream order (later steps may overwrite
+    // the state), then the transfer on the snapshot stream
+    const float* src[3] = {k->h[k->cur], k->hu[k->cur], k->hv[k->cur]};
+    for (int i = 0; i < 3; ++i)
+        HIP_TRY(hipMemcpyAsync(sn.dev[i], src[i], bytes, hipMemcpyDeviceToDevice, k->stream));
+    HIP_TRY(hipEventRecord(sn.staged, k->stream));
+    HIP_TRY(hipStreamWaitEvent(k->snapStream, sn.staged, 0));
+    for (int i = 0; i < 3; ++i)
+        HIP_TRY(hipMemcpyAsync(sn.host[i], sn.dev[i], bytes, hipMemcpyDeviceToHost, k->snapStream));
+    HIP_TRY(hipEventRecord(sn.done, k->snapStream));
+    {
+        std::lock_guard<std::mutex> g(k->snapMu);
+        sn.busy = true;
+    }
+    k->snapNext = (s + 1) % SWE_HIP_SNAPSHOT_SLOTS;
+    *slot = s;
+    return SWE_HIP_OK;
+}
+
+int swe_hip_snapshot_wait(swe_hip_block* k, int slot, const float** h, const float** hu, const float** hv) {
+    if (int r = check(k)) return r;
+    if (slot < 0 || slot >= SWE_HIP_SNAPSHOT_SLOTS || !h || !hu || !hv) {
+        set_error("swe_hip_snapshot_wait: invalid slot %d or null output", slot);
+        return SWE_HIP_EINVAL;
+    }
+    auto& sn = k->snap[slot];
+    hipEvent_t done = nullptr;
+    {
+        std::lock_guard<std::mutex> g(k->snapMu);
+        if (!sn.busy) {
+            set_error("swe_hip_snapshot_wait: slot %d holds no snapshot", slot);
+            return SWE_HIP_ESTATE;
+        }
+        done = sn.done;
+    }
+    HIP_TRY(hipEventSynchronize(done));
+    *h = sn.host[0];
+    *hu = sn.host[1];
+    *hv = sn.host[2];
+    return SWE_HIP_OK;
+}
+
+int swe_hip_snapshot_release(swe_hip_block* k, int slot) {
+    if (int r = check(k)) return r;
+    if (slot < 0 || slot >= SWE_HIP_SNAPSHOT_SLOTS) {
+        set_error("swe_hip_snapshot_release: invalid slot %d", slot);
+        return SWE_HIP_EINVAL;
+    }
+    auto& sn = k->snap[slot];
+    {
+        std::lock_guard<std::mutex> g(k->snapMu);
+        if (!sn.busy) {
+            set_error("swe_hip_snapshot_release: slot %d holds no snapshot", slot);
+            return SWE_HIP_ESTATE;
+        }
+    }
+    // the transfer must be over before the slot is reused
+    HIP_TRY(hipEventSynchronize(sn.done));
+    {
+        std::lock_guard<std::mutex> g(k->snapMu);
+        sn.busy = false;
+    }
+    k->snapCv.notify_all();
+    return SWE_HIP_OK;
+}
 
 int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fields) {
     if (int r = check(dst)) return r;

@@ -40,6 +40,7 @@ struct DriverArgs {
     bool hostInit = false;     // --host-init: sample the scenario on the host (initScenario), not the device
     long warmup = 0;           // --warmup: untimed steps, after which the initial state is loaded again
     int lookahead = 1;         // --lookahead 0: no look-ahead launches (swe_hip_set_lookahead)
+    bool asyncWrite = false;   // --async-write: checkpoints written from snapshots by a writer thread
 };
 
 inline int solver_from_name(const std::string& s) {
@@ -74,6 +75,7 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
                             {"host-init", no_argument, nullptr, 11},
                             {"warmup", required_argument, nullptr, 12},
                             {"lookahead", required_argument, nullptr, 13},
+                            {"async-write", no_argument, nullptr, 14},
                             {nullptr, 0, nullptr, 0}};
     int c;
     while ((c = getopt_long(argc, argv, "t:n:x:y:o:b:l:w:", opts, nullptr)) != -1) {
@@ -141,6 +143,9 @@ inline DriverArgs parse_args(int argc, char** argv, const char* default_scenario
                 break;
             case 13:
                 a.lookahead = std::atoi(optarg) != 0;
+                break;
+            case 14:
+                a.asyncWrite = true;
                 break;
             default:
                 throw std::invalid_argument("bad command line");

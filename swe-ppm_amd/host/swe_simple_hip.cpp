@@ -13,20 +13,106 @@
 // --input starts from a state snapshot of one block instead (sizes from the
 // snapshot, domain and boundary types from the scenario).  --warmup K runs K
 // untimed steps first and starts again from the initial state; --lookahead 0
-// turns off the look-ahead launches of the per-call API.
+// turns off the look-ahead launches of the per-call API; --async-write has
+// the checkpoints written by a writer thread from device snapshots.
 //
 // With -w 1 the state is written at t = 0 and at every checkpoint: VTK files
 // <out>.<k>.vts (the default) or one netCDF file <out>.nc, the two writers
 // the reference chooses between at build time (swe_simple.cpp:44-48, 172-185).
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
 #include <memory>
+#include <mutex>
+#include <thread>
 
 #include "blocks/SWE_DimensionalSplittingHip.hh"
 #include "scenarios/SWE_simple_scenarios.hh"
 #include "swe_drivers_common.hh"
+#include "tools/Float2DView.hh"
 #include "writer/NetCdfWriter.hh"
 #include "writer/VtkWriter.hh"
+
+// --async-write: checkpoints are snapshotted on the device and written by a
+// thread of their own from the snapshot's host arrays while the loop runs on
+// (two snapshots in flight at most, swe_hip_snapshot).
+class AsyncWriter {
+public:
+    AsyncWriter(Writer& w, swe_hip_block* blk, int nx, int ny) : w_(w), blk_(blk), nx_(nx), ny_(ny) {
+        thread_ = std::thread([this] { loop(); });
+    }
+    ~AsyncWriter() {  // on an exception path: stop without throwing
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            done_ = true;
+            cv_.notify_one();
+        }
+        if (thread_.joinable()) thread_.join();
+    }
+    void write(float t) {
+        int slot = 0;
+        if (swe_hip_snapshot(blk_, &slot) != SWE_HIP_OK) throw std::runtime_error(swe_hip_last_error());
+        std::lock_guard<std::mutex> g(mu_);
+        jobs_.push_back({slot, t});
+        cv_.notify_one();
+    }
+    // waits for every queued frame; rethrows the writer thread's failure
+    void finish() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            done_ = true;
+            cv_.notify_one();
+        }
+        if (thread_.joinable()) thread_.join();
+        if (!error_.empty()) {
+            const std::string e = error_;
+            error_.clear();
+            throw std::runtime_error(e);
+        }
+    }
+
+private:
+    struct Job {
+        int slot;
+        float t;
+    };
+    void loop() {
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return done_ || !jobs_.empty(); });
+                if (jobs_.empty()) return;
+                j = jobs_.front();
+                jobs_.pop_front();
+            }
+            const float *h, *hu, *hv;
+            if (swe_hip_snapshot_wait(blk_, j.slot, &h, &hu, &hv) != SWE_HIP_OK) {
+                error_ = swe_hip_last_error();
+                return;
+            }
+            if (error_.empty()) {
+                try {
+                    w_.writeTimeStep(Float2DView(nx_ + 2, ny_ + 2, h), Float2DView(nx_ + 2, ny_ + 2, hu),
+                                     Float2DView(nx_ + 2, ny_ + 2, hv), j.t);
+                } catch (const std::exception& e) {
+                    error_ = e.what();
+                }
+            }
+            swe_hip_snapshot_release(blk_, j.slot);
+        }
+    }
+    Writer& w_;
+    swe_hip_block* blk_;
+    int nx_, ny_;
+    std::thread thread_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Job> jobs_;
+    bool done_ = false;
+    std::string error_;
+};
 
 // the scenario object and its device-sampling id (SWE_SCENARIO_*)
 static std::unique_ptr<SWE_Scenario> make_scenario(const std::string& name, int* id) {
@@ -120,8 +206,17 @@ int main(int argc, char** argv) {
                 writer.reset(new NetCdfWriter(a.out, sim.getBathymetry(), {{1, 1, 1, 1}}, a.nx, a.ny, dx, dy, ox, oy));
             else
                 writer.reset(new VtkWriter(a.out, sim.getBathymetry(), {{1, 1, 1, 1}}, a.nx, a.ny, dx, dy, ox, oy));
-            writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(), 0.f);
         }
+        std::unique_ptr<AsyncWriter> async;
+        if (writer && a.asyncWrite) async.reset(new AsyncWriter(*writer, sim.handle(), a.nx, a.ny));
+        auto write_frame = [&](float time) {
+            if (async)
+                async->write(time);
+            else if (writer)
+                writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(),
+                                      time);
+        };
+        write_frame(0.f);
 
         float t = 0.f;
         long iterations = 0;
@@ -145,9 +240,9 @@ int main(int argc, char** argv) {
             swe_hip_synchronize(sim.handle());
             wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             std::printf("Write timestep (%fs)\n", t);
-            if (writer)
-                writer->writeTimeStep(sim.getWaterHeight(), sim.getMomentumHorizontal(), sim.getMomentumVertical(), t);
+            write_frame(t);
         }
+        if (async) async->finish();
         if (!a.dump.empty()) write_snapshot(a.dump, {snapshot_of(sim)});
         std::printf("iterations %ld simulated %.6f s wall %.4f s  %.2f MCell-updates/s\n", iterations, t, wall,
                     (double)a.nx * a.ny * iterations / wall / 1e6);

@@ -77,6 +77,9 @@ def _load() -> C.CDLL:
         "swe_hip_set_math": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_quiet_skip": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_lookahead": (C.c_int, [vp, C.c_int]),
+        "swe_hip_snapshot": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "swe_hip_snapshot_wait": (C.c_int, [vp, C.c_int] + [C.POINTER(fp)] * 3),
+        "swe_hip_snapshot_release": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
         "swe_hip_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "swe_hip_batch_create": (C.c_int, [C.c_int, ip, ip, fp, fp, ip, ip, C.c_int, C.c_int, C.c_float, C.c_float,
@@ -254,6 +257,22 @@ class SweBlock:
         """Skip quiet strip tiles in the fused device loop (default on);
         results are identical either way."""
         _check(lib.swe_hip_set_quiet_skip(self._h, int(bool(enable))), "set_quiet_skip")
+
+    def snapshot(self) -> int:
+        """Queue an asynchronous copy of h, hu, hv (swe_hip_snapshot); returns its slot."""
+        slot = C.c_int()
+        _check(lib.swe_hip_snapshot(self._h, C.byref(slot)), "snapshot")
+        return slot.value
+
+    def snapshot_wait(self, slot: int):
+        """Wait for a snapshot and return copies of its h, hu, hv."""
+        ps = [C.POINTER(C.c_float)() for _ in range(3)]
+        _check(lib.swe_hip_snapshot_wait(self._h, int(slot), *[C.byref(p) for p in ps]), "snapshot_wait")
+        n = self.shape[0] * self.shape[1]
+        return tuple(np.ctypeslib.as_array(p, shape=(n,)).reshape(self.shape).copy() for p in ps)
+
+    def snapshot_release(self, slot: int) -> None:
+        _check(lib.swe_hip_snapshot_release(self._h, int(slot)), "snapshot_release")
 
     def set_lookahead(self, enable: bool) -> None:
         """Queue the next per-call step while the host runs its loop (default

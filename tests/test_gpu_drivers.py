@@ -133,6 +133,25 @@ def test_swe_simple_driver_writes_netcdf(tmp_path):
     _assert_nc_matches(tmp_path / "run.nc", snapshots.read(f1)[0], 2, t)
 
 
+@pytest.mark.parametrize("writer", ["netcdf", "vtk"])
+def test_async_snapshot_writer_gives_the_same_files(tmp_path, writer):
+    """--async-write (device snapshots, double-buffered, written by a thread
+    of their own while the loop runs on) writes the same bytes as the
+    synchronous writer: 6 checkpoints, 2 snapshots in flight at most."""
+    outs = {}
+    for mode in ("sync", "async"):
+        d = tmp_path / mode
+        d.mkdir()
+        args = [BIN / "swe_simple_hip", "-t", "60", "-n", "6", "-x", "130", "-y", "90", "--solver", "augrie",
+                "--scenario", "radial", "-w", "1", "--writer", writer, "-o", d / "run"]
+        run(args + (["--async-write"] if mode == "async" else []))
+        outs[mode] = {p.name: p.read_bytes() for p in sorted(d.iterdir())}
+    assert len(outs["sync"]) == (1 if writer == "netcdf" else 7)
+    assert outs["sync"].keys() == outs["async"].keys()
+    for name in outs["sync"]:
+        assert outs["sync"][name] == outs["async"][name], name
+
+
 def test_overdecomp_driver_writes_one_netcdf_file_per_block(tmp_path):
     f1 = tmp_path / "final.bin"
     out = run([BIN / "swe_overdecomp_hip", "-t", "1000", "-n", "1", "-x", "60", "-y", "40", "-b", "6", "--solver",

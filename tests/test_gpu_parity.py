@@ -185,6 +185,46 @@ def test_per_call_orders_equal_eager_updates(swe):
             assert same_bits(x, y)
 
 
+def test_snapshots_hold_the_state_of_their_call(swe):
+    """swe_hip_snapshot during stepping: each slot holds the state at its
+    call (as a download there), whatever ran after it; slots alternate, and
+    wait/release on a slot without a snapshot are refused."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=120, seed=6)
+    with swe.SweBlock(120, 120, dx, dy, solver=swe.AUGRIE) as ref, swe.SweBlock(120, 120, dx, dy) as blk:
+        for k in (ref, blk):
+            k.set_boundaries(bnd)
+            k.upload(h, hu, hv, b)
+        with pytest.raises(swe.SweHipError):
+            blk.snapshot_wait(0)
+        expected, slots = [], []
+        for rnd in range(5):
+            for k in (ref, blk):
+                k.run(3)
+            for _ in range(2):  # per-call steps left pending
+                for k in (ref, blk):
+                    k.apply_bc()
+                    k.update(k.fluxes())
+            expected.append(ref.download())
+            slots.append(blk.snapshot())
+            blk.run(4)  # overwrites both state buffers
+            ref.run(4)
+            if rnd % 2 == 1:  # release the two in flight
+                for s in slots[-2:]:
+                    got = blk.snapshot_wait(s)
+                    for a, c in zip(got, expected[slots.index(s, len(slots) - 2)]):
+                        assert same_bits(a, c)
+                    blk.snapshot_release(s)
+        assert slots[:4] == [0, 1, 0, 1]
+        got = blk.snapshot_wait(slots[4])
+        for a, c in zip(got, expected[4]):
+            assert same_bits(a, c)
+        blk.snapshot_release(slots[4])
+        with pytest.raises(swe.SweHipError):
+            blk.snapshot_release(slots[4])
+
+
 def test_update_needs_a_fresh_flux_pass(swe):
     from swe_ppm_amd import scenarios
 
