@@ -686,8 +686,6 @@ def main() -> int:
     if not args.no_batched:
         lts = batched_lts(swe, scenarios, not args.no_autotune, duration=args.lts_duration)
         ens = batched_ensemble(swe, scenarios, not args.no_autotune, steps=args.ens_steps)
-        for b in (lts, ens):
-            b["vs_plain"] = round(b["value"] / mcells, 4)
         line["batched"] = {"lts_28_blocks": lts, "ensemble_512": ens}
     if not args.no_surrogate:
         if Path(weights).exists():

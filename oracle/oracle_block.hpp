@@ -219,6 +219,17 @@ void update_u(int nx, int ny, T dx, T dy, T dt, T* hp, T* hup, T* hvp, NetUpdate
 // x-speeds only, intermediate Q*, y-sweep on the OLD h and hv, update with
 // dx used for the y-part as the reference does
 // (SWE_DimensionalSplitting.cpp:190-286).
+//
+// dt rule: 0.4 * dx / (the maximum edge speed of the x-sweep).  The
+// reference passes its OpenMP reduction variable itself as the solver's
+// speed output (SWE_DimensionalSplitting.cpp:111-136), and the solver
+// overwrites that argument with each edge's speed instead of raising it, so
+// each thread's copy ends as the speed of the last edge that thread solved,
+// and dt depends on the thread count and the loop partition.  This
+// restatement (and the GPU) takes the true maximum over all x-edges, the
+// value the reduction is written to compute (with one thread the
+// reference's dt would come from the last x-edge alone);
+// tests/test_oracle_scheme.py pins it with a known-answer case.
 template <typename T>
 struct SplitScratch {
     int nx, ny;

@@ -287,6 +287,13 @@ int oracle_multiblock_run_f32(int nblocks, const int* nx, const int* ny, float d
     return multiblock_run<float>(nblocks, nx, ny, dx, dy, bnd, nbr, solver, lts, maxTimestepLocal, duration,
                                  nmacro, nthreads, h, hu, hv, b, updates_out, iters_out, dt_out);
 }
+int oracle_multiblock_run_f64(int nblocks, const int* nx, const int* ny, double dx, double dy, const int* bnd,
+                              const int* nbr, int solver, int lts, double maxTimestepLocal, double duration,
+                              int nmacro, int nthreads, double* h, double* hu, double* hv, double* b,
+                              int64_t* updates_out, int32_t* iters_out, double* dt_out) {
+    return multiblock_run<double>(nblocks, nx, ny, dx, dy, bnd, nbr, solver, lts, maxTimestepLocal, duration,
+                                  nmacro, nthreads, h, hu, hv, b, updates_out, iters_out, dt_out);
+}
 // getRoundTimestep of a fresh block (stepSizeCounter == 0), for checking the
 // product's divisor computation against the literal formula.
 int oracle_round_timestep_f32(float maxTimestepLocal, float ts, float* rounded) {

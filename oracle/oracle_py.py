@@ -47,6 +47,10 @@ def _load(path: Path = LIB_PATH) -> C.CDLL:
     lib.oracle_multiblock_run_f32.argtypes = [C.c_int, ip, ip, C.c_float, C.c_float, ip, ip, C.c_int, C.c_int,
                                               C.c_float, C.c_float, C.c_int, C.c_int, fp, fp, fp, fp,
                                               C.POINTER(C.c_int64), C.POINTER(C.c_int32), fp]
+    dp = C.POINTER(C.c_double)
+    lib.oracle_multiblock_run_f64.argtypes = [C.c_int, ip, ip, C.c_double, C.c_double, ip, ip, C.c_int, C.c_int,
+                                              C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp, dp,
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp]
     lib.oracle_round_timestep_f32.argtypes = [C.c_float, C.c_float, fp]
     lib.oracle_lts_trace_f32.argtypes = [C.c_int, ip, ip, C.c_float, C.c_float, ip, ip, C.c_int, C.c_float,
                                          C.c_float, C.c_int, fp, fp, fp, fp, fp, C.POINTER(C.c_int32)]
@@ -177,7 +181,8 @@ def round_timestep(max_local: float, ts: float):
     return k, r.value
 
 
-def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nthreads=1, release=False):
+def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nthreads=1, release=False,
+                   dtype=np.float32):
     """Overdecomposed driver.  blocks: list of dicts with nx, ny, bnd[4], nbr[4]
     (neighbour block index or -1) and h, hu, hv, b arrays.  Returns
     (states, updates, iterations, dts).  release: the Release-flags build
@@ -187,13 +192,14 @@ def multiblock_run(blocks, dx, dy, solver, lts, max_local, duration, nmacro, nth
     ny, nyp = _ints([bl["ny"] for bl in blocks])
     bnd, bndp = _ints(np.concatenate([bl["bnd"] for bl in blocks]))
     nbr, nbrp = _ints(np.concatenate([bl["nbr"] for bl in blocks]))
-    cat = [np.concatenate([np.asarray(bl[k], np.float32).ravel() for bl in blocks]) for k in ("h", "hu", "hv", "b")]
+    cat = [np.concatenate([np.asarray(bl[k], dtype).ravel() for bl in blocks]) for k in ("h", "hu", "hv", "b")]
     upd = np.zeros(n, np.int64)
     its = np.zeros(max(nmacro, 1), np.int32)
-    dts = np.zeros(max(nmacro, 1), np.float32)
+    dts = np.zeros(max(nmacro, 1), dtype)
     L = release_lib() if release else lib
-    rc = L.oracle_multiblock_run_f32(n, nxp, nyp, dx, dy, bndp, nbrp, solver, int(lts), max_local, duration,
-                                       nmacro, nthreads, *[_p(a) for a in cat], _p(upd), _p(its), _p(dts))
+    f = L.oracle_multiblock_run_f32 if dtype == np.float32 else L.oracle_multiblock_run_f64
+    rc = f(n, nxp, nyp, dx, dy, bndp, nbrp, solver, int(lts), max_local, duration, nmacro, nthreads,
+           *[_p(a) for a in cat], _p(upd), _p(its), _p(dts))
     if rc != 0:
         raise RuntimeError("oracle multiblock run hit its iteration cap")
     out, off = [], 0

@@ -66,8 +66,13 @@ constexpr unsigned kQuietAll = 1u, kQuietBottom = 2u, kQuietTop = 4u, kQuietLeft
 // spans XS / kQuietCols of them.
 constexpr int kQuietCols = 4;
 // Waves the quiet-tile plan aims to fill with busy record tiles before it
-// lengthens the pieces it lists.
-constexpr unsigned kPlanWaves = 2048;
+// lengthens the pieces it lists: four per SIMD of the chip.  Measured on the
+// 2048^2 islands, the 4096^2 trench and the 4096^2 islands (us per step):
+// 1024: 92.9 / - / -; 2048: 85.6 / 92.4 / 169.4; 3072: 81.3 / 89.2 / 166.9;
+// 4096: 79.7 / 89.0 / 165.6; 5120: 81.0 / 90.3 / 165.0; 6144: 82.5 / 93.3 /
+// 166.1; 8192: 88.3 / - / -.  Shorter pieces give more waves to hide the
+// busy tiles' latency but solve more halo edges.
+constexpr unsigned kPlanWaves = 4096;
 
 // Lanes the quiet-tile plan gives a wave segment of `fine` record tiles: the
 // smallest power of two >= fine, so that a segment's lanes share one wave.
