@@ -1215,7 +1215,11 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
         if (r == hipSuccess) r = hipStreamSynchronize(k->stream);
         return r;
     };
-    if (e == hipSuccess) e = copy_all(true);
+    bool saved = false;
+    if (e == hipSuccess) {
+        e = copy_all(true);
+        saved = e == hipSuccess;
+    }
     const int xsC[3] = {4, 8, 16};
     const int gC[4] = {2, 4, 8, 0};  // x CUs; 0 = the direct form
     const int oldXs = k->xs, oldGroups = k->groups, oldChunk = k->chunk;
@@ -1259,11 +1263,19 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (e == hipSuccess && rc == 0) e = copy_all(false);
+    // the snapshot goes back also after a failed timed run, which leaves the
+    // batch half-advanced; the first error is the one reported
+    if (saved) {
+        const hipError_t first = e;
+        e = copy_all(false);
+        if (first != hipSuccess) e = first;
+    }
     for (void* p : {(void*)sh, (void*)shu, (void*)shv, (void*)sl, (void*)sst, (void*)sg})
         if (p) (void)hipFree(p);
-    if (rc) return rc;
-    if (e != hipSuccess) return hip_fail(e, "swe_hip_batch_autotune");
+    if (rc || e != hipSuccess) {
+        (void)swe_hip_batch_set_tuning(k, oldXs, oldGroups, oldChunk, -1, -1);
+        return rc ? rc : hip_fail(e, "swe_hip_batch_autotune");
+    }
     if (ncand) *ncand = n;
     return swe_hip_batch_set_tuning(k, bestXs, bestGroups, oldChunk, -1, -1);
 }
