@@ -21,14 +21,18 @@ constexpr int SOLVER_AUGRIE_DEFER = 5;
 // Waves per SIMD the flavour-U sweeps of each solver are compiled for
 // (its amdgpu_waves_per_eu floor; the build fails where the compiler cannot
 // meet it).  A SIMD lane has 512 registers: 7 waves need at most 72 per lane,
-// 6 at most 80.  AugRie, Hybrid and the deferring learned sweep fit 7 (71-72
-// registers, no spills); HLLE (79) and f-wave (80 at this target, 84 without
-// it) would spill at 7; the in-sweep network call holds 136 vector and 32
-// accumulation registers (3 waves).
+// 6 at most 80.  Hybrid and the deferring learned sweep fit 7 (71-72
+// registers, no spills).  AugRie's fused kernel keeps a quiet-tile record,
+// which at 7 waves spills 7 values to scratch; at 6 it takes 79 registers
+// without spills and runs the 2048^2 islands 0.5%, the disturbed islands
+// 2.8% and the 4096^2 trench 0.9% faster (5 waves: 81 registers, slower).
+// HLLE (79) and f-wave (80 at this target, 84 without it) would spill at 7;
+// the in-sweep network call holds 136 vector and 32 accumulation registers
+// (3 waves).
 constexpr int fused_waves_per_simd(int solver) {
-    return (solver == SOLVER_AUGRIE || solver == SOLVER_HYBRID || solver == SOLVER_AUGRIE_DEFER)
-               ? 7
-               : (solver == SOLVER_AUGRIE_MLP ? 3 : 6);
+    return (solver == SOLVER_HYBRID || solver == SOLVER_AUGRIE_DEFER) ? 7
+           : solver == SOLVER_AUGRIE_MLP                             ? 3
+                                                                     : 6;
 }
 
 // Step flavours: U = every parallel block of the reference (all edges from
