@@ -224,7 +224,10 @@ int swe_hip_snapshot_release(swe_hip_block* blk, int slot);
  * Replaces the CONNECT_WITHIN_RANK copies of
  * SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304 (h, hu, hv) and :208-235
  * (bathymetry).  Ordered after src's queued work and before src's later
- * work; the shared edge lengths must match. */
+ * work; the shared edge lengths must match.  If src's last update is still
+ * pending (swe_hip_update), h, hu, hv are copied with that update applied on
+ * the fly and src keeps it pending, so its next flux pass stays one fused
+ * launch. */
 int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fields);
 
 /* Host transfer of one line of length ny (LEFT/RIGHT) or nx (BOTTOM/TOP),

@@ -99,6 +99,9 @@ struct swe_hip_block {
     // each flux pass's CFL maximum is written by a small kernel into a ring
     // of coherent host words, tagged with the pass's sequence number
     unsigned long long* rbRing = nullptr;
+    // events of swe_hip_copy_layer into this block, made once and reused (an
+    // event pair per call cost more than the copy itself)
+    hipEvent_t copyEv[2] = {nullptr, nullptr};
     unsigned rbSeq = 0;  // sequence number of the last report asked for (never 0)
     // asynchronous snapshots (swe_hip_snapshot): per slot a device copy of
     // h, hu, hv taken on the block's stream and pinned host arrays filled
@@ -789,6 +792,8 @@ int swe_hip_destroy(swe_hip_block* k) {
     if (k->tlist) (void)hipFree(k->tlist);
     if (k->tcount) (void)hipFree(k->tcount);
     if (k->rbRing) (void)hipHostFree(k->rbRing);
+    for (hipEvent_t ev : k->copyEv)
+        if (ev) (void)hipEventDestroy(ev);
     if (k->stream) (void)hipStreamDestroy(k->stream);
     delete k;
     return SWE_HIP_OK;
@@ -1294,12 +1299,19 @@ int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fie
         lockDst.lock();
     else
         std::lock(lockDst, lockSrc);
-    // the source's copy layer must hold its latest update; the destination's
-    // pending update keeps the ghost lines of CONNECT and PASSIVE sides as
-    // they are (the next flux pass carries them over), but rewrites those of
-    // OUTFLOW and WALL sides, so a copy into such a line waits for it.  A
-    // look-ahead launch of the destination read the old ghost line.
-    if (int r = settle(src)) return r;
+    // The copy layer must be the source's state with its latest update.  A
+    // source update still pending stays pending (its next flux pass applies
+    // it on load): h, hu, hv are copied with the update applied on the fly,
+    // the same arithmetic as applying it.  The destination's pending update
+    // keeps the ghost lines of CONNECT and PASSIVE sides as they are (the next
+    // flux pass carries them over), but rewrites those of OUTFLOW and WALL
+    // sides, so a copy into such a line waits for it.  A look-ahead launch of
+    // the destination read the old ghost line.
+    const int moving = fields & (SWE_FIELD_H | SWE_FIELD_HU | SWE_FIELD_HV);
+    const bool onTheFly = src->flavour == FLAVOUR_U && src->pending && src != dst &&
+                          (moving == 0 || moving == (SWE_FIELD_H | SWE_FIELD_HU | SWE_FIELD_HV));
+    if (!onTheFly)
+        if (int r = settle(src)) return r;
     drop_lookahead(dst);
     dst->pcClean = false;
     dst->pcStep = 0;
@@ -1311,13 +1323,34 @@ int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fie
         set_error("swe_hip_copy_layer: edge lengths differ (%d vs %d)", gd.n, gs.n);
         return SWE_HIP_EINVAL;
     }
-    hipEvent_t e1 = nullptr, e2 = nullptr;
-    HIP_TRY(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
-    hipError_t e = hipEventCreateWithFlags(&e2, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventRecord(e1, src->stream);
+    for (hipEvent_t& ev : dst->copyEv)
+        if (!ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipEvent_t e1 = dst->copyEv[0], e2 = dst->copyEv[1];
+    hipError_t e = hipEventRecord(e1, src->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(dst->stream, e1, 0);
-    for (int bit = 1; bit <= 8 && e == hipSuccess; bit <<= 1) {
+    int rc = 0;
+    if (onTheFly && moving && e == hipSuccess) {
+        const SumsView ps = pc_sums(src);
+        LinePending L;
+        L.dh = dst->h[dst->cur] + gd.off;
+        L.dhu = dst->hu[dst->cur] + gd.off;
+        L.dhv = dst->hv[dst->cur] + gd.off;
+        L.dstride = gd.stride;
+        L.sh = src->h[src->cur] + gs.off;
+        L.shu = src->hu[src->cur] + gs.off;
+        L.shv = src->hv[src->cur] + gs.off;
+        L.sums = SumsView{ps.sx + gs.off, ps.sy + gs.off, ps.su + gs.off, ps.sv + gs.off};
+        L.sstride = gs.stride;
+        L.n = gs.n;
+        L.dt = src->pendingDt;
+        L.dx = src->dx;
+        L.dy = src->dy;
+        rc = src->math == 1 ? launch_copy_line_pending_variant<1>(L, dst->stream)
+                            : launch_copy_line_pending_variant<0>(L, dst->stream);
+    }
+    for (int bit = 1; bit <= 8 && e == hipSuccess && rc == 0; bit <<= 1) {
         if (!(fields & bit)) continue;
+        if (onTheFly && bit != SWE_FIELD_B) continue;  // copied above
         float* d = field_ptr(dst, bit) + gd.off;
         const float* s = field_ptr(src, bit) + gs.off;
         e = hipMemcpy2DAsync(d, gd.stride * sizeof(float), s, gs.stride * sizeof(float), sizeof(float), gd.n,
@@ -1325,8 +1358,7 @@ int swe_hip_copy_layer(swe_hip_block* dst, int side, swe_hip_block* src, int fie
     }
     if (e == hipSuccess) e = hipEventRecord(e2, dst->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(src->stream, e2, 0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
+    if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "swe_hip_copy_layer");
     return SWE_HIP_OK;
 }

@@ -76,6 +76,20 @@ int launch_apply_s(StateView v, SumsView sums, float dt, hipStream_t stream);
 template <int V>
 int launch_apply_u_variant(StateView v, SumsView sums, BoundaryInfo bi, DeviceCtl* ctl, Slots sl, float dtFixed,
                            int fuseBc, float* dtLog, int logIdx, hipStream_t stream);
+// A source block's copy layer (h, hu, hv at sh/shu/shv, stride sstride, its
+// pending update's sums at the same offsets) updated with dt on the fly into
+// a destination ghost line (dh/dhu/dhv, stride dstride), n cells.
+struct LinePending {
+    float *dh, *dhu, *dhv;
+    size_t dstride;
+    const float *sh, *shu, *shv;
+    SumsView sums;
+    size_t sstride;
+    int n;
+    float dt, dx, dy;
+};
+template <int V>
+int launch_copy_line_pending_variant(const LinePending& L, hipStream_t stream);
 int launch_solve_edges(int solver, int variant, int64_t n, const float* in, float* out, hipStream_t stream);
 
 // Per-solver instantiations (one translation unit each, compiled in parallel).

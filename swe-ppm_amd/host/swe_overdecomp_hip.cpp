@@ -286,8 +286,8 @@ int main(int argc, char** argv) {
             stop = true;
         }
         for (int i = 0; i < a.checkpoints && !stop; i++) {
+            auto t0 = std::chrono::steady_clock::now();
             while (t < cp[i]) {
-                auto t0 = std::chrono::steady_clock::now();
                 bool synchronized = true;
                 do {
 #pragma omp parallel for schedule(dynamic, 1)
@@ -321,13 +321,15 @@ int main(int argc, char** argv) {
                 macro++;
                 if (lts)
                     for (auto& b : blocks) b->resetStepSizeCounter();
-                for (auto& b : blocks) swe_hip_synchronize(b->handle());
-                wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 if (a.max_steps > 0 && macro >= a.max_steps) {
                     stop = true;
                     break;
                 }
             }
+            // the last updates are still pending or queued: the clock stops
+            // once every block has finished them
+            for (auto& b : blocks) swe_hip_synchronize(b->handle());
+            wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             std::printf("Write timestep (%fs)\n", t);
             write_all(t);
         }

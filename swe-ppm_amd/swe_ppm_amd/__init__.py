@@ -77,6 +77,7 @@ def _load() -> C.CDLL:
         "swe_hip_set_math": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_quiet_skip": (C.c_int, [vp, C.c_int]),
         "swe_hip_set_lookahead": (C.c_int, [vp, C.c_int]),
+        "swe_hip_copy_layer": (C.c_int, [vp, C.c_int, vp, C.c_int]),
         "swe_hip_snapshot": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "swe_hip_snapshot_wait": (C.c_int, [vp, C.c_int] + [C.POINTER(fp)] * 3),
         "swe_hip_snapshot_release": (C.c_int, [vp, C.c_int]),
@@ -257,6 +258,11 @@ class SweBlock:
         """Skip quiet strip tiles in the fused device loop (default on);
         results are identical either way."""
         _check(lib.swe_hip_set_quiet_skip(self._h, int(bool(enable))), "set_quiet_skip")
+
+    def copy_layer(self, side: int, src: "SweBlock", fields: int = 7) -> None:
+        """This block's ghost line on `side` <- src's copy layer at the shared
+        edge (swe_hip_copy_layer; fields: 1 h, 2 hu, 4 hv, 8 b)."""
+        _check(lib.swe_hip_copy_layer(self._h, int(side), src._h, int(fields)), "copy_layer")
 
     def snapshot(self) -> int:
         """Queue an asynchronous copy of h, hu, hv (swe_hip_snapshot); returns its slot."""

@@ -225,6 +225,52 @@ def test_snapshots_hold_the_state_of_their_call(swe):
             blk.snapshot_release(slots[4])
 
 
+@pytest.mark.parametrize("bnd_outer", [0, 1])
+def test_connected_blocks_per_call_match_the_oracle(swe, oracle, bnd_outer):
+    """Two blocks sharing an edge, driven per call as the overdecomposed
+    driver does (setGhostLayer with the in-rank copy, fluxes, the global dt
+    minimum, update): the copy reads the neighbour's copy layer with its
+    pending update applied on the fly, so both blocks keep the fused step.
+    Bitwise against the oracle's connected global-dt run."""
+    from swe_ppm_amd import scenarios
+
+    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=128, seed=12)
+    # left block x in [1, 70], right block x in [71, 128]
+    cut = 70
+    parts = [(0, cut), (cut, 128)]
+    blocks, ob = [], []
+    for k, (x0, x1) in enumerate(parts):
+        nxk = x1 - x0
+        sl = slice(x0, x1 + 2)
+        bk = [bnd_outer] * 4
+        bk[1 if k == 0 else 0] = 4  # CONNECT_WITHIN_RANK towards the other block
+        nbr = [-1, -1, -1, -1]
+        nbr[1 if k == 0 else 0] = 1 - k
+        blk = swe.SweBlock(nxk, 128, dx, dy, solver=swe.AUGRIE)
+        blk.set_boundaries(bk)
+        blk.upload(h[sl], hu[sl], hv[sl], b[sl])
+        blocks.append(blk)
+        ob.append(dict(nx=nxk, ny=128, bnd=bk, nbr=nbr, h=h[sl].copy(), hu=hu[sl].copy(), hv=hv[sl].copy(),
+                       b=b[sl].copy()))
+    nsteps = 25
+    try:
+        for step in range(nsteps):
+            for k, blk in enumerate(blocks):
+                blk.apply_bc()
+                blk.copy_layer(1 if k == 0 else 0, blocks[1 - k])
+            dt = min(blk.fluxes() for blk in blocks)
+            for blk in blocks:
+                blk.update(dt)
+        got = [blk.download() for blk in blocks]
+    finally:
+        for blk in blocks:
+            blk.close()
+    res, _, _, _ = oracle.multiblock_run(ob, np.float32(dx), np.float32(dy), swe.AUGRIE, False, 1.0, 1e30, nsteps)
+    for k in range(2):
+        for a, c in zip(got[k], res[k][:3]):
+            assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), f"block {k}"
+
+
 def test_update_needs_a_fresh_flux_pass(swe):
     from swe_ppm_amd import scenarios
 
