@@ -163,8 +163,11 @@ def cpu_baseline(h, hu, hv, b, dx, dy, bnd, solver, budget_s: float, reps: int =
     for mode, fn in (("single_block", single), ("overdecomposed", overdecomposed)):
         modes[mode] = {}
         for build, release in (("release", True), ("ieee", False)):
+            # a 1-step call is mostly the call's fixed cost (thread start, array
+            # copies): size K from it, time K steps once, then rescale K
             one = fn(release, 1)
-            k = int(min(200, max(2, per_sample / max(one, 1e-6))))
+            k = int(min(400, max(2, per_sample / max(one, 1e-6))))
+            k = int(min(400, max(2, k * per_sample / max(fn(release, k), 1e-6))))
             times = sorted(fn(release, k) for _ in range(reps))
             med = times[len(times) // 2]
             modes[mode][build] = {"value": round(nx * ny * k / med / 1e6, 2), "steps": k,
