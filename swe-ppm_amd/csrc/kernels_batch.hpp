@@ -9,20 +9,27 @@
 //     LTS loop, swe_mpi_overdecomp.cpp:263-343, with the per-block state
 //     machine of SWE_Block.hh:311-562).
 //
-// One driver iteration is three launches (five with a separate apply pass).
+// One driver iteration is two launches with the fused strategy (sweep,
+// round), five with the store strategy (lines, sweep, round, apply, fill).
 // Every launch reads its parameters from device memory, so a chunk of
 // iterations is captured once in a hipGraph and replayed:
-//   lines : per (block, side, piece of the side) workgroup, setGhostLayer's
-//           CONNECT_WITHIN_RANK part
+//   lines : setGhostLayer's CONNECT_WITHIN_RANK part
 //           (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304): the
 //           neighbour's copy layer straight into the ghost line (global dt),
 //           or, for LTS, into the side's receive line when the side receives
 //           this iteration, followed by checkAllGhostlayers' copy or
-//           interpolation (SWE_Block.hh:375-436).
+//           interpolation (SWE_Block.hh:375-436).  The previous round wrote
+//           each side's job (what to do, from which buffers).  The fused
+//           sweep does this step itself: a tile of a block that computes
+//           takes its ghost cells from the job as it loads them, and the
+//           sides of the blocks that wait are extra items of its queue.  The
+//           store strategy runs it as a launch of its own, one workgroup per
+//           (block, side, piece of the side).
 //   sweep : waves take (block, tile) items from the queue and solve the
 //           tile's edges: per-cell net-update sums + per-block CFL maximum;
 //           with the fused strategy a block's pending update is applied as
-//           its cells are loaded.  By default the fused sweep's grid has one
+//           its cells are loaded (and, for LTS, a waiting block's ghost-line
+//           steps are items too).  By default the fused sweep's grid has one
 //           wave per possible item and a wave takes the item of its index
 //           (the queue is compact, so the waves past its count leave at
 //           once); a persistent grid that pulls items in a loop is the
@@ -33,9 +40,9 @@
 //           step; then, as every block's time is now known, the next
 //           iteration's LTS bookkeeping: which sides receive and with what
 //           time stamp (receiveGhostLayer, SWE_Block.hh:311-374) and the
-//           ghost-layer state transitions of checkAllGhostlayers; and the
-//           work queue of the next iteration (the tiles of the blocks in
-//           sync).
+//           ghost-layer state transitions of checkAllGhostlayers; the next
+//           iteration's ghost-line jobs and work queue (the tiles of the
+//           blocks in sync).
 //   apply : (store strategy only) waves pull the same items and update their
 //           cells, writing the OUTFLOW / WALL ghosts that the next
 //           setGhostLayer would write; a one-workgroup fill launch then
@@ -54,6 +61,7 @@
 namespace swe {
 
 constexpr int kTileBits = 20;  // work item = block << kTileBits | tile
+constexpr int kLineTile = (1 << kTileBits) - 4;  // item tile kLineTile + side: the side's ghost-line step
 constexpr int kMaxBatchBlocks = 2047;
 constexpr int kMaxDivisor = 1024;  // SWE_Block.hh:538 (maxDivisor)
 
@@ -118,6 +126,93 @@ struct BatchGlobals {
     unsigned deferCount;   // edges queued by the deferring sweep this iteration (SOLVER_AUGRIE_DEFER)
 };
 
+// ---- ghost lines of an iteration ---------------------------------------------
+// Fixed geometry of one (block, side), set at create: n == 0 unless the side
+// is CONNECT_WITHIN_RANK.
+struct GhostGeom {
+    long long src;   // arena offset of the neighbour's copy-layer element 1
+    long long dst;   // arena offset of the block's ghost-line element 1
+    long long line;  // offset of the side's receive line in the line arena (LTS)
+    int n, srcStride, dstStride;
+    float ndx, ndy;  // the neighbour's cell sizes
+};
+constexpr int kLineReceive = 1;   // read the neighbour's copy layer this iteration
+constexpr int kLineKeep = 2;      // LTS: through the side's receive line
+constexpr int kLineActShift = 2;  // ghost action: 0 none, 1 copy, 2 interpolate
+// What the lines launch does for one (block, side) in the next iteration,
+// written by the round (op == 0: nothing).
+struct GhostJob {
+    int op;
+    int srcBuf, pend, sidx;  // the neighbour's state buffer, pending flag and sums set
+    float ndt;               // the neighbour's pending dt
+    int dstBuf;              // the block's state buffer
+    float w;                 // interpolation weight t_me / t_neighbour
+};
+
+// Outputs of the round besides the block states and globals.
+struct RoundIO {
+    int* items;         // the next iteration's queue
+    float* dtLog;       // per completed unit: global dt (0 otherwise)
+    int* iterLog;       // per completed unit: driver iterations
+    int* hostStat;      // pinned: stop, units done, iterations of the unit
+    GhostJob* jobs;     // 4 per block
+};
+
+struct BQ3 {
+    float h, hu, hv;
+};
+
+// One element of a side's ghost-line step: the neighbour's copy-layer value
+// (its pending update applied) or the side's receive line; into the receive
+// line when the side keeps one and receives (writeLine: this thread owns the
+// element), and the new ghost value: the copy, old + (new - old) * w, or the
+// ghost's value `old` unchanged.
+__device__ __forceinline__ BQ3 ghost_element(const BatchArena& A, const GhostJob& j, const GhostGeom& g, int i,
+                                             BQ3 old, bool writeLine) {
+    float* L = A.lines + g.line;
+    const int n = g.n;
+    BQ3 nv;
+    if (j.op & kLineReceive) {
+        const long long c = g.src + (long long)i * g.srcStride;
+        Cell q{A.h[j.srcBuf][c], A.hu[j.srcBuf][c], A.hv[j.srcBuf][c]};
+        if (j.pend)
+            q = update_cell(q.h, q.hu, q.hv, j.ndt / g.ndx, j.ndt / g.ndy, A.s[j.sidx][0][c], A.s[j.sidx][1][c],
+                            A.s[j.sidx][2][c], A.s[j.sidx][3][c]);
+        nv = BQ3{q.h, q.hu, q.hv};
+        if ((j.op & kLineKeep) && writeLine) {
+            L[i] = nv.h;
+            L[n + i] = nv.hu;
+            L[2 * n + i] = nv.hv;
+        }
+    } else {
+        nv = BQ3{L[i], L[n + i], L[2 * n + i]};
+    }
+    const int act = j.op >> kLineActShift;
+    if (act == 1) return nv;
+    if (act == 2)
+        return BQ3{old.h + (nv.h - old.h) * j.w, old.hu + (nv.hu - old.hu) * j.w, old.hv + (nv.hv - old.hv) * j.w};
+    return old;
+}
+
+// A side's ghost-line step done in place by one wave (a waiting block's queue
+// item, or a launch of the store strategy): the ghost line of buffer dstBuf.
+__device__ __forceinline__ void ghost_line_step(const BatchArena& A, const GhostJob& j, const GhostGeom& g, int i0,
+                                                int stride) {
+    const int act = j.op >> kLineActShift;
+    float* f[3] = {A.h[j.dstBuf] + g.dst, A.hu[j.dstBuf] + g.dst, A.hv[j.dstBuf] + g.dst};
+    for (int i = i0; i < g.n; i += stride) {
+        const long long gi = (long long)i * g.dstStride;
+        BQ3 old{0.f, 0.f, 0.f};
+        if (act == 2) old = BQ3{f[0][gi], f[1][gi], f[2][gi]};
+        const BQ3 q = ghost_element(A, j, g, i, old, true);
+        if (act) {
+            f[0][gi] = q.h;
+            f[1][gi] = q.hu;
+            f[2][gi] = q.hv;
+        }
+    }
+}
+
 __device__ __forceinline__ StateView batch_view(const BatchArena& A, const BatchBlockDesc& d, int buf) {
     StateView v;
     v.h = A.h[buf] + d.off;
@@ -162,8 +257,8 @@ __device__ __forceinline__ int lts_step_size(float T, float ts) {
 }
 
 // geometry of one ghost line (ghost = 1) or copy layer (ghost = 0), elements 1..n
-__device__ __forceinline__ void side_line(const BatchBlockDesc& d, int side, int ghost, long long& off, int& stride,
-                                          int& n) {
+__host__ __device__ __forceinline__ void side_line(const BatchBlockDesc& d, int side, int ghost, long long& off,
+                                                   int& stride, int& n) {
     switch (side) {
         case BND_LEFT:
             off = (long long)(ghost ? 0 : 1) * d.pitch + 1;
@@ -187,6 +282,12 @@ __device__ __forceinline__ void side_line(const BatchBlockDesc& d, int side, int
             break;
     }
 }
+
+}  // namespace swe
+
+#include "kernels_round.hpp"
+
+namespace swe {
 
 // ---- queue pull ------------------------------------------------------------
 // One fetch per workgroup: thread 0 takes kWavesPerGroup consecutive items
@@ -333,21 +434,21 @@ __global__ __launch_bounds__(256) void batch_apply_kernel(BatchArena A, const Ba
 }
 
 // ---- fused strategy: previous update applied on load ----------------------
-struct BQ3 {
-    float h, hu, hv;
-};
 
 // One strip tile of a block with its pending update (sums sIn, dt) applied as
 // the cells are loaded: the updated state goes to buffer `o` (interior cells,
-// OUTFLOW/WALL ghosts derived from the updated interior, other ghosts copied,
-// corners from the diagonal interior cell), the edges are solved on it and the
-// new sums go to sOut.  The same loads, edge order and arithmetic as the
-// single-block fused kernel, so sums and maxima equal the sweep + apply pair.
+// OUTFLOW/WALL ghosts derived from the updated interior, CONNECT_WITHIN_RANK
+// ghosts from this iteration's ghost-line step (jobs jb, geometry gg of the
+// block's four sides), corners from the diagonal interior cell), the edges are
+// solved on it and the new sums go to sOut.  The same loads, edge order and
+// arithmetic as the single-block fused kernel, so sums and maxima equal the
+// sweep + apply pair.
 template <int S, int XS>
 __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const StateView& v, const StateView& o,
                                             const SumsView& sIn, const SumsView& sOut, bool pend, float dt,
                                             const int* bnd, const TileLanes& t, int blk, uint2* deferQ,
-                                            unsigned* deferCount) {
+                                            unsigned* deferCount, const BatchArena& A, const GhostJob* jb,
+                                            const GhostGeom* gg) {
     const int nx = v.nx, ny = v.ny;
     const float dtdx = dt / v.dx;
     const float dtdy = dt / v.dy;
@@ -404,6 +505,49 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
         o.hv[c] = q.hv;
     };
 
+    // CONNECT_WITHIN_RANK sides: this iteration's ghost-line step, in place
+    // in v before the cells are loaded, each ghost cell by the tile that owns
+    // it: a ghost row's cells by the tile of their column (the lanes spread
+    // over the row's columns), a ghost column's cells by the owned lanes of
+    // their row.  Other tiles may load a ghost cell before or after the
+    // update only as a halo cell, whose edges add neither sums nor speeds
+    // (the speeds of x-edges are taken on owned rows; every interior row is
+    // owned by one strip, so the maximum is that over all edges).
+    bool ghostRows = false;
+    auto row_step = [&](int side, bool isRow) {
+        const GhostJob j = jb[side];
+        if (j.op == 0) return;
+        const GhostGeom g = gg[side];
+        const unsigned long long rows = __ballot(isRow);  // one lane per group of a packed item
+        const int groups = __popcll(rows), row = (side == BND_BOTTOM) ? 0 : ny + 1;
+        for (int p = (int)(threadIdx.x & 63); p < groups * n; p += 64) {
+            const int x = x0 + (p / n) * XS + p % n;
+            const size_t c = (size_t)x * v.pitch + row;
+            const BQ3 q = ghost_element(A, j, g, x - 1, BQ3{v.h[c], v.hu[c], v.hv[c]}, true);
+            v.h[c] = q.h;
+            v.hu[c] = q.hu;
+            v.hv[c] = q.hv;
+        }
+        ghostRows = true;
+    };
+    if (tB == CONNECT_WITHIN_RANK && t.yt == 0) row_step(BND_BOTTOM, rowBottom);
+    if (tT == CONNECT_WITHIN_RANK && kStripRows * t.yt + 63 >= ny + 1) row_step(BND_TOP, rowTop);
+    auto column_step = [&](int side, int x) {
+        const GhostJob j = jb[side];
+        if (j.op == 0 || !owned) return;
+        const size_t c = at(x, rowC);
+        const BQ3 q = ghost_element(A, j, gg[side], y - 1, BQ3{v.h[c], v.hu[c], v.hv[c]}, true);
+        v.h[c] = q.h;
+        v.hu[c] = q.hu;
+        v.hv[c] = q.hv;
+    };
+    if (t.first && tL == CONNECT_WITHIN_RANK) column_step(BND_LEFT, 0);
+    if (t.last && tR == CONNECT_WITHIN_RANK) column_step(BND_RIGHT, nx + 1);
+    if (ghostRows) {  // other lanes of the wave load the row cells
+        __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+
     BQ3 cur = fix_rows(load_updated(x0));
     BQ3 prev = load_updated(x0 - 1);
     if (t.first) prev = fix_column(prev, cur, tL);  // x0 == 1
@@ -416,7 +560,7 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
     if constexpr (S == SOLVER_AUGRIE_DEFER)  // edge (0|1): only the first segment's items have t.first
         e = defer_edge(e, owned && t.first, make_uint2(blkBits, (unsigned)y), deferQ, deferCount);
     float carryH = e.hR, carryHu = e.huR;
-    float smaxv = rowInterior ? e.speed : 0.f;
+    float smaxv = owned ? e.speed : 0.f;
     for (int k = 0; k < n; ++k) {
         const int x = x0 + k;
         const bool rightGhost = t.last && k == n - 1;  // x + 1 == nx + 1
@@ -428,7 +572,7 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
         dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
         if constexpr (S == SOLVER_AUGRIE_DEFER)
             ev = defer_edge(ev, owned, make_uint2(blkBits | ((unsigned)x + xLane), (unsigned)y), deferQ, deferCount);
-        if (rowInterior) smaxv = dev::smax(smaxv, ev.speed);
+        if (owned) smaxv = dev::smax(smaxv, ev.speed);
         dev::EdgeOut eh = solver.solve(cq.yside(), cq.yside_from_lane_above());
         if constexpr (S == SOLVER_AUGRIE_DEFER)
             eh = defer_edge(eh, owned || y == 0, make_uint2(kDeferY | blkBits | ((unsigned)x + xLane), (unsigned)y),
@@ -453,7 +597,10 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
 
 // direct = 0: persistent grid, waves pull items until the queue is drained;
 // direct = 1: one wave per possible item (the grid covers every block's
-// tiles), a wave takes item blockIdx.x * kWavesPerGroup + wave or leaves.
+// tiles and every side's ghost-line step), a wave takes item
+// blockIdx.x * kWavesPerGroup + wave or leaves.  An item is a strip tile of a
+// block that computes this iteration or, for LTS, the ghost-line step of a
+// side of a block that waits (tile kLineTile + side).
 // Seven waves per SIMD (at most 72 VGPRs, no spills) rather than the six the
 // register count would give: an LTS iteration has about one item per wave
 // slot, so more waves in flight hide more latency (measured 2-4% faster; the
@@ -463,7 +610,7 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
 template <int S, int XS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == SOLVER_AUGRIE_MLP ? 1 : 7, 8))) void
 batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, BatchGlobals* G, const int* items,
-                   int direct, uint2* deferQ) {
+                   int direct, uint2* deferQ, const GhostJob* jobs, const GhostGeom* geo) {
     if (G->skip) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -473,6 +620,11 @@ batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, B
         const int code = __builtin_amdgcn_readfirstlane(items[item]);
         const int blk = code >> kTileBits;
         const int tile = code & ((1 << kTileBits) - 1);
+        if (tile >= kLineTile) {  // a waiting block's side
+            const int js = 4 * blk + tile - kLineTile;
+            ghost_line_step(A, jobs[js], geo[js], lane, 64);
+            return;
+        }
         const BatchBlockDesc& d = D[blk];
         const int cur = __builtin_amdgcn_readfirstlane(st[blk].cur);
         const int sidx = __builtin_amdgcn_readfirstlane(st[blk].sidx);
@@ -480,7 +632,8 @@ batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, B
         const float dt = st[blk].maxTimestep;
         const float m = fused_tile<S, XS>(solver, batch_view(A, d, cur), batch_view(A, d, cur ^ 1),
                                           batch_sums(A, d, sidx), batch_sums(A, d, sidx ^ 1), pend, dt, d.bnd,
-                                          tile_lanes(d, tile, lane), blk, deferQ, &G->deferCount);
+                                          tile_lanes(d, tile, lane), blk, deferQ, &G->deferCount, A, jobs + 4 * blk,
+                                          geo + 4 * blk);
         if (lane == 0) atomicMax(&st[blk].smax, __float_as_uint(m));
     };
     if (direct) {

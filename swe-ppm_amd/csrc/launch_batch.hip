@@ -42,17 +42,20 @@ template int launch_batch_sweep_solver<SWE_SOLVER_ID>(int, int, BatchArena, cons
 template <int S>
 int launch_batch_fused_solver(int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
                               BatchBlockState* st, BatchGlobals* G, const int* items, uint2* deferQ,
-                              hipStream_t stream) {
+                              const GhostJob* jobs, const GhostGeom* geo, hipStream_t stream) {
     const dim3 grid(groups), block(64 * kWavesPerGroup);
     switch (xs) {
         case 4:
-            hipLaunchKernelGGL((batch_fused_kernel<S, 4>), grid, block, 0, stream, A, D, st, G, items, direct, deferQ);
+            hipLaunchKernelGGL((batch_fused_kernel<S, 4>), grid, block, 0, stream, A, D, st, G, items, direct, deferQ,
+                               jobs, geo);
             break;
         case 16:
-            hipLaunchKernelGGL((batch_fused_kernel<S, 16>), grid, block, 0, stream, A, D, st, G, items, direct, deferQ);
+            hipLaunchKernelGGL((batch_fused_kernel<S, 16>), grid, block, 0, stream, A, D, st, G, items, direct, deferQ,
+                               jobs, geo);
             break;
         default:
-            hipLaunchKernelGGL((batch_fused_kernel<S, 8>), grid, block, 0, stream, A, D, st, G, items, direct, deferQ);
+            hipLaunchKernelGGL((batch_fused_kernel<S, 8>), grid, block, 0, stream, A, D, st, G, items, direct, deferQ,
+                               jobs, geo);
             break;
     }
     const hipError_t e = hipGetLastError();
@@ -62,7 +65,7 @@ int launch_batch_fused_solver(int xs, int groups, int direct, BatchArena A, cons
 
 template int launch_batch_fused_solver<SWE_SOLVER_ID>(int, int, int, BatchArena, const BatchBlockDesc*,
                                                       BatchBlockState*, BatchGlobals*, const int*, uint2*,
-                                                      hipStream_t);
+                                                      const GhostJob*, const GhostGeom*, hipStream_t);
 
 #if SWE_SOLVER_ID == 5
 namespace {

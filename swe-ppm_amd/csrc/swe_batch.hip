@@ -80,177 +80,47 @@ __global__ __launch_bounds__(kCtlThreads) void batch_init_kernel(BatchArena A, c
 constexpr int kSideChunks = 4;
 
 // ---- lines: the ghost-line data of an iteration ----------------------------
-// Per (block, side, piece) workgroup of a CONNECT_WITHIN_RANK side:
+// Per (block, side, piece) workgroup, the job the previous round (or the
+// start of the run) wrote for the side:
 //   global dt: setGhostLayer's in-rank copy
 //     (SWE_DimensionalSplittingMPIOverdecomp.cpp:273-304): the neighbour's
 //     copy layer, with its pending update applied, into the ghost line;
 //   LTS: the same copy into the side's receive line when the side receives
-//     this iteration (bit `side` of BatchBlockState::recv), then
-//     checkAllGhostlayers' ghost-line step (SWE_Block.hh:375-436): copy when
-//     the time stamps agree, old + (new - old) * (t_me / t_neighbour) when
-//     the neighbour is ahead, nothing when it is behind.
-// Which sides receive, their time stamps and the block's own time were fixed
-// by the previous round (lts_transitions).  Grid (blocks, 4, kSideChunks).
-__global__ __launch_bounds__(kCtlThreads) void batch_lines_kernel(BatchArena A, const BatchBlockDesc* D,
-                                                                  const BatchBlockState* st, const BatchGlobals* G) {
-    if (G->skip) return;
-    const int blk = blockIdx.x, side = blockIdx.y, chunk = blockIdx.z;
-    const BatchBlockDesc& d = D[blk];
-    if (d.bnd[side] != CONNECT_WITHIN_RANK) return;
-    const BatchBlockState& s = st[blk];
-    const bool lts = G->mode == BATCH_LTS;
-    const bool receive = !lts || ((s.recv >> side) & 1);
-    int act = 0;  // LTS: 1 copy, 2 interpolate
-    float w = 0.f;
-    if (lts) {
-        const float bt = s.borderTs[side], tot = s.total;
-        if ((double)fabsf(bt - tot) <= DBL_EPSILON) {
-            act = 1;
-        } else if (bt > tot) {
-            act = 2;
-            w = tot / bt;  // interpolation weight t_me / t_neighbour
-        }
-        if (!receive && !act) return;
-    }
-    const int nb = d.nbr[side];
-    const BatchBlockDesc& dn = D[nb];
-    const BatchBlockState& sn = st[nb];
-    long long so, go;
-    int ss, gs, n, nn;
-    side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
-    side_line(d, side, 1, go, gs, n);        // my ghost line
-    // the neighbour's current state: its buffer, with a pending update applied
-    const int ncur = sn.cur;
-    const bool npend = sn.pend != 0;
-    const float ndt = sn.maxTimestep;
-    const float ndtdx = ndt / dn.dx, ndtdy = ndt / dn.dy;
-    const float* sh = A.h[ncur] + dn.off + so;
-    const float* shu = A.hu[ncur] + dn.off + so;
-    const float* shv = A.hv[ncur] + dn.off + so;
-    const SumsView ps = batch_sums(A, dn, sn.sidx);
-    auto value = [&](int i) -> Cell {
-        const long long c = so + (long long)i * ss;
-        Cell q{sh[(long long)i * ss], shu[(long long)i * ss], shv[(long long)i * ss]};
-        if (npend) q = update_cell(q.h, q.hu, q.hv, ndtdx, ndtdy, ps.sx[c], ps.sy[c], ps.su[c], ps.sv[c]);
-        return q;
-    };
-    float* f[3] = {A.h[s.cur] + d.off + go, A.hu[s.cur] + d.off + go, A.hv[s.cur] + d.off + go};
-    const int i0 = chunk * blockDim.x + threadIdx.x, stride = blockDim.x * gridDim.z;
-    if (!lts) {
-        for (int i = i0; i < n; i += stride) {
-            const Cell q = value(i);
-            f[0][(long long)i * gs] = q.h;
-            f[1][(long long)i * gs] = q.hu;
-            f[2][(long long)i * gs] = q.hv;
-        }
-        return;
-    }
-    float* L = A.lines + d.line[side];
-    for (int i = i0; i < n; i += stride) {
-        float nv[3];
-        if (receive) {
-            const Cell q = value(i);
-            L[i] = nv[0] = q.h;
-            L[n + i] = nv[1] = q.hu;
-            L[2 * n + i] = nv[2] = q.hv;
-        } else {
-            for (int c = 0; c < 3; ++c) nv[c] = L[c * n + i];
-        }
-        if (act)
-            for (int c = 0; c < 3; ++c) {
-                float& g = f[c][(long long)i * gs];
-                g = (act == 1) ? nv[c] : (g + (nv[c] - g) * w);
-            }
-    }
-}
-
-// ---- LTS bookkeeping of one block at the start of an iteration -------------
-// All that setGhostLayer / receiveGhostLayer and checkAllGhostlayers decide
-// at the start of an iteration follows from the blocks' times at the end of
-// the previous one (BatchBlockState::total): which sides receive and the
-// neighbour's time stamp they take (SWE_Block.hh:311-374), then the
-// ghost-layer states, notifiedLastTimestep, the sub-step counter and whether
-// the block is in sync (SWE_Block.hh:375-436).  The lines launch moves the
-// data afterwards.
-template <typename Desc, typename NeighbourTotal>
-__device__ void lts_transitions(const Desc& d, BatchBlockState& s, const NeighbourTotal& neighbourTotal,
-                                float duration) {
-    const float tot = s.total;
-    int recv = 0;
-    for (int side = 0; side < 4; ++side) {
-        if (d.bnd[side] != CONNECT_WITHIN_RANK || !lts_receivable(s, side)) continue;
-        recv |= 1 << side;
-        s.borderTs[side] = neighbourTotal(d.nbr[side]);
-    }
-    s.recv = recv;
-    for (int side = 0; side < 4; ++side) {
-        if (d.bnd[side] != CONNECT && d.bnd[side] != CONNECT_WITHIN_RANK) continue;
-        const float bt = s.borderTs[side];
-        if ((double)fabsf(bt - tot) <= DBL_EPSILON)
-            s.gl[side] = GL_SYNC;
-        else if (bt > tot)
-            s.gl[side] = GL_INTER;
-        else
-            s.gl[side] = GL_UNVALID;
-        if (bt >= duration) s.gl[side] = GL_DONE;
-    }
-    if ((double)fabsf(duration - tot) <= DBL_EPSILON || tot >= duration) s.notified = 1;
-    if (lts_in_sync(s))
-        for (int side = 0; side < 4; ++side)
-            if (s.gl[side] != GL_DONE) s.gl[side] = (s.gl[side] == GL_INTER) ? GL_INTER : GL_NEXT;
-    if (lts_in_sync(s) && s.stepSizeCounter < s.stepSize) s.stepSizeCounter++;
-    s.active = lts_in_sync(s) ? 1 : 0;
-}
-
-// exclusive prefix sum over the workgroup (all threads, full waves)
-__device__ int wg_exclusive_scan(int v, int& total) {
-    __shared__ int sWave[16];  // up to 1024 threads
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) sWave[wave] = x;
-    __syncthreads();
-    int before = 0, all = 0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-        const int t = sWave[k];
-        if (k < wave) before += t;
-        all += t;
-    }
-    __syncthreads();
-    total = all;
-    return before + x - v;
+//     this iteration, then checkAllGhostlayers' ghost-line step
+//     (SWE_Block.hh:375-436): copy when the time stamps agree,
+//     old + (new - old) * (t_me / t_neighbour) when the neighbour is ahead,
+//     nothing when it is behind.
+// Grid (blocks, 4, kSideChunks).
+__global__ __launch_bounds__(kCtlThreads) void batch_lines_kernel(BatchArena A, const GhostGeom* geo,
+                                                                  const GhostJob* jobs) {
+    const int js = 4 * blockIdx.x + blockIdx.y;
+    const GhostJob j = jobs[js];
+    if (j.op == 0) return;
+    ghost_line_step(A, j, geo[js], blockIdx.z * blockDim.x + threadIdx.x, blockDim.x * gridDim.z);
 }
 
 // ---- work queue of an iteration: the tiles of every active block -----------
 // One workgroup: a prefix sum over the blocks gives each block's first item,
 // then each wave writes the items of every fourth block.
 __device__ void fill_queue(const BatchBlockDesc* D, const BatchBlockState* st, BatchGlobals* G, int* items,
-                           bool all) {
+                           bool all, const int* lineMask = nullptr) {
     __shared__ int sBase[kMaxBatchBlocks];
     const int nb = G->nblocks;
     const int per = (nb + blockDim.x - 1) / blockDim.x;
     const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
-    auto queued = [&](int b) { return (all || st[b].active) ? D[b].ntiles : 0; };
+    auto active = [&](int b) { return all || st[b].active != 0; };
+    auto mask = [&](int b) { return lineMask ? lineMask[b] : 0; };
     int mine = 0;
-    for (int b = b0; b < b1; ++b) mine += queued(b);
+    for (int b = b0; b < b1; ++b) mine += queue_entries(active(b), D[b].ntiles, mask(b));
     int total;
-    int run = wg_exclusive_scan(mine, total);
+    int run = wg_exclusive_scan<kCtlThreads>(mine, total);
     for (int b = b0; b < b1; ++b) {
-        const int q = queued(b);
-        sBase[b] = q ? run : -1;
-        run += q;
+        sBase[b] = run;
+        run += queue_entries(active(b), D[b].ntiles, mask(b));
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-    for (int b = wave; b < nb; b += waves) {
-        const int base = sBase[b];
-        if (base < 0) continue;
-        const int nt = D[b].ntiles;
-        for (int i = lane; i < nt; i += 64) items[base + i] = (b << kTileBits) | i;
-    }
+    for (int b = wave; b < nb; b += waves) queue_write(items, sBase[b], b, active(b), D[b].ntiles, mask(b), lane);
     if (threadIdx.x == 0) {
         G->count = total;
         G->head = 0;
@@ -271,178 +141,11 @@ __device__ void next_queue(const BatchBlockDesc* D, const BatchBlockState* st, B
     if (threadIdx.x == 0) G->skip = G->stop;
 }
 
-// ---- round: per-block dt (SWE_DimensionalSplittingMpi.cpp:448-463), the
-// global minimum (swe_mpi_overdecomp.cpp:297-307), the completion test
-// (hasMaxLocalTimestep, SWE_Block.hh:355-358, swe_mpi_overdecomp.cpp:327-342)
-// and resetStepSizeCounter (SWE_Block.hh:530-535); then every block's time,
-// the next iteration's LTS bookkeeping and, with the fused strategy, its
-// queue.  One workgroup of kRoundThreads; each thread keeps the state of its
-// blocks (blk = thread + r * kRoundThreads) in registers from one load to one
-// store, and what crosses blocks (neighbour times, queue offsets) goes
-// through LDS: the kernel is a short chain of dependent accesses.
-constexpr int kRoundThreads = 1024;
-constexpr int kRoundPer = (kMaxBatchBlocks + kRoundThreads - 1) / kRoundThreads;
-
-struct RoundDesc {
-    float dx, dy;
-    int bnd[4], nbr[4];
-    int ntiles;
-};
-
+// ---- round: one workgroup after the sweep (round_body) -----------------------
 __global__ __launch_bounds__(kRoundThreads) void batch_round_kernel(const BatchBlockDesc* D, BatchBlockState* st,
-                                                                    BatchGlobals* G, int* items, float* dtLog,
-                                                                    int* iterLog, int* hostStat) {
+                                                                    BatchGlobals* G, RoundIO io) {
     if (G->skip) return;
-    if (threadIdx.x == 0) G->deferCount = 0u;  // the deferred edges of this iteration are solved
-    const int mode = G->mode;
-    const float T = G->T;
-    const int nb = G->nblocks;
-    const bool fused = G->fused != 0;
-    const float duration = G->duration;
-    __shared__ unsigned int sminBits;
-    __shared__ int allSynced, macroEnd;
-    __shared__ float sTotal[kMaxBatchBlocks];
-    __shared__ int sBase[kMaxBatchBlocks];
-    __shared__ int sTiles[kMaxBatchBlocks];
-    BatchBlockState S[kRoundPer];
-    RoundDesc R[kRoundPer];
-#pragma unroll
-    for (int r = 0; r < kRoundPer; ++r) {
-        const int blk = threadIdx.x + r * kRoundThreads;
-        if (blk >= nb) continue;
-        S[r] = st[blk];
-        const BatchBlockDesc& d = D[blk];
-        R[r].dx = d.dx;
-        R[r].dy = d.dy;
-        for (int k = 0; k < 4; ++k) {
-            R[r].bnd[k] = d.bnd[k];
-            R[r].nbr[k] = d.nbr[k];
-        }
-        R[r].ntiles = d.ntiles;
-    }
-    if (threadIdx.x == 0) {
-        sminBits = __float_as_uint(FLT_MAX);
-        allSynced = 1;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRoundPer; ++r) {
-        if (threadIdx.x + r * kRoundThreads >= nb) continue;
-        BatchBlockState& s = S[r];
-        if (s.active) {
-            float dt = timestep_from_max(__uint_as_float(s.smax), R[r].dx, R[r].dy);
-            if (mode == BATCH_LTS) {
-                if (s.stepSizeCounter <= 0) s.stepSize = lts_step_size(T, dt);
-                dt = T / s.stepSize;
-            }
-            if (mode == BATCH_ENSEMBLE) s.t += dt;
-            s.updates += 1;
-            s.maxTimestep = dt;
-            if (mode == BATCH_GLOBAL) atomicMin(&sminBits, __float_as_uint(dt));  // dt > 0: bits order as floats
-        }
-        s.smax = 0u;
-        if (mode == BATCH_LTS) {
-            const bool has = (s.stepSizeCounter >= s.stepSize && lts_in_sync(s)) || s.notified;
-            if (!has) atomicAnd(&allSynced, 0);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        G->iterations += 1;
-        G->itersUnit += 1;
-        if (mode == BATCH_GLOBAL) G->gdt = __uint_as_float(sminBits);
-        const bool unitDone = (mode != BATCH_LTS) || allSynced;
-        if (unitDone) {
-            const int u = G->done;
-            if (u < G->logCap) {
-                dtLog[u] = (mode == BATCH_GLOBAL) ? __uint_as_float(sminBits) : 0.f;
-                iterLog[u] = G->itersUnit;
-            }
-            G->done = u + 1;
-            G->itersUnit = 0;
-            if (u + 1 >= G->target) G->stop = 1;
-        }
-        macroEnd = (mode == BATCH_LTS) && allSynced;
-        hostStat[0] = G->stop;
-        hostStat[1] = G->done;
-        hostStat[2] = G->itersUnit;
-    }
-    __syncthreads();
-    const float gdt = __uint_as_float(sminBits);
-#pragma unroll
-    for (int r = 0; r < kRoundPer; ++r) {
-        const int blk = threadIdx.x + r * kRoundThreads;
-        if (blk >= nb) continue;
-        BatchBlockState& s = S[r];
-        if (s.active) {
-            // every block of the global loop updates with the minimum (swe_mpi_overdecomp.cpp:305)
-            if (mode == BATCH_GLOBAL) s.maxTimestep = gdt;
-            // fused strategy: the update is applied by the block's next load, into the other buffer
-            if (fused) {
-                s.cur ^= 1;
-                s.sidx ^= 1;
-                s.pend = 1;
-            }
-        }
-        if (macroEnd) {
-            s.stepSizeCounter = 0;
-            s.stepSize = 0;
-            s.timestepCounter += 1;
-        }
-        if (mode == BATCH_LTS) sTotal[blk] = s.total = lts_total(T, s);
-    }
-    if (mode == BATCH_LTS) {
-        __syncthreads();  // every block's total is in LDS
-#pragma unroll
-        for (int r = 0; r < kRoundPer; ++r)
-            if (threadIdx.x + r * kRoundThreads < nb)
-                lts_transitions(R[r], S[r], [&](int k) { return sTotal[k]; }, duration);
-    }
-#pragma unroll
-    for (int r = 0; r < kRoundPer; ++r) {
-        const int blk = threadIdx.x + r * kRoundThreads;
-        if (blk < nb) st[blk] = S[r];
-    }
-    if (!fused) return;  // the store strategy's queue is built after its apply launch
-    if (mode != BATCH_LTS) {  // every block computes every iteration: the queue stays as filled
-        if (threadIdx.x == 0) {
-            G->count = G->allItems;
-            G->head = 0;
-            G->headApply = 0;
-            G->skip = G->stop;
-        }
-        return;
-    }
-    // the next queue: the tiles of the blocks in sync, block by block
-    int mine = 0;
-#pragma unroll
-    for (int r = 0; r < kRoundPer; ++r)
-        if (threadIdx.x + r * kRoundThreads < nb && S[r].active) mine += R[r].ntiles;
-    int total;
-    int run = wg_exclusive_scan(mine, total);
-#pragma unroll
-    for (int r = 0; r < kRoundPer; ++r) {
-        const int blk = threadIdx.x + r * kRoundThreads;
-        if (blk >= nb) continue;
-        const bool q = S[r].active != 0;
-        sBase[blk] = q ? run : -1;
-        sTiles[blk] = R[r].ntiles;
-        if (q) run += R[r].ntiles;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-    for (int b = wave; b < nb; b += waves) {
-        const int base = sBase[b];
-        if (base < 0) continue;
-        const int nt = sTiles[b];
-        for (int i = lane; i < nt; i += 64) items[base + i] = (b << kTileBits) | i;
-    }
-    if (threadIdx.x == 0) {
-        G->count = total;
-        G->head = 0;
-        G->headApply = 0;
-        G->skip = G->stop;
-    }
+    round_body<kRoundThreads, kMaxBatchBlocks>(threadIdx.x, D, st, G, io);
 }
 
 // ---- store strategy: the next queue, after the apply launch ------------------
@@ -548,11 +251,12 @@ __global__ void batch_flags_kernel(BatchBlockState* st, int nblocks) {
     }
 }
 
-// start of a run: `target` more units, the queue of the first iteration (the
-// tiling may have changed since the last run).  One workgroup.
+// start of a run: `target` more units, the queue and the ghost-line jobs of
+// the first iteration (the tiling or the block states may have changed since
+// the last run).  One workgroup.
 __global__ __launch_bounds__(kCtlThreads) void batch_begin_kernel(const BatchBlockDesc* D, const BatchBlockState* st,
                                                                   BatchGlobals* G, int* items, int target, int logCap,
-                                                                  int* hostStat) {
+                                                                  int* hostStat, GhostJob* jobs) {
     if (threadIdx.x == 0) {
         G->target = target;
         G->done = 0;
@@ -564,7 +268,26 @@ __global__ __launch_bounds__(kCtlThreads) void batch_begin_kernel(const BatchBlo
         hostStat[1] = 0;
         hostStat[2] = 0;
     }
-    fill_queue(D, st, G, items, G->mode != BATCH_LTS);
+    // the jobs; with the fused strategy, LTS queues the steps of the waiting blocks' sides
+    __shared__ int sMask[kMaxBatchBlocks];
+    const int mode = G->mode;
+    const bool fusedLts = mode == BATCH_LTS && G->fused != 0;
+    for (int blk = threadIdx.x; blk < G->nblocks; blk += blockDim.x) {
+        int m = 0;
+        if (mode != BATCH_ENSEMBLE) {
+            for (int side = 0; side < 4; ++side) {
+                const int bnd = D[blk].bnd[side], nb = D[blk].nbr[side];
+                const bool link = bnd == CONNECT_WITHIN_RANK;
+                const GhostJob j = line_job(st[blk], side, bnd, mode == BATCH_LTS, target <= 0,
+                                            link ? line_flags(st[nb]) : 0, link ? st[nb].maxTimestep : 0.f);
+                jobs[4 * blk + side] = j;
+                if (j.op != 0 && !st[blk].active && fusedLts) m |= 1 << side;
+            }
+        }
+        sMask[blk] = m;
+    }
+    __syncthreads();
+    fill_queue(D, st, G, items, mode != BATCH_LTS, sMask);
 }
 
 template <int XS>
@@ -576,22 +299,27 @@ int launch_apply(int groups, BatchArena A, const BatchBlockDesc* D, const BatchB
 }
 
 int launch_fused(int solver, int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
-                 BatchBlockState* st, BatchGlobals* G, const int* items, uint2* deferQ, hipStream_t stream) {
+                 BatchBlockState* st, BatchGlobals* G, const int* items, uint2* deferQ, const GhostJob* jobs,
+                 const GhostGeom* geo, hipStream_t stream) {
     switch (solver) {
         case SOLVER_HLLE:
-            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
+            return launch_batch_fused_solver<SOLVER_HLLE>(xs, groups, direct, A, D, st, G, items, nullptr, jobs,
+                                                          geo, stream);
         case SOLVER_FWAVE:
-            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
+            return launch_batch_fused_solver<SOLVER_FWAVE>(xs, groups, direct, A, D, st, G, items, nullptr, jobs,
+                                                           geo, stream);
         case SOLVER_AUGRIE:
-            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE>(xs, groups, direct, A, D, st, G, items, nullptr, jobs,
+                                                            geo, stream);
         case SOLVER_AUGRIE_MLP:
-            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, direct, A, D, st, G, items, nullptr,
-                                                                stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE_MLP>(xs, groups, direct, A, D, st, G, items, nullptr, jobs,
+                                                                geo, stream);
         case SOLVER_AUGRIE_DEFER:
-            return launch_batch_fused_solver<SOLVER_AUGRIE_DEFER>(xs, groups, direct, A, D, st, G, items, deferQ,
-                                                                  stream);
+            return launch_batch_fused_solver<SOLVER_AUGRIE_DEFER>(xs, groups, direct, A, D, st, G, items, deferQ, jobs,
+                                                                  geo, stream);
         default:
-            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, direct, A, D, st, G, items, nullptr, stream);
+            return launch_batch_fused_solver<SOLVER_HYBRID>(xs, groups, direct, A, D, st, G, items, nullptr, jobs,
+                                                            geo, stream);
     }
 }
 
@@ -626,13 +354,16 @@ struct swe_hip_batch {
     BatchBlockState* dState = nullptr;
     BatchGlobals* dG = nullptr;
     int* dItems = nullptr;
+    GhostGeom* dGeom = nullptr;  // 4 per block, fixed at create
+    GhostJob* dJobs = nullptr;   // 4 per block, written by the round and the start of a run
     float* dDtLog = nullptr;
     int* dIterLog = nullptr;
     int logCap = 0;
     int* hostStat = nullptr;  // pinned, written by the round kernel
     int* dHostStat = nullptr;
     int xs = 8, groups = 0, chunk = 16, useGraph = 1;
-    long long itemsAll = 0;  // work items of all blocks at the current xs
+    long long itemsAll = 0;   // work items of all blocks at the current xs
+    long long itemsMost = 0;  // the longest queue an iteration can have (the direct grid)
     int cus = 256;
     hipStream_t stream = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -651,7 +382,7 @@ void free_graph(swe_hip_batch* k) {
 }
 
 int tiles_for(swe_hip_batch* k, int xs, bool upload) {
-    long long count = 0;
+    long long count = 0, most = 0;
     for (auto& d : k->desc) {
         const StripTiling g = strip_tiling(d.nx, d.ny, xs);
         d.nyt = g.nyt;
@@ -659,10 +390,13 @@ int tiles_for(swe_hip_batch* k, int xs, bool upload) {
         d.pack = g.pack;
         d.ntiles = g.items;
         count += d.ntiles;
+        // an LTS queue holds a block's tiles or the steps of its four sides
+        most += (k->mode == BATCH_LTS) ? std::max(d.ntiles, 4) : d.ntiles;
     }
     k->itemsAll = count;
-    if (count > k->itemCap) {
-        set_error("batch: %lld work items exceed the queue (%d)", count, k->itemCap);
+    k->itemsMost = most;
+    if (most > k->itemCap) {
+        set_error("batch: %lld work items exceed the queue (%d)", most, k->itemCap);
         return SWE_HIP_EINVAL;
     }
     if (upload) {
@@ -683,9 +417,10 @@ int default_groups(swe_hip_batch* k) {
 // the launches of one driver iteration
 int enqueue_iteration(swe_hip_batch* k) {
     const dim3 ctl(kCtlThreads);
-    if (k->mode != BATCH_ENSEMBLE)
-        hipLaunchKernelGGL(batch_lines_kernel, dim3(k->nblocks, 4, kSideChunks), ctl, 0, k->stream, k->A, k->dDesc,
-                           k->dState, k->dG);
+    // the fused sweep does the ghost-line steps itself
+    if (k->mode != BATCH_ENSEMBLE && !k->fused)
+        hipLaunchKernelGGL(batch_lines_kernel, dim3(k->nblocks, 4, kSideChunks), ctl, 0, k->stream, k->A, k->dGeom,
+                           k->dJobs);
     const int solver = (k->mlp && k->solver == SOLVER_AUGRIE) ? SOLVER_AUGRIE_MLP : k->solver;
     // the learned middle state's fused sweep defers its hard edges to a second launch
     const bool defer = k->fused && solver == SOLVER_AUGRIE_MLP && k->deferQ;
@@ -693,10 +428,10 @@ int enqueue_iteration(swe_hip_batch* k) {
     // sweep; the store strategy's persistent grids then take the default size
     const bool direct = k->groups == 0;
     const int persistent = direct ? default_groups(k) : k->groups;
-    const int directGrid = (int)((k->itemsAll + kWavesPerGroup - 1) / kWavesPerGroup);
+    const int directGrid = (int)((k->itemsMost + kWavesPerGroup - 1) / kWavesPerGroup);
     int rc = k->fused ? launch_fused(defer ? SOLVER_AUGRIE_DEFER : solver, k->xs,
                                      direct ? std::max(1, directGrid) : k->groups, direct ? 1 : 0, k->A, k->dDesc,
-                                     k->dState, k->dG, k->dItems, k->deferQ, k->stream)
+                                     k->dState, k->dG, k->dItems, k->deferQ, k->dJobs, k->dGeom, k->stream)
                       : launch_sweep(solver, k->xs, persistent, k->A, k->dDesc, k->dState, k->dG, k->dItems, k->stream);
     if (rc) return rc;
     if (defer) {
@@ -705,8 +440,9 @@ int enqueue_iteration(swe_hip_batch* k) {
                                       k->stream);
         if (rc) return rc;
     }
+    const RoundIO io{k->dItems, k->dDtLog, k->dIterLog, k->dHostStat, k->dJobs};
     hipLaunchKernelGGL(batch_round_kernel, dim3(1), dim3(kRoundThreads), 0, k->stream, k->dDesc, k->dState, k->dG,
-                       k->dItems, k->dDtLog, k->dIterLog, k->dHostStat);
+                       io);
     if (!k->fused) {
         switch (k->xs) {
             case 4:
@@ -786,7 +522,7 @@ int ensure_log(swe_hip_batch* k, int units) {
 // Advance until `units` are done; returns the iterations replayed.
 int advance(swe_hip_batch* k, int units, long long maxIter, long long* itersOut) {
     hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(kCtlThreads), 0, k->stream, k->dDesc, k->dState, k->dG,
-                       k->dItems, units, k->logCap, k->dHostStat);
+                       k->dItems, units, k->logCap, k->dHostStat, k->dJobs);
     HIP_TRY(hipGetLastError());
     long long iters = 0;
     if (k->mode != BATCH_LTS) {
@@ -885,11 +621,11 @@ int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float*
         off += (long long)(nx[k] + 2) * (ny[k] + 2);
         const int nyt = (ny[k] + kStripRows - 1) / kStripRows;
         const long long tiles4 = (long long)nyt * ((nx[k] + 3) / 4);
-        if (tiles4 >= (1LL << kTileBits)) {
+        if (tiles4 > kLineTile) {
             set_error("swe_hip_batch_create: block %d too large for the work queue", k);
             return SWE_HIP_EINVAL;
         }
-        items += tiles4;
+        items += (mode == BATCH_LTS) ? std::max<long long>(tiles4, 4) : tiles4;  // LTS: or its sides' steps
         for (int s = 0; s < 4; ++s) {
             const int t = bnd[4 * k + s];
             if (t != OUTFLOW && t != WALL && t != CONNECT_WITHIN_RANK && t != PASSIVE) {
@@ -956,6 +692,36 @@ int swe_hip_batch_create(int nblocks, const int* nx, const int* ny, const float*
     if (e == hipSuccess) e = hipMalloc(&k->dState, sizeof(BatchBlockState) * nblocks);
     if (e == hipSuccess) e = hipMalloc(&k->dG, sizeof(BatchGlobals));
     if (e == hipSuccess) e = hipMalloc(&k->dItems, sizeof(int) * (size_t)std::max(k->itemCap, 1));
+    if (e == hipSuccess) e = hipMalloc(&k->dGeom, sizeof(GhostGeom) * 4 * (size_t)nblocks);
+    if (e == hipSuccess) e = hipMalloc(&k->dJobs, sizeof(GhostJob) * 4 * (size_t)nblocks);
+    if (e == hipSuccess) e = hipMemsetAsync(k->dJobs, 0, sizeof(GhostJob) * 4 * (size_t)nblocks, k->stream);
+    if (e == hipSuccess) {
+        // the fixed part of every side's ghost-line job
+        std::vector<GhostGeom> geo(4 * (size_t)nblocks);
+        for (int b = 0; b < nblocks; ++b) {
+            const BatchBlockDesc& d = desc[b];
+            for (int side = 0; side < 4; ++side) {
+                GhostGeom& g = geo[4 * (size_t)b + side];
+                std::memset(&g, 0, sizeof(g));
+                if (d.bnd[side] != CONNECT_WITHIN_RANK) continue;
+                const BatchBlockDesc& dn = desc[d.nbr[side]];
+                long long so, go;
+                int ss, gs, n, nn;
+                side_line(dn, side ^ 1, 0, so, ss, nn);  // neighbour's copy layer at the shared edge
+                side_line(d, side, 1, go, gs, n);        // the block's ghost line
+                g.src = dn.off + so;
+                g.dst = d.off + go;
+                g.line = d.line[side];
+                g.n = n;
+                g.srcStride = ss;
+                g.dstStride = gs;
+                g.ndx = dn.dx;
+                g.ndy = dn.dy;
+            }
+        }
+        e = hipMemcpyAsync(k->dGeom, geo.data(), sizeof(GhostGeom) * geo.size(), hipMemcpyHostToDevice, k->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(k->stream);
+    }
     if (e == hipSuccess) e = hipHostMalloc((void**)&k->hostStat, 4 * sizeof(int), hipHostMallocMapped);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&k->dHostStat, k->hostStat, 0);
     if (e == hipSuccess) {
@@ -993,6 +759,8 @@ int swe_hip_batch_destroy(swe_hip_batch* k) {
     if (k->dState) (void)hipFree(k->dState);
     if (k->dG) (void)hipFree(k->dG);
     if (k->dItems) (void)hipFree(k->dItems);
+    if (k->dGeom) (void)hipFree(k->dGeom);
+    if (k->dJobs) (void)hipFree(k->dJobs);
     if (k->deferQ) (void)hipFree(k->deferQ);
     if (k->dDtLog) (void)hipFree(k->dDtLog);
     if (k->dIterLog) (void)hipFree(k->dIterLog);
@@ -1235,7 +1003,7 @@ int swe_hip_batch_autotune(swe_hip_batch* k, int iterations, float* cand_ms, int
             if (rc) break;
             // warm-up chunk (also captures the graph), then the timed chunks
             hipLaunchKernelGGL(batch_begin_kernel, dim3(1), dim3(kCtlThreads), 0, k->stream, k->dDesc, k->dState,
-                               k->dG, k->dItems, units, k->logCap, k->dHostStat);
+                               k->dG, k->dItems, units, k->logCap, k->dHostStat, k->dJobs);
             rc = enqueue_chunk(k);
             if (rc) break;
             e = hipEventRecord(e0, k->stream);

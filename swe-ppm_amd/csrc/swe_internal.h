@@ -118,10 +118,11 @@ int launch_sweep_s_solver(int dir, StateView v, SumsView sums, DeviceCtl* ctl, i
 template <int S>
 int launch_batch_sweep_solver(int xs, int groups, BatchArena A, const BatchBlockDesc* D, BatchBlockState* st,
                               BatchGlobals* G, const int* items, hipStream_t stream);
+// The fused sweep also does the ghost-line steps (jobs, geometry: 4 per block).
 template <int S>
 int launch_batch_fused_solver(int xs, int groups, int direct, BatchArena A, const BatchBlockDesc* D,
                               BatchBlockState* st, BatchGlobals* G, const int* items, uint2* deferQ,
-                              hipStream_t stream);
+                              const GhostJob* jobs, const GhostGeom* geo, hipStream_t stream);
 // The edges the deferring batched sweep queued (G->deferCount of them).
 int launch_batch_defer_solve(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, BatchGlobals* G,
                              const uint2* queue, unsigned cap, int groups, hipStream_t stream);
