@@ -269,12 +269,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves
     // kQuiet* bits of the record tile's parts whose owned sums were not all +0
     unsigned loudParts = 0u;
 
+    // Column x+1's loads are issued one column ahead (a software pipeline):
+    // their latency overlaps column x's edges.  The loads past x1 + 1 read
+    // column nx + 1 at most and are not used.  Not for the learned middle
+    // state, whose network call leaves no registers for them.
+    struct Raw {
+        float h, hu, hv, sx, sy, su, sv, b;
+    };
+    auto load_raw = [&](int x) -> Raw {
+        const size_t c = idx(v, min(x, nx + 1), yc);
+        return Raw{io.h[c], io.hu[c], io.hv[c], io.sIn.sx[c], io.sIn.sy[c], io.sIn.su[c], io.sIn.sv[c], B[c]};
+    };
+    constexpr bool kAhead = S != SOLVER_AUGRIE_MLP && S != SOLVER_HYBRID && S != SOLVER_AUGRIE_DEFER;
+    Raw ahead{};
+    if constexpr (kAhead) ahead = load_raw(x0 + 1);
     for (int x = x0; x <= x1; ++x) {
         // column x+1 of Q(k)
-        Q3 nxt = fix_rows(load_updated(x + 1));
+        const Raw r = kAhead ? ahead : load_raw(x + 1);
+        if constexpr (kAhead) ahead = load_raw(x + 2);
+        Q3 nxt{r.h, r.hu, r.hv};
+        if (x + 1 <= nx && rowInterior) {
+            const Cell u = update_cell(nxt.h, nxt.hu, nxt.hv, dtdx, dtdy, r.sx, r.sy, r.su, r.sv);
+            nxt = Q3{u.h, u.hu, u.hv};
+        }
+        nxt = fix_rows(nxt);
         if (x + 1 == nx + 1) nxt = fix_column(nxt, cur, tR);
-        const float bn = B[idx(v, x + 1, yc)];
-        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, bn);
+        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, r.b);
 
         if (carryRow) store(x, cur);
         if (x + 1 == nx + 1 && carryRow) store(nx + 1, nxt);

@@ -22,16 +22,18 @@ constexpr int SOLVER_AUGRIE_DEFER = 5;
 // (its amdgpu_waves_per_eu floor; the build fails where the compiler cannot
 // meet it).  A SIMD lane has 512 registers: 7 waves need at most 72 per lane,
 // 6 at most 80.  Hybrid and the deferring learned sweep fit 7 (71-72
-// registers, no spills).  AugRie's fused kernel keeps a quiet-tile record,
-// which at 7 waves spills 7 values to scratch; at 6 it takes 79 registers
-// without spills and runs the 2048^2 islands 0.5%, the disturbed islands
-// 2.8% and the 4096^2 trench 0.9% faster (5 waves: 81 registers, slower).
-// HLLE (79) and f-wave (80 at this target, 84 without it) would spill at 7;
-// the in-sweep network call holds 136 vector and 32 accumulation registers
-// (3 waves).
+// registers, no spills).  AugRie's fused kernel keeps a quiet-tile record
+// and loads each column one column ahead (8 registers): 86 registers at 5
+// waves.  Measured on the 2048^2 islands against 6 waves without the early
+// loads (79 registers; the loads spill at 6): 2.5% faster with quiet tiles
+// skipped, 12% faster with every tile computed, 2.6% slower on the disturbed
+// islands, where every edge runs the full solver.  HLLE (77) and f-wave (78)
+// keep 6 with the early loads; the in-sweep network call holds 136 vector
+// and 32 accumulation registers (3 waves).
 constexpr int fused_waves_per_simd(int solver) {
     return (solver == SOLVER_HYBRID || solver == SOLVER_AUGRIE_DEFER) ? 7
            : solver == SOLVER_AUGRIE_MLP                             ? 3
+           : solver == SOLVER_AUGRIE                                 ? 5
                                                                      : 6;
 }
 
