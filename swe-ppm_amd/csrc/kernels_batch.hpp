@@ -561,12 +561,33 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
         e = defer_edge(e, owned && t.first, make_uint2(blkBits, (unsigned)y), deferQ, deferCount);
     float carryH = e.hR, carryHu = e.huR;
     float smaxv = owned ? e.speed : 0.f;
+    // Column x+1's loads are issued one column ahead (a software pipeline),
+    // as in the single-block fused kernel; past the tile they read column
+    // nx + 1 at most (a packed item's segments are inner ones) and are not
+    // used.  Not for the deferring sweep, which has no registers to spare.
+    struct Raw {
+        float h, hu, hv, sx, sy, su, sv, b;
+    };
+    auto load_raw = [&](int x) -> Raw {
+        const size_t c = at(min(x, nx + 1), rowC);
+        return Raw{v.h[c], v.hu[c], v.hv[c], sIn.sx[c], sIn.sy[c], sIn.su[c], sIn.sv[c], v.b[c]};
+    };
+    constexpr bool kAhead = S != SOLVER_AUGRIE_DEFER;
+    Raw ahead{};
+    if constexpr (kAhead) ahead = load_raw(x0 + 1);
     for (int k = 0; k < n; ++k) {
         const int x = x0 + k;
         const bool rightGhost = t.last && k == n - 1;  // x + 1 == nx + 1
-        BQ3 nxt = fix_rows(load_updated(x + 1));
+        const Raw r = kAhead ? ahead : load_raw(x + 1);
+        if constexpr (kAhead) ahead = load_raw(x + 2);
+        BQ3 nxt{r.h, r.hu, r.hv};
+        if (pend && x + 1 <= nx && rowInterior) {
+            const Cell u = update_cell(nxt.h, nxt.hu, nxt.hv, dtdx, dtdy, r.sx, r.sy, r.su, r.sv);
+            nxt = BQ3{u.h, u.hu, u.hv};
+        }
+        nxt = fix_rows(nxt);
         if (rightGhost) nxt = fix_column(nxt, cur, tR);
-        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, v.b[at(x + 1, rowC)]);
+        const CellQ nq = cell_q(nxt.h, nxt.hu, nxt.hv, r.b);
         if (carryRow) store(x, cur);
         if (rightGhost && carryRow) store(nx + 1, nxt);
         dev::EdgeOut ev = solver.solve(cq.xside(), nq.xside());
@@ -601,14 +622,13 @@ __device__ __forceinline__ float fused_tile(const BatchSolver<S>& solver, const 
 // blockIdx.x * kWavesPerGroup + wave or leaves.  An item is a strip tile of a
 // block that computes this iteration or, for LTS, the ghost-line step of a
 // side of a block that waits (tile kLineTile + side).
-// Seven waves per SIMD (at most 72 VGPRs, no spills) rather than the six the
-// register count would give: an LTS iteration has about one item per wave
-// slot, so more waves in flight hide more latency (measured 2-4% faster; the
-// single-block kernel, two items per slot, gains nothing from it).  The
-// learned-middle-state solver keeps its own register budget (its network
-// call needs far more).
+// Six waves per SIMD (at most 80 VGPRs, no spills) with the early column
+// loads; at seven without them the LTS tiling ran 5% and the ensemble 4%
+// slower.  The deferring sweep runs at six without the early loads (77
+// registers; at seven it spilled), the learned-middle-state solver at its
+// own register budget (its network call needs far more).
 template <int S, int XS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == SOLVER_AUGRIE_MLP ? 1 : 7, 8))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == SOLVER_AUGRIE_MLP ? 1 : 6, 8))) void
 batch_fused_kernel(BatchArena A, const BatchBlockDesc* D, BatchBlockState* st, BatchGlobals* G, const int* items,
                    int direct, uint2* deferQ, const GhostJob* jobs, const GhostGeom* geo) {
     if (G->skip) return;

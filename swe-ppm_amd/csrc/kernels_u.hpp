@@ -391,6 +391,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == MOD
     float ch = H[o], chu = HU[o], chv = HV[o], cb = B[o];
     o += v.pitch;
     float nh = H[o], nhu = HU[o], nhv = HV[o], nb = B[o];
+    // HLLE, f-wave and AugRie load column x+1 one column ahead (a software
+    // pipeline: the loads' latency overlaps column x's edges); past x1 + 1
+    // the loads stay on column nx + 1 and are not used.
+    constexpr bool kAhead = S == SOLVER_HLLE || S == SOLVER_FWAVE || S == SOLVER_AUGRIE;
+    float ah = 0.f, ahu = 0.f, ahv = 0.f, ab = 0.f;
+    if constexpr (kAhead) {
+        o += v.pitch;  // column x0 + 1 <= nx + 1
+        ah = H[o];
+        ahu = HU[o];
+        ahv = HV[o];
+        ab = B[o];
+    }
 
     // In MODE_UPDATE the new buffer must end up a complete state: ghost cells
     // not rewritten by the fused boundary conditions are carried over.  Each
@@ -420,11 +432,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == MOD
         chv = nhv;
         cb = nb;
         cq = nq;
-        o += v.pitch;  // column x+1 (<= nx+1)
-        nh = H[o];
-        nhu = HU[o];
-        nhv = HV[o];
-        nb = B[o];
+        if constexpr (kAhead) {
+            nh = ah;
+            nhu = ahu;
+            nhv = ahv;
+            nb = ab;
+            if (x + 2 <= v.nx + 1) o += v.pitch;  // column x+2, else stay on nx+1
+            ah = H[o];
+            ahu = HU[o];
+            ahv = HV[o];
+            ab = B[o];
+        } else {
+            o += v.pitch;  // column x+1 (<= nx+1)
+            nh = H[o];
+            nhu = HU[o];
+            nhv = HV[o];
+            nb = B[o];
+        }
         nq = cell_q(nh, nhu, nhv, nb);
 
         // vertical edge (x | x+1)
