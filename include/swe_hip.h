@@ -143,6 +143,43 @@ int swe_hip_init_grid_scenario(swe_hip_block* blk, const swe_hip_grid* bathymetr
  * initScenario for these scenarios. */
 int swe_hip_init_scenario(swe_hip_block* blk, int scenario);
 
+/* The seeded benchmark workloads (swe_ppm_amd/scenarios.py) as parameter
+ * sets, for swe_hip_init_seeded.  With (x, y) a cell centre as above and
+ * every operation in single precision, each exponential evaluated in double
+ * and rounded once to single:
+ *   SWE_SEEDED_DAM:    b = sum over k < nbumps of
+ *                        bump_a[k] * exp(-((x - bump_x[k])^2 + (y - bump_y[k])^2) / bump_s2[k])
+ *                      (summed in k order from 0), eta = eta_in where the
+ *                      distance to (dam_x, dam_y) is below dam_radius, eta_out
+ *                      elsewhere, h = max(0, eta - b);
+ *   SWE_SEEDED_TRENCH: depth = 10 + 3990 * exp(-(x - trench_x)^2 / trench_w2),
+ *                      b = -depth, h = depth + 2 * exp(-((x - hump_x)^2 +
+ *                      (y - hump_y)^2) / hump_r2);
+ * hu = hv = 0. */
+#define SWE_SEEDED_DAM 0
+#define SWE_SEEDED_TRENCH 1
+#define SWE_SEEDED_MAX_BUMPS 32
+typedef struct swe_hip_seeded {
+    int kind;
+    int nbumps;
+    float dam_x, dam_y, dam_radius, eta_in, eta_out;
+    float trench_x, trench_w2, hump_x, hump_y, hump_r2;
+    float bump_x[SWE_SEEDED_MAX_BUMPS];
+    float bump_y[SWE_SEEDED_MAX_BUMPS];
+    float bump_a[SWE_SEEDED_MAX_BUMPS];
+    float bump_s2[SWE_SEEDED_MAX_BUMPS];
+} swe_hip_seeded;
+
+/* initScenario (SWE_Block.hh:650-682) of a seeded workload, sampled on the
+ * device at every cell, ghosts included, then applyBoundaryConditions and
+ * applyBoundaryBathymetry with the block's boundary types (set them first).
+ * The result equals swe_ppm_amd.scenarios' host sampling of the same
+ * parameters bit for bit.  SWE_HIP_EINVAL for an unknown kind or more than
+ * SWE_SEEDED_MAX_BUMPS bumps.  No counterpart entry in the reference: it
+ * replaces the host loop of initScenario for these workloads, whose host
+ * sampling takes seconds at the largest sizes. */
+int swe_hip_init_seeded(swe_hip_block* blk, const swe_hip_seeded* workload);
+
 /* computeNumericalFluxes (flavour U: SWE_DimensionalSplittingMpi.cpp:383-456;
  * flavour S: SWE_DimensionalSplitting.cpp:105-179).  Solves every edge,
  * keeps the net updates on the device and returns the block's CFL time step

@@ -677,6 +677,42 @@ __global__ void analytic_scenario_kernel(StateView v, float ox, float oy, int sc
     const_cast<float*>(v.b)[c] = b;
 }
 
+// exp of a single-precision argument, evaluated in double and rounded once
+__device__ __forceinline__ float exp_rounded(float q) { return (float)exp((double)q); }
+
+// The seeded workloads (swe_hip_seeded): the single-precision operations of
+// swe_ppm_amd.scenarios.sample_seeded in the same order.
+__global__ void seeded_scenario_kernel(StateView v, float ox, float oy, swe_hip_seeded p) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)(v.nx + 2) * v.pitch;
+    if (c >= total) return;
+    const int i = (int)(c / v.pitch), j = (int)(c % v.pitch);
+    const float x = (float)((double)ox + ((double)i - 0.5) * (double)v.dx);
+    const float y = (float)((double)oy + ((double)j - 0.5) * (double)v.dy);
+    float b, h;
+    if (p.kind == SWE_SEEDED_TRENCH) {
+        const float ax = x - p.trench_x;
+        const float depth = 10.f + 3990.f * exp_rounded(-(ax * ax) / p.trench_w2);
+        const float hx = x - p.hump_x, hy = y - p.hump_y;
+        const float eta = 2.f * exp_rounded(-(hx * hx + hy * hy) / p.hump_r2);
+        b = -depth;
+        h = depth + eta;
+    } else {
+        b = 0.f;
+        for (int k = 0; k < p.nbumps; ++k) {
+            const float bx = x - p.bump_x[k], by = y - p.bump_y[k];
+            b = b + p.bump_a[k] * exp_rounded(-(bx * bx + by * by) / p.bump_s2[k]);
+        }
+        const float eta = scen_radius(x, y, p.dam_x, p.dam_y) < p.dam_radius ? p.eta_in : p.eta_out;
+        const float d = eta - b;
+        h = d > 0.f ? d : 0.f;
+    }
+    v.h[c] = h;
+    v.hu[c] = 0.f;
+    v.hv[c] = 0.f;
+    const_cast<float*>(v.b)[c] = b;
+}
+
 bool grid_ok(const swe_hip_grid* g) {
     return g && g->values && g->nx >= 1 && g->ny >= 1 && g->dx > 0.0 && g->dy > 0.0;
 }
@@ -913,6 +949,27 @@ int swe_hip_init_scenario(swe_hip_block* k, int scenario) {
     const long total = (long)field_elems(k);
     hipLaunchKernelGGL(analytic_scenario_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, k->stream,
                        view(k, k->cur), k->ox, k->oy, scenario);
+    HIP_TRY(hipGetLastError());
+    // applyBoundaryConditions, then applyBoundaryBathymetry (SWE_Block.hh:679-681)
+    if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;
+    if (int r = launch_apply_bc_bathymetry(k->b, k->nx, k->ny, k->pitch, k->bi, k->stream)) return r;
+    HIP_TRY(hipStreamSynchronize(k->stream));
+    return SWE_HIP_OK;
+}
+
+int swe_hip_init_seeded(swe_hip_block* k, const swe_hip_seeded* workload) {
+    if (int r = check(k)) return r;
+    std::lock_guard<std::recursive_mutex> lock(k->mu);
+    if (!workload || (workload->kind != SWE_SEEDED_DAM && workload->kind != SWE_SEEDED_TRENCH) ||
+        workload->nbumps < 0 || workload->nbumps > SWE_SEEDED_MAX_BUMPS) {
+        set_error("swe_hip_init_seeded: invalid workload (kind %d, %d bumps)", workload ? workload->kind : -1,
+                  workload ? workload->nbumps : -1);
+        return SWE_HIP_EINVAL;
+    }
+    discard_per_call(k);  // the state is replaced
+    const long total = (long)field_elems(k);
+    hipLaunchKernelGGL(seeded_scenario_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, k->stream,
+                       view(k, k->cur), k->ox, k->oy, *workload);
     HIP_TRY(hipGetLastError());
     // applyBoundaryConditions, then applyBoundaryBathymetry (SWE_Block.hh:679-681)
     if (int r = launch_apply_bc(view(k, k->cur), k->bi, k->stream)) return r;

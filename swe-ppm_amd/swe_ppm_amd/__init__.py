@@ -43,6 +43,18 @@ class _Grid(C.Structure):
                 ("dy", C.c_double), ("values", C.POINTER(C.c_float))]
 
 
+_MAX_BUMPS = 32  # SWE_SEEDED_MAX_BUMPS
+
+
+class _Seeded(C.Structure):
+    """swe_hip_seeded (include/swe_hip.h)"""
+    _fields_ = [("kind", C.c_int), ("nbumps", C.c_int), ("dam_x", C.c_float), ("dam_y", C.c_float),
+                ("dam_radius", C.c_float), ("eta_in", C.c_float), ("eta_out", C.c_float), ("trench_x", C.c_float),
+                ("trench_w2", C.c_float), ("hump_x", C.c_float), ("hump_y", C.c_float), ("hump_r2", C.c_float),
+                ("bump_x", C.c_float * _MAX_BUMPS), ("bump_y", C.c_float * _MAX_BUMPS),
+                ("bump_a", C.c_float * _MAX_BUMPS), ("bump_s2", C.c_float * _MAX_BUMPS)]
+
+
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise SweHipError(
@@ -65,6 +77,7 @@ def _load() -> C.CDLL:
         "swe_hip_apply_bc_bathymetry": (C.c_int, [vp]),
         "swe_hip_init_grid_scenario": (C.c_int, [vp, gp, gp]),
         "swe_hip_init_scenario": (C.c_int, [vp, C.c_int]),
+        "swe_hip_init_seeded": (C.c_int, [vp, C.POINTER(_Seeded)]),
         "swe_hip_fluxes": (C.c_int, [vp, fp]),
         "swe_hip_update": (C.c_int, [vp, C.c_float]),
         "swe_hip_run": (C.c_int, [vp, C.c_int, fp, fp]),
@@ -202,6 +215,24 @@ class SweBlock:
         (swe_hip_init_scenario).  Set the boundaries first; they are applied
         afterwards."""
         _check(lib.swe_hip_init_scenario(self._h, int(scenario)), "init_scenario")
+
+    def init_seeded(self, spec) -> None:
+        """A seeded workload's parameter set (swe_ppm_amd.scenarios.*_spec)
+        sampled on the device (swe_hip_init_seeded), bit for bit as
+        scenarios.sample_seeded + finish sample it on the host.  Set the
+        boundaries first; they are applied afterwards."""
+        w = _Seeded()
+        w.kind = int(spec["kind"])
+        for f in ("dam_x", "dam_y", "dam_radius", "eta_in", "eta_out", "trench_x", "trench_w2", "hump_x", "hump_y",
+                  "hump_r2"):
+            setattr(w, f, float(spec.get(f, 0.0)))
+        bumps = spec.get("bumps", [])
+        if len(bumps) > _MAX_BUMPS:
+            raise ValueError(f"at most {_MAX_BUMPS} bumps")
+        w.nbumps = len(bumps)
+        for k, (bx, by, a, s2) in enumerate(bumps):
+            w.bump_x[k], w.bump_y[k], w.bump_a[k], w.bump_s2[k] = bx, by, a, s2
+        _check(lib.swe_hip_init_seeded(self._h, C.byref(w)), "init_seeded")
 
     def download(self, with_b: bool = False):
         out = [np.empty(self.shape, np.float32) for _ in range(4 if with_b else 3)]

@@ -16,6 +16,11 @@ Workloads:
   * lts_tiling             28 blocks of uneven size tiling 2048^2, per-block
                            still-water depth
   * trench_hump            4096^2 trench (10 m .. 4000 m) with a 2 m hump
+
+The seeded workloads are parameter sets (`*_spec`, a dict per block) that
+sample_seeded evaluates at the cell centres; swe_hip_init_seeded evaluates
+the same parameters on the device with the same single-precision operations
+(each exponential in double, rounded once), so both give the same bits.
 """
 from __future__ import annotations
 
@@ -110,19 +115,58 @@ def dam_break_c1(n: int = 256):
     """256^2 f-wave workload: h = 15 within 100 m of (500, 500), 10 elsewhere,
     flat b = 0, outflow boundaries, domain [0,1000]^2."""
     dx = np.float32(1000.0 / n)
-    X, Y = cell_centres(n, n, dx, dx)
-    h = np.where(_radius(X, Y, 500.0, 500.0) < np.float32(100.0), np.float32(15.0), np.float32(10.0))
-    b = np.zeros(h.shape, np.float32)
+    h, b, _ = sample_seeded(dam_break_c1_spec(), n, n, dx, dx)
     bnd = [OUTFLOW] * 4
     return (*finish(h, b, bnd), float(dx), float(dx), bnd)
 
 
-def gaussian_bumps(X, Y, centres, amps, sigmas):
+def dam_break_c1_spec():
+    return dam_spec(500.0, 500.0, 100.0, 15.0, 10.0)
+
+
+SEEDED_DAM, SEEDED_TRENCH = 0, 1  # include/swe_hip.h
+SEEDED_MAX_BUMPS = 32
+
+
+def _f32(v) -> float:
+    return float(np.float32(v))
+
+
+def _exp32(q):
+    """exp of a float32 array, evaluated in double and rounded once."""
+    return np.exp(np.asarray(q, np.float32).astype(np.float64)).astype(np.float32)
+
+
+def dam_spec(dam_x, dam_y, dam_radius, eta_in, eta_out, centres=(), amps=(), sigmas=()):
+    """A dam (free surface eta_in within dam_radius of (dam_x, dam_y), eta_out
+    elsewhere) over Gaussian bumps A exp(-r^2 / (2 sigma^2)); h = max(0, eta - b)."""
+    bumps = [(_f32(cx), _f32(cy), _f32(a), _f32(2.0 * s * s)) for (cx, cy), a, s in zip(centres, amps, sigmas)]
+    if len(bumps) > SEEDED_MAX_BUMPS:
+        raise ValueError(f"at most {SEEDED_MAX_BUMPS} bumps")
+    return dict(kind=SEEDED_DAM, dam_x=_f32(dam_x), dam_y=_f32(dam_y), dam_radius=_f32(dam_radius),
+                eta_in=_f32(eta_in), eta_out=_f32(eta_out), bumps=bumps)
+
+
+def sample_seeded(spec, nx: int, ny: int, dx: float, dy: float, ox: float = 0.0, oy: float = 0.0):
+    """h, b (and, for a dam, the free surface eta) of a seeded workload at
+    every cell centre, ghosts included, before boundary conditions."""
+    X, Y = cell_centres(nx, ny, dx, dy, ox, oy)
+    if spec["kind"] == SEEDED_TRENCH:
+        ax = X - np.float32(spec["trench_x"])
+        depth = np.float32(10.0) + np.float32(3990.0) * _exp32(-(ax * ax) / np.float32(spec["trench_w2"]))
+        hx = X - np.float32(spec["hump_x"])
+        hy = Y - np.float32(spec["hump_y"])
+        eta = np.float32(2.0) * _exp32(-(hx * hx + hy * hy) / np.float32(spec["hump_r2"]))
+        return (depth + eta).astype(np.float32), (-depth).astype(np.float32), None
     b = np.zeros(X.shape, np.float32)
-    for (cx, cy), a, s in zip(centres, amps, sigmas):
-        r2 = (X - np.float32(cx)) ** 2 + (Y - np.float32(cy)) ** 2
-        b += (np.float32(a) * np.exp(-r2 / np.float32(2.0 * s * s))).astype(np.float32)
-    return b
+    for bx, by, a, s2 in spec["bumps"]:
+        ex = X - np.float32(bx)
+        ey = Y - np.float32(by)
+        b = (b + np.float32(a) * _exp32(-(ex * ex + ey * ey) / np.float32(s2))).astype(np.float32)
+    eta = np.where(_radius(X, Y, spec["dam_x"], spec["dam_y"]) < np.float32(spec["dam_radius"]),
+                   np.float32(spec["eta_in"]), np.float32(spec["eta_out"]))
+    h = np.maximum(np.float32(0.0), eta - b).astype(np.float32)
+    return h, b, eta
 
 
 def island_bumps(seed: int, length: float = 1000.0, nbumps: int = 16, n_islands: int = 3,
@@ -153,10 +197,7 @@ def dam_break_islands(n: int = 2048, seed: int = 2048, length: float = 1000.0, s
     uniform noise of that amplitude (m) to eta everywhere, so that no edge
     starts at rest (the fully disturbed form of the workload)."""
     dx = np.float32(length / n)
-    X, Y = cell_centres(n, n, dx, dx)
-    centres, amps, sigmas = island_bumps(seed, length)
-    b = gaussian_bumps(X, Y, centres, amps, sigmas)
-    eta = np.where(_radius(X, Y, length / 2, length / 2) < np.float32(100.0), np.float32(15.0), np.float32(10.0))
+    h, b, eta = sample_seeded(dam_break_islands_spec(seed, length), n, n, dx, dx)
     if surface_noise > 0.0:
         rng = np.random.default_rng(seed + 1)
         eta = (eta + rng.uniform(-surface_noise, surface_noise, eta.shape).astype(np.float32)).astype(np.float32)
@@ -165,11 +206,16 @@ def dam_break_islands(n: int = 2048, seed: int = 2048, length: float = 1000.0, s
     return (*finish(h, b, bnd), float(dx), float(dx), bnd)
 
 
+def dam_break_islands_spec(seed: int = 2048, length: float = 1000.0):
+    centres, amps, sigmas = island_bumps(seed, length)
+    return dam_spec(length / 2, length / 2, 100.0, 15.0, 10.0, centres, amps, sigmas)
+
+
 def ensemble_instances(count: int = 512, seed: int = 512, nmin: int = 64, nmax: int = 384, length: float = 1000.0):
     """Independent dam-break instances of uneven size: NX, NY ~ U{nmin..nmax},
     dam radius ~ U(50,200) m, inner/outer depth ~ U(5,20) / U(1,10) m, four
     random bumps (A ~ U(0.5,4) m, sigma ~ U(20,80) m).  Returns a list of
-    dicts with nx, ny, dx, dy, bnd, h, hu, hv, b."""
+    dicts with nx, ny, dx, dy, bnd, h, hu, hv, b and the parameter set (spec)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -180,14 +226,11 @@ def ensemble_instances(count: int = 512, seed: int = 512, nmin: int = 64, nmax: 
         amps = rng.uniform(0.5, 4.0, size=4)
         sigmas = rng.uniform(20.0, 80.0, size=4)
         dx, dy = np.float32(length / nx), np.float32(length / ny)
-        X, Y = cell_centres(nx, ny, dx, dy)
-        b = gaussian_bumps(X, Y, centres, amps, sigmas)
-        eta = np.where(_radius(X, Y, length / 2, length / 2) < np.float32(radius), np.float32(h_in),
-                       np.float32(h_out))
-        h = np.maximum(np.float32(0.0), eta - b).astype(np.float32)
+        spec = dam_spec(length / 2, length / 2, radius, h_in, h_out, centres, amps, sigmas)
+        h, b, _ = sample_seeded(spec, nx, ny, dx, dy)
         bnd = [OUTFLOW] * 4
         hh, hu, hv, bb = finish(h, b, bnd)
-        out.append(dict(nx=nx, ny=ny, dx=float(dx), dy=float(dy), bnd=bnd, h=hh, hu=hu, hv=hv, b=bb))
+        out.append(dict(nx=nx, ny=ny, dx=float(dx), dy=float(dy), bnd=bnd, h=hh, hu=hu, hv=hv, b=bb, spec=spec))
     return out
 
 
@@ -226,10 +269,10 @@ def lts_tiling(seed: int = 28, n: int = 2048, cols: int = 4, rows: int = 7, leng
             nbr = [k - rows if c > 0 else -1, k + rows if c < cols - 1 else -1, k - 1 if r > 0 else -1,
                    k + 1 if r < rows - 1 else -1]
             bnd = [4 if m >= 0 else OUTFLOW for m in nbr]  # CONNECT_WITHIN_RANK = 4
-            h = np.full((nx + 2, ny + 2), np.float32(depths[k]), np.float32)
-            b = np.zeros_like(h)
+            spec = dam_spec(0.0, 0.0, 0.0, depths[k], depths[k])  # still water at the block's depth, b = 0
+            h, b, _ = sample_seeded(spec, nx, ny, dx, dx, ox, oy)
             hh, hu, hv, bb = finish(h, b, bnd)
-            blocks.append(dict(nx=nx, ny=ny, ox=ox, oy=oy, bnd=bnd, nbr=nbr, h=hh, hu=hu, hv=hv, b=bb))
+            blocks.append(dict(nx=nx, ny=ny, ox=ox, oy=oy, bnd=bnd, nbr=nbr, h=hh, hu=hu, hv=hv, b=bb, spec=spec))
     return blocks, float(dx), float(dx)
 
 
@@ -251,21 +294,20 @@ def trench_hump(n: int = 4096, seed: int = 4096, length: float = 100000.0):
     """Trench: still-water depth 10 m far away, 4000 m on a seeded axis, with
     a Gaussian cross-section; a 2 m Gaussian sea-surface hump at a seeded
     position; sea level 0, b = -depth, outflow boundaries."""
-    rng = np.random.default_rng(seed)
     dx = np.float32(length / n)
-    X, Y = cell_centres(n, n, dx, dx)
+    h, b, _ = sample_seeded(trench_hump_spec(seed, length), n, n, dx, dx)
+    bnd = [OUTFLOW] * 4
+    return (*finish(h, b, bnd), float(dx), float(dx), bnd)
+
+
+def trench_hump_spec(seed: int = 4096, length: float = 100000.0):
+    rng = np.random.default_rng(seed)
     axis_x = rng.uniform(0.35, 0.65) * length
     width = rng.uniform(0.04, 0.08) * length
     hx, hy = rng.uniform(0.3, 0.7, size=2) * length
     hr = rng.uniform(0.01, 0.03) * length
-    depth = np.float32(10.0) + np.float32(3990.0) * np.exp(-((X - np.float32(axis_x)) ** 2) /
-                                                           np.float32(2.0 * width * width))
-    eta = np.float32(2.0) * np.exp(-((X - np.float32(hx)) ** 2 + (Y - np.float32(hy)) ** 2) /
-                                   np.float32(2.0 * hr * hr))
-    b = (-depth).astype(np.float32)
-    h = (depth + eta).astype(np.float32)
-    bnd = [OUTFLOW] * 4
-    return (*finish(h, b, bnd), float(dx), float(dx), bnd)
+    return dict(kind=SEEDED_TRENCH, trench_x=_f32(axis_x), trench_w2=_f32(2.0 * width * width), hump_x=_f32(hx),
+                hump_y=_f32(hy), hump_r2=_f32(2.0 * hr * hr))
 
 
 def interior_mass(h: np.ndarray, dx: float, dy: float) -> float:

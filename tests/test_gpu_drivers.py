@@ -237,6 +237,42 @@ def test_analytic_scenarios_sampled_on_device_equal_host_loop(tmp_path):
             assert same_bits(d[f].view(np.uint32), h[f].view(np.uint32)), ("block", k, f)
 
 
+def test_seeded_workloads_sampled_on_device_equal_host_generator():
+    """swe_hip_init_seeded against swe_ppm_amd.scenarios' host sampling of
+    the same parameter sets, bit for bit (ghosts and boundary conditions
+    included): the 256^2 dam break, the 2048^2 islands, the 4096^2 trench,
+    ensemble instances (dx != dy) and LTS blocks at their own origins with
+    connected sides."""
+    import swe_ppm_amd as swe
+    from swe_ppm_amd import scenarios
+
+    def check(name, nx, ny, dx, dy, bnd, spec, host, ox=0.0, oy=0.0):
+        print("seeded workload", name, nx, ny, flush=True)
+        with swe.SweBlock(nx, ny, dx, dy, ox, oy, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            blk.init_seeded(spec)
+            dev = blk.download(with_b=True)
+        for f, d, e in zip(("h", "hu", "hv", "b"), dev, host):
+            assert same_bits(d.view(np.uint32), np.asarray(e, np.float32).view(np.uint32)), (name, f)
+
+    st = scenarios.dam_break_c1(256)
+    check("c1", 256, 256, st[4], st[5], st[6], scenarios.dam_break_c1_spec(), st[:4])
+    st = scenarios.dam_break_islands(n=2048, seed=2048)
+    check("islands", 2048, 2048, st[4], st[5], st[6], scenarios.dam_break_islands_spec(2048), st[:4])
+    st = scenarios.trench_hump(n=4096, seed=4096)
+    check("trench", 4096, 4096, st[4], st[5], st[6], scenarios.trench_hump_spec(4096), st[:4])
+    for k, i in enumerate(scenarios.ensemble_instances(count=6, seed=512)):
+        check(f"ensemble {k}", i["nx"], i["ny"], i["dx"], i["dy"], i["bnd"], i["spec"],
+              (i["h"], i["hu"], i["hv"], i["b"]))
+    blocks, dx, dy = scenarios.lts_tiling(seed=28, n=512)
+    for k, bl in enumerate(blocks[:6]):
+        check(f"lts block {k}", bl["nx"], bl["ny"], dx, dy, bl["bnd"], bl["spec"],
+              (bl["h"], bl["hu"], bl["hv"], bl["b"]), bl["ox"], bl["oy"])
+    with swe.SweBlock(8, 8, 1.0, 1.0) as blk:
+        with pytest.raises(swe.SweHipError):
+            blk.init_seeded(dict(kind=7))
+
+
 def _summary(out):
     upd = [ln for ln in out.splitlines() if ln.startswith("updates ")][0].split()[1:]
     head = [ln for ln in out.splitlines() if ln.startswith("blocks ")][0].split()
