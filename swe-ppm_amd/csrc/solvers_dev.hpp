@@ -156,9 +156,9 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // with ReuseFirst it takes them instead of recomputing them, so it costs one
 // square root and no division.  The values are the ones the loop computes, so
 // both forms give identical results.  The compiler peels the first iteration
-// for it, a second copy of the loop body; under the 72-register budget of
-// seven waves per SIMD that pays (the single-block sweeps and the batched
-// kernels use it).
+// for it, a second copy of the loop body; measured under the 72-register
+// budget of seven waves per SIMD, that pays (the single-block sweeps and the
+// batched kernels use it).
 // ---------------------------------------------------------------------------
 struct Middle {
     float hm, s1m, s2m;

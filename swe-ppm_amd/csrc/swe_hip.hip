@@ -139,7 +139,7 @@ size_t field_elems(const swe_hip_block* k) { return (size_t)(k->nx + 2) * k->pit
 // still holds at least the solver's threshold of waves (waves per SIMD =
 // ceil(ny/62) * ceil(nx/xs) / (CUs x 4)), else 8:
 //                                      16     12
-//   AugRie (6 waves per SIMD)          12      4
+//   AugRie (5 waves per SIMD)          12      4
 //   learned middle state (deferring)    5      3   lighter, even waves
 //   HLLE, f-wave, Hybrid                2      1   light edges
 // Widths are whole numbers of the quiet record's 4-column tiles, so that
