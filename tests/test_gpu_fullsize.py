@@ -10,6 +10,9 @@
 - the 28-block LTS tiling of 2048^2: one macro step bit-identical to the
   oracle's overdecomposed LTS loop.
 - all 512 ensemble instances: 5 steps bit-identical to 512 oracle runs.
+- the learned middle state (packaged network) on the 2048^2 islands and on
+  128 ensemble instances through the batched engine: within 1.5x the exact
+  path's distance to FP64, in the maximum and in the relative L2 norm.
 """
 import os
 
@@ -119,6 +122,91 @@ def test_trench_4096_learned_middle_state(swe, oracle):
                               np.max(np.abs(ref[f][1:-1, 1:-1]))) for f in range(3))
     print(f"rel Linf vs FP64: exact {err['exact']:.3e}, learned {err['learned']:.3e}", flush=True)
     assert err["learned"] <= 1.5 * err["exact"] + 1e-6
+
+
+def _rel_linf(a, ref):
+    return max(float(np.max(np.abs(a[f][1:-1, 1:-1] - ref[f][1:-1, 1:-1])) / np.max(np.abs(ref[f][1:-1, 1:-1])))
+               for f in range(3))
+
+
+def _rel_l2(a, ref):
+    return max(float(np.sqrt(np.sum((a[f][1:-1, 1:-1].astype(np.float64) - ref[f][1:-1, 1:-1]) ** 2) /
+                             np.sum(np.asarray(ref[f][1:-1, 1:-1], np.float64) ** 2))) for f in range(3))
+
+
+def _differing(a, c):
+    return sum(int(np.count_nonzero(a[f][1:-1, 1:-1] != c[f][1:-1, 1:-1])) for f in range(3))
+
+
+def test_islands_2048_learned_middle_state(swe, oracle):
+    """Config 2 with the packaged network: 100 steps of the 2048^2 islands
+    with the learned middle state attached stay within 1.5x the exact FP32
+    run's distance to the FP64 oracle run of the same scheme."""
+    from swe_ppm_amd import scenarios, surrogate
+
+    st = scenarios.dam_break_islands(n=2048, seed=2048)
+    h, hu, hv, b, dx, dy, bnd = st
+    steps = 100
+    net = surrogate.load_default()
+    out = {}
+    for name, attach in (("exact", False), ("learned", True)):
+        print(f"2048^2 islands, {steps} steps, {name}", flush=True)
+        with swe.SweBlock(2048, 2048, dx, dy, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            if attach:
+                blk.set_surrogate(net)
+            blk.upload(h, hu, hv, b)
+            blk.run(steps, want_dts=False)
+            out[name] = blk.download()
+    net.close()
+    print("2048^2 islands, FP64 oracle", flush=True)
+    ref = oracle.run_u(h, hu, hv, b, dx, dy, bnd, swe.AUGRIE, steps, nthreads=THREADS, dtype=np.float64)
+    linf = {k: _rel_linf(v, ref) for k, v in out.items()}
+    l2 = {k: _rel_l2(v, ref) for k, v in out.items()}
+    print(f"rel Linf vs FP64: exact {linf['exact']:.3e}, learned {linf['learned']:.3e}; rel L2: exact "
+          f"{l2['exact']:.3e}, learned {l2['learned']:.3e}; values differing from exact: "
+          f"{_differing(out['learned'], out['exact'])}", flush=True)
+    assert linf["learned"] <= 1.5 * linf["exact"] + 1e-6
+    assert l2["learned"] <= 1.5 * l2["exact"] + 1e-7
+
+
+def test_ensemble_learned_middle_state(swe, oracle):
+    """Config 4 with the packaged network: the first 128 ensemble instances,
+    100 steps through the batched engine with and without the learned middle
+    state; against per-instance FP64 oracle runs the learned form's largest
+    and median errors stay within 1.5x the exact form's."""
+    from swe_ppm_amd import scenarios, surrogate
+
+    inst = scenarios.ensemble_instances(count=512, seed=512)[:128]
+    steps = 100
+    net = surrogate.load_default()
+    out = {}
+    for name, attach in (("exact", False), ("learned", True)):
+        print(f"128-instance ensemble, {steps} steps, {name}", flush=True)
+        with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+            if attach:
+                bt.set_surrogate(net)
+            bt.upload(*[[i[f] for i in inst] for f in ("h", "hu", "hv", "b")])
+            bt.run(steps)
+            out[name] = bt.download()
+    net.close()
+    print("128 instances, FP64 oracle", flush=True)
+    err = {"exact": [], "learned": []}
+    differing = 0
+    for k, i in enumerate(inst):
+        ref = oracle.run_u(i["h"], i["hu"], i["hv"], i["b"], i["dx"], i["dy"], i["bnd"], swe.AUGRIE, steps,
+                           nthreads=THREADS, dtype=np.float64)
+        got = {name: [out[name][f][k] for f in range(3)] for name in out}
+        for name in err:
+            err[name].append(_rel_linf(got[name], ref))
+        differing += _differing(got["learned"], got["exact"])
+    mx = {k: max(v) for k, v in err.items()}
+    med = {k: float(np.median(v)) for k, v in err.items()}
+    print(f"rel Linf vs FP64, max / median over instances: exact {mx['exact']:.3e} / {med['exact']:.3e}, "
+          f"learned {mx['learned']:.3e} / {med['learned']:.3e}; values differing from exact: {differing}",
+          flush=True)
+    assert mx["learned"] <= 1.5 * mx["exact"] + 1e-6
+    assert med["learned"] <= 1.5 * med["exact"] + 1e-7
 
 
 def test_lts_tiling_2048_one_macro_step(swe, oracle):
