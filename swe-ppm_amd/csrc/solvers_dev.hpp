@@ -151,11 +151,12 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // concave on the search interval, so the iterates increase monotonically.
 // cL, cR are sqrt(g hL), sqrt(g hR); invL, invR are 1 / hL, 1 / hR, computed
 // once per cell.
-// ReuseFirst: the first iteration starts at hMax or hMin, where 1/h and one of
-// the two curve square roots are already known (the bracket's own quantities);
-// with ReuseFirst it takes them instead of recomputing them, so it costs one
-// square root and no division.  The values are the ones the loop computes, so
-// both forms give identical results.  The compiler peels the first iteration
+// ReuseFirst: the first iteration starts at hMax or hMin, where 1/h is known
+// and, for two shocks, one of the two curve square roots (the bracket's own
+// quantities); with ReuseFirst it takes them instead of recomputing them, so
+// it costs no division and one square root (two shocks) or two (shock and
+// rarefaction).  The values are the ones the loop computes, so both forms give
+// identical results.  The compiler peels the first iteration
 // for it, a second copy of the loop body; measured under the 72-register
 // budget of seven waves per SIMD, that pays (the single-block sweeps and the
 // batched kernels use it).
@@ -199,12 +200,13 @@ __device__ __forceinline__ Middle middle_state(float hL, float hR, float uL, flo
     float invH = 0.f, gMin = 0.f, s2 = 0.f;
     if constexpr (ReuseFirst) {
         // at h = hMax the shock curve of hMin is gMaxMin and the new root is
-        // gMax(hMax); at h = hMin the rarefaction root is cMin and the new root
-        // is gMin(hMin)
+        // gMax(hMax); at h = hMin the new root is gMin(hMin) and the
+        // rarefaction root sqrt(g hMin) is taken as the loop takes it (the
+        // cell's celerity cMin = sqrt(g) sqrt(hMin) can differ in the last bit)
         invH = shocks ? invMax : invMin;
         const float r0 = sqrtf(halfG * (invH + invH));
         gMin = shocks ? gMaxMin : r0;
-        s2 = shocks ? r0 : cMin;
+        s2 = shocks ? r0 : sqrtf(kGravity * hMin);
     }
     for (int it = 0; it < kMaxNewton; ++it) {
         if (!ReuseFirst || it > 0) {

@@ -62,6 +62,32 @@ def test_ensemble_matches_independent_oracle_runs(swe, oracle, xs, chunk, graph,
     _assert_blocks_equal(oracle, got, want, inst, f"ensemble xs={xs}")
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+def test_ensemble_random_states_bitwise(swe, oracle, fused):
+    """Instances of uncorrelated random cell states (every Riemann case on
+    many edges, wet/dry included) through the batched sweeps: two steps
+    bit-identical to independent oracle runs."""
+    rng = np.random.default_rng(91)
+    inst = []
+    for _ in range(6):
+        nx, ny = (int(v) for v in rng.integers(70, 260, size=2))
+        h = rng.uniform(0.2, 20.0, (nx + 2, ny + 2)).astype(np.float32)
+        h[rng.random(h.shape) < 0.03] = 0.0
+        hu = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+        hv = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+        b = rng.uniform(-2.0, 2.0, h.shape).astype(np.float32)
+        inst.append(dict(nx=nx, ny=ny, dx=1.0, dy=1.0, bnd=[0, 0, 0, 0], h=h, hu=hu, hv=hv, b=b))
+    nsteps = 2
+    with swe.SweBatch(inst, solver=swe.AUGRIE, mode=swe.BATCH_ENSEMBLE) as bt:
+        bt.set_tuning(xs=8, groups=-1, chunk=16, graph=1, fused=fused)
+        bt.upload([i["h"] for i in inst], [i["hu"] for i in inst], [i["hv"] for i in inst], [i["b"] for i in inst])
+        bt.run(nsteps)
+        got = bt.download()
+    want = [oracle.run_u(i["h"], i["hu"], i["hv"], i["b"], i["dx"], i["dy"], i["bnd"], swe.AUGRIE, nsteps)[:3]
+            for i in inst]
+    _assert_blocks_equal(oracle, got, want, inst, f"random-state ensemble fused={fused}")
+
+
 def test_ensemble_runs_continue_across_calls(swe, oracle):
     from swe_ppm_amd import scenarios
 
