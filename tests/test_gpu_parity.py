@@ -24,7 +24,17 @@ def test_solve_edges_bitwise(swe, oracle, solver, name):
         assert same_bits(a, b), f"{name} output {k}: max diff {np.nanmax(np.abs(a - b))}"
 
 
-@pytest.mark.parametrize("solver", [2, 3])
+def _random_states(n, seed):
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(0.2, 20.0, (n + 2, n + 2)).astype(np.float32)
+    h[rng.random(h.shape) < 0.03] = 0.0
+    hu = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+    hv = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+    b = rng.uniform(-2.0, 2.0, h.shape).astype(np.float32)
+    return h, hu, hv, b
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
 @pytest.mark.parametrize("strategy,xseg", [(1, 16), (2, 12)])
 def test_random_states_bitwise(swe, oracle, solver, strategy, xseg):
     """Uncorrelated random cell states, so that every Riemann case occurs on
@@ -32,13 +42,7 @@ def test_random_states_bitwise(swe, oracle, solver, strategy, xseg):
     wet/dry edges): one and two steps bit-identical to the oracle, through the
     sweeps and the fused step, whose Newton loops start from the bracket's
     quantities."""
-    rng = np.random.default_rng(77)
-    n = 768
-    h = rng.uniform(0.2, 20.0, (n + 2, n + 2)).astype(np.float32)
-    h[rng.random(h.shape) < 0.03] = 0.0
-    hu = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
-    hv = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
-    b = rng.uniform(-2.0, 2.0, h.shape).astype(np.float32)
+    h, hu, hv, b = _random_states(768, 77)
     bnd = [0, 0, 0, 0]
     for steps in (1, 2):
         g, _, gdts = _run_gpu(swe, h, hu, hv, b, 1.0, 1.0, bnd, solver, steps, strategy=strategy, xseg=xseg)
@@ -340,6 +344,23 @@ def test_flavour_s_parity(swe, oracle, solver):
         gh, ghu, ghv = blk.download()
     oh, ohu, ohv, odts = oracle.run_s(h, hu, hv, b, dx, bnd, solver, nsteps)
     _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+def test_flavour_s_random_states_bitwise(swe, oracle, solver):
+    """swe_simple's split step on uncorrelated random cell states: two steps
+    bit-identical to the oracle."""
+    h, hu, hv, b = _random_states(384, 78)
+    bnd = [0, 0, 0, 0]
+    with swe.SweBlock(384, 384, 1.0, 1.0, solver=solver, flavour=swe.FLAVOUR_S) as blk:
+        blk.set_boundaries(bnd)
+        blk.upload(h, hu, hv, b)
+        _, gdts = blk.run(2)
+        g = blk.download()
+    oh, ohu, ohv, odts = oracle.run_s(h, hu, hv, b, 1.0, bnd, solver, 2)
+    for name, a, c in zip(("h", "hu", "hv"), g, (oh, ohu, ohv)):
+        assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
+    assert same_bits(gdts, odts)
 
 
 def test_apply_bc_matches_oracle(swe, oracle):
