@@ -150,6 +150,35 @@ def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx
     net.close()
 
 
+def test_deferred_hard_edges_match_the_sweep_on_random_states(swe):
+    """The same agreement on uncorrelated random cell states, where every
+    Riemann case and many hard edges occur: two steps, the deferring fused
+    loop against the network inside the store sweep, bit for bit."""
+    from swe_ppm_amd import surrogate
+
+    rng = np.random.default_rng(79)
+    n = 320
+    h = rng.uniform(0.2, 20.0, (n + 2, n + 2)).astype(np.float32)
+    h[rng.random(h.shape) < 0.03] = 0.0
+    hu = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+    hv = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+    b = rng.uniform(-2.0, 2.0, h.shape).astype(np.float32)
+    net = surrogate.load_default()
+    out = {}
+    for strategy in (1, 2):
+        with swe.SweBlock(n, n, 1.0, 1.0, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries([swe.OUTFLOW] * 4)
+            blk.set_tuning(strategy, 12)
+            blk.set_surrogate(net)
+            blk.upload(h, hu, hv, b)
+            _, dts = blk.run(2)
+            out[strategy] = (blk.download(), np.asarray(dts))
+    net.close()
+    for name, a, c in zip(("h", "hu", "hv"), out[1][0], out[2][0]):
+        assert same_bits(a.view(np.uint32), c.view(np.uint32)), name
+    assert same_bits(out[1][1], out[2][1])
+
+
 def test_batched_deferred_hard_edges_match_the_sweep_and_single_blocks(swe):
     """The batched engine with a network attached: its fused sweep defers the
     hard edges to a second launch, its store strategy solves them inside the
