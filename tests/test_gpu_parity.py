@@ -35,15 +35,15 @@ def _random_states(n, seed):
 
 
 @pytest.mark.parametrize("solver", [0, 1, 2, 3])
-@pytest.mark.parametrize("strategy,xseg", [(1, 16), (2, 12)])
-def test_random_states_bitwise(swe, oracle, solver, strategy, xseg):
+@pytest.mark.parametrize("strategy,xseg,bnd", [(1, 16, (0, 0, 0, 0)), (2, 12, (0, 0, 0, 0)), (2, 8, (1, 0, 1, 1))])
+def test_random_states_bitwise(swe, oracle, solver, strategy, xseg, bnd):
     """Uncorrelated random cell states, so that every Riemann case occurs on
     many edges (two shocks, shock and rarefaction, two rarefactions, dry and
     wet/dry edges): one and two steps bit-identical to the oracle, through the
     sweeps and the fused step, whose Newton loops start from the bracket's
     quantities."""
     h, hu, hv, b = _random_states(768, 77)
-    bnd = [0, 0, 0, 0]
+    bnd = list(bnd)
     for steps in (1, 2):
         g, _, gdts = _run_gpu(swe, h, hu, hv, b, 1.0, 1.0, bnd, solver, steps, strategy=strategy, xseg=xseg)
         oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, 1.0, 1.0, bnd, solver, steps)
