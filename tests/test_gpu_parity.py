@@ -346,6 +346,44 @@ def test_flavour_s_parity(swe, oracle, solver):
     _assert_parity((gh, ghu, ghv), (oh, ohu, ohv), gdts, odts)
 
 
+def _threshold_states(n, seed):
+    """Random cells crowded at the solvers' branch points: depths about the
+    dry tolerance (0.01 m), the still-water minimum (0.1 m) and the hybrid
+    switch (1 m); zero momentum on many cells; free surfaces from a small set,
+    so that neighbours often share one and still edges (and near-still ones,
+    off by a depth ratio over 2) occur next to moving and dry ones."""
+    rng = np.random.default_rng(seed)
+    shape = (n + 2, n + 2)
+    pick = rng.random(shape)
+    h = np.where(pick < 0.25, rng.uniform(0.0, 0.02, shape),
+                 np.where(pick < 0.45, rng.uniform(0.09, 0.11, shape),
+                          np.where(pick < 0.65, rng.uniform(0.9, 1.1, shape), rng.uniform(0.2, 20.0, shape))))
+    h = h.astype(np.float32)
+    h[rng.random(shape) < 0.1] = 0.0
+    eta = rng.choice(np.array([0.0, 0.5, 2.0], np.float32), size=shape)
+    b = (eta - h).astype(np.float32)
+    still = rng.random(shape) < 0.5
+    hu = np.where(still, 0.0, h * rng.uniform(-2.0, 2.0, shape)).astype(np.float32)
+    hv = np.where(still, 0.0, h * rng.uniform(-2.0, 2.0, shape)).astype(np.float32)
+    return h, hu, hv, b
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+@pytest.mark.parametrize("strategy,xseg", [(1, 16), (2, 12)])
+def test_threshold_states_bitwise(swe, oracle, solver, strategy, xseg):
+    """States crowded at the branch points of the solvers (dry, still, hybrid
+    switch): one and two steps bit-identical to the oracle."""
+    h, hu, hv, b = _threshold_states(512, 80 + solver)
+    bnd = [0, 1, 0, 1]
+    for steps in (1, 2):
+        g, _, gdts = _run_gpu(swe, h, hu, hv, b, 1.0, 1.0, bnd, solver, steps, strategy=strategy, xseg=xseg)
+        oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, 1.0, 1.0, bnd, solver, steps)
+        for name, a, c in zip(("h", "hu", "hv"), g, (oh, ohu, ohv)):
+            nd = int(np.count_nonzero(a[1:-1, 1:-1].view(np.uint32) != c[1:-1, 1:-1].view(np.uint32)))
+            assert nd == 0, f"{steps} step(s): {name} differs in {nd} cells"
+        assert same_bits(gdts, odts)
+
+
 @pytest.mark.parametrize("solver", [0, 1, 2, 3])
 def test_flavour_s_random_states_bitwise(swe, oracle, solver):
     """swe_simple's split step on uncorrelated random cell states: two steps
