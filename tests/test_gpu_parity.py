@@ -688,6 +688,48 @@ def test_quiet_skip_is_bit_identical(swe, oracle, solver, n, steps, xseg):
             assert same_bits(got[1][0][f][1:-1, 1:-1], o[f][1:-1, 1:-1]), f
 
 
+@pytest.mark.parametrize("xseg", [12, 4])
+def test_quiet_skip_random_patches_bit_identical(swe, oracle, xseg):
+    """A lake at rest over random bathymetry (depths 5-9 m, so that every
+    edge of the far field is a still edge and its tiles go quiet, and dry
+    islands) with patches of uncorrelated random states: the patches'
+    disturbances spread into quiet tiles through every Riemann case.
+    Skipping on and off and the oracle: identical states and dts."""
+    n, steps = 320, 80
+    rng = np.random.default_rng(83)
+    shape = (n + 2, n + 2)
+    b = -rng.uniform(5.0, 9.0, shape).astype(np.float32)
+    for _ in range(10):
+        x0, y0 = (int(v) for v in rng.integers(1, n - 8, size=2))
+        b[x0:x0 + 8, y0:y0 + 8] = 1.0
+    h = np.maximum(np.float32(0.0) - b, np.float32(0.0)).astype(np.float32)
+    hu = np.zeros(shape, np.float32)
+    hv = np.zeros(shape, np.float32)
+    for _ in range(6):
+        x0, y0 = (int(v) for v in rng.integers(1, n - 24, size=2))
+        sl = (slice(x0, x0 + 24), slice(y0, y0 + 24))
+        h[sl] = rng.uniform(0.0, 15.0, (24, 24))
+        hu[sl] = h[sl] * rng.uniform(-2.0, 2.0, (24, 24))
+        hv[sl] = h[sl] * rng.uniform(-2.0, 2.0, (24, 24))
+    bnd = [1, 0, 0, 1]
+    got = {}
+    for skip in (1, 0):
+        with swe.SweBlock(n, n, 1.0, 1.0, solver=swe.AUGRIE) as blk:
+            blk.set_boundaries(bnd)
+            blk.set_tuning(2, xseg)
+            blk.set_quiet_skip(bool(skip))
+            blk.upload(h, hu, hv, b)
+            _, d = blk.run(steps)
+            got[skip] = (blk.download(), np.asarray(d))
+    for f in range(3):
+        assert same_bits(got[1][0][f], got[0][0][f]), f
+    assert same_bits(got[1][1], got[0][1])
+    o = oracle.run_u(h, hu, hv, b, 1.0, 1.0, bnd, swe.AUGRIE, steps, nthreads=16)
+    for f in range(3):
+        assert same_bits(got[1][0][f][1:-1, 1:-1], o[f][1:-1, 1:-1]), f
+    assert same_bits(got[1][1], o[3])
+
+
 def test_quiet_skip_learned_middle_state_bit_identical(swe):
     """The learned middle state's deferring fused loop with quiet tiles
     skipped against the same loop computing every tile."""
