@@ -144,6 +144,39 @@ def test_lts_blocks_match_oracle(swe, oracle, T, duration, nmacro, fused, groups
     _assert_blocks_equal(oracle, got, states, blocks, f"LTS T={T}")
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+def test_lts_random_states_match_oracle(swe, oracle, fused):
+    """Local time stepping over the 28-block tiling with uncorrelated random
+    cell states, each block at its own depth scale (so that blocks take
+    different power-of-two sub-steps): one macro step bit-identical to the
+    oracle's LTS driver, ghost copies and interpolations included."""
+    from swe_ppm_amd import scenarios
+
+    blocks, dx, dy = scenarios.lts_tiling(n=256)
+    rng = np.random.default_rng(95)
+    for bl in blocks:
+        shape = (bl["nx"] + 2, bl["ny"] + 2)
+        scale = rng.uniform(0.5, 20.0)
+        h = (scale * rng.uniform(0.5, 1.5, shape)).astype(np.float32)
+        h[rng.random(shape) < 0.02] = 0.0
+        bl["h"] = h
+        bl["hu"] = (h * rng.uniform(-1.0, 1.0, shape)).astype(np.float32)
+        bl["hv"] = (h * rng.uniform(-1.0, 1.0, shape)).astype(np.float32)
+        bl["b"] = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+    with swe.SweBatch(blocks, solver=swe.AUGRIE, mode=swe.BATCH_LTS, dx=dx, dy=dy, max_local=1.0,
+                      duration=1.0) as bt:
+        bt.set_tuning(chunk=5, fused=fused)
+        bt.upload(*[[bl[f] for bl in blocks] for f in ("h", "hu", "hv", "b")])
+        _, its, _ = bt.run(1)
+        got = bt.download()
+        upd, _ = bt.stats()
+    states, oupd, oits, _ = oracle.multiblock_run(blocks, dx, dy, swe.AUGRIE, True, 1.0, 1.0, 1, nthreads=8)
+    assert same_bits(its, oits), (its, oits)
+    assert same_bits(upd, oupd)
+    assert len(set(upd.tolist())) > 1, "blocks must take different numbers of sub-steps"
+    _assert_blocks_equal(oracle, got, states, blocks, f"LTS random states fused={fused}")
+
+
 def test_lts_macro_steps_split_over_calls_and_autotune(swe, oracle):
     """Two runs of 1 and 2 macro steps, with the autotuner in between (it
     restores the state it times on), equal one oracle run of 3."""
