@@ -412,6 +412,29 @@ def test_threshold_states_bitwise(swe, oracle, solver, strategy, xseg):
 
 
 @pytest.mark.parametrize("solver", [0, 1, 2, 3])
+@pytest.mark.parametrize("strategy,xseg", [(1, 16), (2, 12)])
+def test_extreme_states_bitwise(swe, oracle, solver, strategy, xseg):
+    """Depths log-uniform over 1e-5 .. 5000 m (5% dry), velocities up to
+    50 m/s, bed levels of +-100 m, walls on two sides: two steps bit-identical
+    to the oracle, with no NaN."""
+    n = 384
+    rng = np.random.default_rng(200 + solver)
+    shape = (n + 2, n + 2)
+    h = np.exp(rng.uniform(np.log(1e-5), np.log(5000.0), shape)).astype(np.float32)
+    h[rng.random(shape) < 0.05] = 0.0
+    hu = (h * rng.uniform(-50.0, 50.0, shape).astype(np.float32)).astype(np.float32)
+    hv = (h * rng.uniform(-50.0, 50.0, shape).astype(np.float32)).astype(np.float32)
+    b = rng.uniform(-100.0, 100.0, shape).astype(np.float32)
+    bnd = [0, 1, 0, 1]
+    g, _, gdts = _run_gpu(swe, h, hu, hv, b, 10.0, 10.0, bnd, solver, 2, strategy=strategy, xseg=xseg)
+    oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, 10.0, 10.0, bnd, solver, 2)
+    for name, a, c in zip(("h", "hu", "hv"), g, (oh, ohu, ohv)):
+        assert not np.any(np.isnan(a)), name
+        assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), name
+    assert same_bits(gdts, odts)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
 def test_flavour_s_random_states_bitwise(swe, oracle, solver):
     """swe_simple's split step on uncorrelated random cell states: two steps
     bit-identical to the oracle."""
