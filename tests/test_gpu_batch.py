@@ -62,19 +62,26 @@ def test_ensemble_matches_independent_oracle_runs(swe, oracle, xs, chunk, graph,
     _assert_blocks_equal(oracle, got, want, inst, f"ensemble xs={xs}")
 
 
-@pytest.mark.parametrize("fused", [1, 0])
-def test_ensemble_random_states_bitwise(swe, oracle, fused):
+@pytest.mark.parametrize("fused,extreme", [(1, False), (0, False), (1, True)])
+def test_ensemble_random_states_bitwise(swe, oracle, fused, extreme):
     """Instances of uncorrelated random cell states (every Riemann case on
-    many edges, wet/dry included) through the batched sweeps: two steps
+    many edges, wet/dry included; extreme: depths log-uniform over 1e-5 ..
+    5000 m, velocities to 50 m/s) through the batched sweeps: two steps
     bit-identical to independent oracle runs."""
     rng = np.random.default_rng(91)
     inst = []
     for _ in range(6):
         nx, ny = (int(v) for v in rng.integers(70, 260, size=2))
-        h = rng.uniform(0.2, 20.0, (nx + 2, ny + 2)).astype(np.float32)
+        shape = (nx + 2, ny + 2)
+        if extreme:
+            h = np.exp(rng.uniform(np.log(1e-5), np.log(5000.0), shape)).astype(np.float32)
+            vel = 50.0
+        else:
+            h = rng.uniform(0.2, 20.0, shape).astype(np.float32)
+            vel = 3.0
         h[rng.random(h.shape) < 0.03] = 0.0
-        hu = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
-        hv = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
+        hu = (h * rng.uniform(-vel, vel, h.shape)).astype(np.float32)
+        hv = (h * rng.uniform(-vel, vel, h.shape)).astype(np.float32)
         b = rng.uniform(-2.0, 2.0, h.shape).astype(np.float32)
         inst.append(dict(nx=nx, ny=ny, dx=1.0, dy=1.0, bnd=[0, 0, 0, 0], h=h, hu=hu, hv=hv, b=b))
     nsteps = 2
@@ -85,7 +92,7 @@ def test_ensemble_random_states_bitwise(swe, oracle, fused):
         got = bt.download()
     want = [oracle.run_u(i["h"], i["hu"], i["hv"], i["b"], i["dx"], i["dy"], i["bnd"], swe.AUGRIE, nsteps)[:3]
             for i in inst]
-    _assert_blocks_equal(oracle, got, want, inst, f"random-state ensemble fused={fused}")
+    _assert_blocks_equal(oracle, got, want, inst, f"random-state ensemble fused={fused} extreme={extreme}")
 
 
 def test_ensemble_runs_continue_across_calls(swe, oracle):
