@@ -150,19 +150,28 @@ def test_deferred_hard_edges_match_the_learned_middle_state_in_the_sweep(swe, nx
     net.close()
 
 
-def test_deferred_hard_edges_match_the_sweep_on_random_states(swe):
+@pytest.mark.parametrize("extreme", [False, True])
+def test_deferred_hard_edges_match_the_sweep_on_random_states(swe, extreme):
     """The same agreement on uncorrelated random cell states, where every
-    Riemann case and many hard edges occur: two steps, the deferring fused
-    loop against the network inside the store sweep, bit for bit."""
+    Riemann case and many hard edges occur (extreme: depths log-uniform over
+    1e-5 .. 5000 m, velocities to 50 m/s, bed levels of +-100 m): two steps,
+    the deferring fused loop against the network inside the store sweep, bit
+    for bit, with no NaN."""
     from swe_ppm_amd import surrogate
 
     rng = np.random.default_rng(79)
     n = 320
-    h = rng.uniform(0.2, 20.0, (n + 2, n + 2)).astype(np.float32)
+    shape = (n + 2, n + 2)
+    if extreme:
+        h = np.exp(rng.uniform(np.log(1e-5), np.log(5000.0), shape)).astype(np.float32)
+        vel, bed = 50.0, 100.0
+    else:
+        h = rng.uniform(0.2, 20.0, shape).astype(np.float32)
+        vel, bed = 3.0, 2.0
     h[rng.random(h.shape) < 0.03] = 0.0
-    hu = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
-    hv = (h * rng.uniform(-3.0, 3.0, h.shape)).astype(np.float32)
-    b = rng.uniform(-2.0, 2.0, h.shape).astype(np.float32)
+    hu = (h * rng.uniform(-vel, vel, h.shape)).astype(np.float32)
+    hv = (h * rng.uniform(-vel, vel, h.shape)).astype(np.float32)
+    b = rng.uniform(-bed, bed, h.shape).astype(np.float32)
     net = surrogate.load_default()
     out = {}
     for strategy in (1, 2):
@@ -175,6 +184,7 @@ def test_deferred_hard_edges_match_the_sweep_on_random_states(swe):
             out[strategy] = (blk.download(), np.asarray(dts))
     net.close()
     for name, a, c in zip(("h", "hu", "hv"), out[1][0], out[2][0]):
+        assert not np.any(np.isnan(a)), name
         assert same_bits(a.view(np.uint32), c.view(np.uint32)), name
     assert same_bits(out[1][1], out[2][1])
 
