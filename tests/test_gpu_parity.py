@@ -411,6 +411,29 @@ def test_threshold_states_bitwise(swe, oracle, solver, strategy, xseg):
         assert same_bits(gdts, odts)
 
 
+@pytest.mark.parametrize("nx,ny", [(1, 1), (1, 70), (70, 1), (2, 3), (63, 65), (130, 17), (61, 62), (17, 125),
+                                   (5, 200), (200, 5)])
+def test_ragged_random_states_bitwise(swe, oracle, nx, ny):
+    """Ragged sizes with random cell states and a random mix of wall and
+    outflow sides, every solver and strategy: three steps bit-identical to
+    the oracle."""
+    rng = np.random.default_rng(nx * 1000 + ny)
+    shape = (nx + 2, ny + 2)
+    h = rng.uniform(0.2, 20.0, shape).astype(np.float32)
+    h[rng.random(shape) < 0.05] = 0.0
+    hu = (h * rng.uniform(-3.0, 3.0, shape)).astype(np.float32)
+    hv = (h * rng.uniform(-3.0, 3.0, shape)).astype(np.float32)
+    b = rng.uniform(-2.0, 2.0, shape).astype(np.float32)
+    bnd = [int(v) for v in rng.integers(0, 2, size=4)]
+    for solver in (0, 1, 2, 3):
+        for strategy, xseg in ((0, 8), (1, 16), (2, 4), (2, 12)):
+            g, _, gdts = _run_gpu(swe, h, hu, hv, b, 1.0, 1.0, bnd, solver, 3, strategy=strategy, xseg=xseg)
+            oh, ohu, ohv, odts = oracle.run_u(h, hu, hv, b, 1.0, 1.0, bnd, solver, 3)
+            for name, a, c in zip(("h", "hu", "hv"), g, (oh, ohu, ohv)):
+                assert same_bits(a[1:-1, 1:-1], c[1:-1, 1:-1]), (solver, strategy, xseg, bnd, name)
+            assert same_bits(gdts, odts), (solver, strategy, xseg, bnd)
+
+
 @pytest.mark.parametrize("solver", [0, 1, 2, 3])
 @pytest.mark.parametrize("strategy,xseg", [(1, 16), (2, 12)])
 def test_extreme_states_bitwise(swe, oracle, solver, strategy, xseg):
