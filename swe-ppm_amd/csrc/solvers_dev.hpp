@@ -149,8 +149,8 @@ __device__ __forceinline__ EdgeOut fwave(const Side& Lin, const Side& Rin) {
 // phi / dphi is taken with both sides scaled by den, so an iteration divides
 // twice (1/h and the step).  phi is increasing and
 // concave on the search interval, so the iterates increase monotonically.
-// cL, cR are sqrt(g hL), sqrt(g hR); invL, invR are 1 / hL, 1 / hR, computed
-// once per cell.
+// cL, cR are the cells' celerities sqrt(g) sqrt(h); invL, invR are 1 / hL,
+// 1 / hR, computed once per cell.
 // ReuseFirst: the first iteration starts at hMax or hMin, where 1/h is known
 // and, for two shocks, one of the two curve square roots (the bracket's own
 // quantities); with ReuseFirst it takes them instead of recomputing them, so
