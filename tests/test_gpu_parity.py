@@ -281,16 +281,21 @@ def test_snapshots_hold_the_state_of_their_call(swe):
             blk.snapshot_release(slots[4])
 
 
-@pytest.mark.parametrize("bnd_outer", [0, 1])
-def test_connected_blocks_per_call_match_the_oracle(swe, oracle, bnd_outer):
+@pytest.mark.parametrize("bnd_outer,states", [(0, "islands"), (1, "islands"), (1, "random")])
+def test_connected_blocks_per_call_match_the_oracle(swe, oracle, bnd_outer, states):
     """Two blocks sharing an edge, driven per call as the overdecomposed
     driver does (setGhostLayer with the in-rank copy, fluxes, the global dt
     minimum, update): the copy reads the neighbour's copy layer with its
     pending update applied on the fly, so both blocks keep the fused step.
-    Bitwise against the oracle's connected global-dt run."""
+    Bitwise against the oracle's connected global-dt run, on the islands
+    dam break and on uncorrelated random cell states."""
     from swe_ppm_amd import scenarios
 
-    h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=128, seed=12)
+    if states == "islands":
+        h, hu, hv, b, dx, dy, bnd = scenarios.dam_break_islands(n=128, seed=12)
+    else:
+        h, hu, hv, b = _random_states(128, 86)
+        dx = dy = 1.0
     # left block x in [1, 70], right block x in [71, 128]
     cut = 70
     parts = [(0, cut), (cut, 128)]
